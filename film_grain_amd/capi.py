@@ -1,0 +1,202 @@
+"""ctypes binding of libfilmgrain_hip.so (include/film_grain_hip.h).
+
+Loads the in-tree library built by film_grain_amd.build and raises if it is
+missing: there is no fallback path.  Device buffers are torch tensors (HIP
+memory) passed by data_ptr(); streams are torch.cuda streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libfilmgrain_hip.so"
+
+FG_OK = 0
+FG_EINVAL = 1
+FG_ENODEV = 2
+FG_ENOMEM = 3
+FG_ECANCELLED = 4
+FG_EHIP = 5
+FG_ENEIGHBOURHOOD = 6
+
+ERROR_NAMES = {
+    FG_EINVAL: "FG_EINVAL", FG_ENODEV: "FG_ENODEV", FG_ENOMEM: "FG_ENOMEM",
+    FG_ECANCELLED: "FG_ECANCELLED", FG_EHIP: "FG_EHIP", FG_ENEIGHBOURHOOD: "FG_ENEIGHBOURHOOD",
+}
+
+# Every symbol include/film_grain_hip.h declares.
+EXPORTED = [
+    "fg_ctx_create", "fg_ctx_destroy", "fg_last_error", "fg_render_pixelwise",
+    "fg_render_grainwise", "fg_render_pixelwise_device", "fg_render_grainwise_device",
+    "fg_stream_status", "fg_pixelwise_stats_device", "fg_grainwise_stats_device",
+    "fg_set_launch_config", "fg_get_launch_config", "fg_version",
+]
+
+
+class PlaneDesc(C.Structure):
+    _fields_ = [
+        ("in_w", C.c_uint32), ("in_h", C.c_uint32), ("out_w", C.c_uint32), ("out_h", C.c_uint32),
+        ("n_samples", C.c_uint32), ("dist_kind", C.c_uint32), ("seed", C.c_uint64),
+        ("zoom", C.c_float), ("delta", C.c_float), ("rm", C.c_float),
+        ("radius_mean", C.c_float), ("log_mu", C.c_float), ("log_sigma", C.c_float),
+    ]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [
+        ("pix_tile_w", C.c_int), ("pix_tile_h", C.c_int), ("pix_lds_bytes", C.c_int),
+        ("grain_tile_w", C.c_int), ("grain_tile_h", C.c_int), ("grain_lds_bytes", C.c_int),
+        ("grain_grains_per_thread", C.c_int),
+    ]
+
+
+CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class FilmGrainError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -m film_grain_amd.build` "
+            "(the HIP backend has no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    L.fg_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.fg_ctx_create.restype = C.c_int
+    L.fg_ctx_destroy.argtypes = [vp]
+    L.fg_ctx_destroy.restype = None
+    L.fg_last_error.argtypes = [vp]
+    L.fg_last_error.restype = C.c_char_p
+    for name in ("fg_render_pixelwise", "fg_render_grainwise"):
+        f = getattr(L, name)
+        f.argtypes = [vp, C.POINTER(PlaneDesc), fp, fp, fp, CANCEL_FN, vp]
+        f.restype = C.c_int
+    for name in ("fg_render_pixelwise_device", "fg_render_grainwise_device"):
+        f = getattr(L, name)
+        f.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, vp]
+        f.restype = C.c_int
+    L.fg_stream_status.argtypes = [vp, vp]
+    L.fg_stream_status.restype = C.c_int
+    for name in ("fg_pixelwise_stats_device", "fg_grainwise_stats_device"):
+        f = getattr(L, name)
+        f.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, C.POINTER(C.c_uint64), vp]
+        f.restype = C.c_int
+    L.fg_set_launch_config.argtypes = [vp, C.POINTER(LaunchConfig)]
+    L.fg_set_launch_config.restype = C.c_int
+    L.fg_get_launch_config.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchConfig)]
+    L.fg_get_launch_config.restype = C.c_int
+    L.fg_version.argtypes = []
+    L.fg_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def make_desc(in_w, in_h, out_w, out_h, n_samples, dist_kind, seed, zoom, delta, rm, radius_mean,
+              log_mu=0.0, log_sigma=0.0) -> PlaneDesc:
+    d = PlaneDesc()
+    d.in_w, d.in_h, d.out_w, d.out_h = in_w, in_h, out_w, out_h
+    d.n_samples, d.dist_kind, d.seed = n_samples, dist_kind, seed
+    d.zoom, d.delta, d.rm, d.radius_mean = zoom, delta, rm, radius_mean
+    d.log_mu, d.log_sigma = log_mu, log_sigma
+    return d
+
+
+class Context:
+    """Owns one fg_ctx (one HIP device)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.fg_ctx_create(device, C.byref(h))
+        if rc != FG_OK:
+            raise FilmGrainError(rc, "no compatible GPU adapter found" if rc == FG_ENODEV
+                                 else "context creation failed")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._L.fg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        return self._L.fg_last_error(self.handle).decode()
+
+    def _check(self, rc: int):
+        if rc != FG_OK:
+            raise FilmGrainError(rc, self.last_error())
+
+    # ---- host-memory entry points (numpy f32 arrays) ----
+    def render_host(self, algo: str, desc: PlaneDesc, lam, offsets, cancel=None):
+        import numpy as np
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        offs = np.ascontiguousarray(offsets, dtype=np.float32)
+        out = np.empty((desc.out_h, desc.out_w), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        cb = CANCEL_FN(lambda _u: 1 if cancel() else 0) if cancel else CANCEL_FN(0)
+        fn = self._L.fg_render_pixelwise if algo == "pixel" else self._L.fg_render_grainwise
+        rc = fn(self.handle, C.byref(desc), lam.ctypes.data_as(fp), offs.ctypes.data_as(fp),
+                out.ctypes.data_as(fp), cb, None)
+        self._check(rc)
+        return out
+
+    # ---- device-memory entry points (torch tensors on the HIP device) ----
+    def render_device(self, algo: str, desc: PlaneDesc, lam, offsets, out, stream=None):
+        fn = self._L.fg_render_pixelwise_device if algo == "pixel" else self._L.fg_render_grainwise_device
+        rc = fn(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
+                C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
+        self._check(rc)
+
+    def stream_status(self, stream=None):
+        self._check(self._L.fg_stream_status(self.handle, C.c_void_p(_stream_handle(stream))))
+
+    def stats_device(self, algo: str, desc: PlaneDesc, lam, offsets, out, stream=None):
+        fn = self._L.fg_pixelwise_stats_device if algo == "pixel" else self._L.fg_grainwise_stats_device
+        cnt = (C.c_uint64 * 3)()
+        rc = fn(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
+                C.c_void_p(out.data_ptr()), cnt, C.c_void_p(_stream_handle(stream)))
+        self._check(rc)
+        return [int(v) for v in cnt]
+
+    def set_launch_config(self, **kw):
+        cfg = LaunchConfig()
+        for k, v in kw.items():
+            setattr(cfg, k, int(v))
+        self._check(self._L.fg_set_launch_config(self.handle, C.byref(cfg)))
+
+    def reset_launch_config(self):
+        self._check(self._L.fg_set_launch_config(self.handle, None))
+
+    def launch_config(self, desc: PlaneDesc, algo: str) -> dict:
+        cfg = LaunchConfig()
+        self._check(self._L.fg_get_launch_config(self.handle, C.byref(desc), 1 if algo == "grain" else 2,
+                                                 C.byref(cfg)))
+        return {name: getattr(cfg, name) for name, _ in LaunchConfig._fields_}
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream

@@ -1,0 +1,492 @@
+// fg_capi.hip -- C ABI of the gfx950 film-grain backend (include/film_grain_hip.h).
+//
+// Host side of the boundary that replaces the reference's per-plane GPU arm
+// (src/lib.rs:151-165, src/wgpu/mod.rs:336-659): argument checks, a cached
+// per-device context, launch-shape choice, and the synchronous host-memory
+// entry points built on the asynchronous device-memory ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/film_grain_hip.h"
+#include "fg_kernels.h"
+#include "fg_rng.h"
+
+struct fg_ctx {
+    int device = 0;
+    int max_lds = 0;      // per-workgroup LDS limit
+    int num_cus = 0;
+    std::mutex mu;
+    std::string last_error;
+    hipStream_t stream = nullptr;  // stream of the host-memory entry points
+    // cached device buffers of the host-memory entry points
+    float *d_lambda = nullptr;
+    size_t cap_lambda = 0;
+    float *d_offs = nullptr;
+    size_t cap_offs = 0;
+    float *d_out = nullptr;
+    size_t cap_out = 0;
+    // per-context device scratch: status word, stats counters, offsets range
+    int *d_status = nullptr;
+    unsigned long long *d_stats = nullptr;
+    float4 *d_range = nullptr;
+    fg_launch_config cfg{};
+};
+
+namespace {
+
+int set_error(fg_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int hip_error(fg_ctx *ctx, hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        return set_error(ctx, FG_ENOMEM,
+                         std::string(what) +
+                             " ran out of GPU memory. Try lowering the output size or sample count.");
+    }
+    return set_error(ctx, FG_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void free_buffers(fg_ctx *ctx)
+{
+    (void)hipFree(ctx->d_lambda);
+    (void)hipFree(ctx->d_offs);
+    (void)hipFree(ctx->d_out);
+    ctx->d_lambda = ctx->d_offs = ctx->d_out = nullptr;
+    ctx->cap_lambda = ctx->cap_offs = ctx->cap_out = 0;
+}
+
+int ensure(fg_ctx *ctx, float **buf, size_t *cap, size_t n_floats, const char *what)
+{
+    if (*cap >= n_floats) return FG_OK;
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    const size_t bytes = std::max<size_t>(n_floats, 1) * sizeof(float);
+    const hipError_t e = hipMalloc(buf, bytes);
+    if (e != hipSuccess) {
+        // An allocation failure is fatal for the cached state, as the reference
+        // drops its context after an out-of-memory error.
+        free_buffers(ctx);
+        (void)hipGetLastError();
+        return hip_error(ctx, e, what);
+    }
+    *cap = n_floats;
+    return FG_OK;
+}
+
+int validate(fg_ctx *ctx, const fg_plane_desc *d)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d) return set_error(ctx, FG_EINVAL, "null plane description");
+    if (d->in_w == 0 || d->in_h == 0) return set_error(ctx, FG_EINVAL, "input image is empty");
+    if (d->out_w == 0 || d->out_h == 0)
+        return set_error(ctx, FG_EINVAL, "output dimensions must be positive");
+    if (d->in_w > (1u << 24) || d->in_h > (1u << 24) || d->out_w > (1u << 24) ||
+        d->out_h > (1u << 24))
+        return set_error(ctx, FG_EINVAL, "image dimensions exceed 2^24");
+    if (d->n_samples == 0) return set_error(ctx, FG_EINVAL, "n_samples must be >= 1");
+    if (d->dist_kind > 1) return set_error(ctx, FG_EINVAL, "unknown radius distribution");
+    if (!(d->zoom > 0.0f) || !std::isfinite(d->zoom)) return set_error(ctx, FG_EINVAL, "zoom must be > 0");
+    if (!(d->delta > 0.0f) || !std::isfinite(d->delta)) return set_error(ctx, FG_EINVAL, "cell size must be > 0");
+    if (!(d->rm > 0.0f) || !std::isfinite(d->rm)) return set_error(ctx, FG_EINVAL, "max radius must be > 0");
+    if (!std::isfinite(d->radius_mean)) return set_error(ctx, FG_EINVAL, "radius must be finite");
+    if (d->dist_kind == 1 && (!std::isfinite(d->log_mu) || !(d->log_sigma >= 0.0f) ||
+                              !std::isfinite(d->log_sigma)))
+        return set_error(ctx, FG_EINVAL, "invalid log-normal parameters");
+    return FG_OK;
+}
+
+// Largest tile (from a fixed ladder) whose cell neighbourhood index fits in
+// about half of the LDS budget, for an assumed offset spread of +-3.5 px
+// (sigma = 0.8 output px, a few hundred samples).  Tiles whose actual
+// neighbourhood is larger are split inside the kernel, so this is a speed
+// choice, not a correctness one.
+void choose_pixel_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
+{
+    static const int ladder[][2] = {{32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}};
+    const double iz = 1.0 / d->zoom;
+    const double spread = 2.0 * 3.5 * iz;
+    for (const auto &t : ladder) {
+        const double W = ((t[0] - 1) * iz + spread + 2.0 * d->rm) / d->delta + 3.0;
+        const double H = ((t[1] - 1) * iz + spread + 2.0 * d->rm) / d->delta + 3.0;
+        if ((W * H + 1) * 4.0 <= 0.5 * lds) {
+            *tw = t[0];
+            *th = t[1];
+            return;
+        }
+    }
+    *tw = 4;
+    *th = 4;
+}
+
+void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
+{
+    const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : 64 * 1024;
+    *lds = std::min(budget, ctx->max_lds - 6 * 1024);
+    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 &&
+        ctx->cfg.pix_tile_w * ctx->cfg.pix_tile_h <= fg::kMaxTilePixels) {
+        *tw = ctx->cfg.pix_tile_w;
+        *th = ctx->cfg.pix_tile_h;
+    } else {
+        choose_pixel_tile(d, *lds, tw, th);
+    }
+}
+
+void grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, int *gpt)
+{
+    const int words = (int)((d->n_samples + 31) / 32);
+    *gpt = ctx->cfg.grain_grains_per_thread > 0 ? ctx->cfg.grain_grains_per_thread : 4;
+    const int budget = ctx->cfg.grain_lds_bytes > 0 ? ctx->cfg.grain_lds_bytes : 64 * 1024;
+    const int cap = std::min(budget, ctx->max_lds - 2 * 1024);
+    const int slots = fg::kBlock * (*gpt) * 16;
+    if (ctx->cfg.grain_tile_w > 0 && ctx->cfg.grain_tile_h > 0) {
+        *tw = ctx->cfg.grain_tile_w;
+        *th = ctx->cfg.grain_tile_h;
+    } else {
+        static const int ladder[][2] = {{128, 64}, {64, 64}, {64, 32}, {32, 32}, {32, 16},
+                                        {16, 16}, {16, 8},   {8, 8},   {8, 4},   {4, 4},
+                                        {2, 2},   {1, 1}};
+        *tw = 1;
+        *th = 1;
+        for (const auto &t : ladder) {
+            if ((size_t)t[0] * t[1] * words * 4 + slots <= (size_t)cap) {
+                *tw = t[0];
+                *th = t[1];
+                break;
+            }
+        }
+    }
+    const size_t bits = (((size_t)(*tw) * (*th) * words * 4) + 15) & ~(size_t)15;
+    *lds = (int)(bits + slots);
+}
+
+int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
+                 float *d_out, hipStream_t s, bool stats)
+{
+    int tw, th, lds;
+    pixel_config(ctx, d, &tw, &th, &lds);
+    fg::PixelArgs a{};
+    a.lambda = d_lambda;
+    a.offs = reinterpret_cast<const float2 *>(d_offs);
+    a.out = d_out;
+    a.status = ctx->d_status;
+    a.stats = ctx->d_stats;
+    a.tile_list = nullptr;
+    a.seed = d->seed;
+    a.log_mu = (double)d->log_mu;
+    a.log_sigma = (double)d->log_sigma;
+    a.in_w = (int)d->in_w;
+    a.in_h = (int)d->in_h;
+    a.out_w = (int)d->out_w;
+    a.out_h = (int)d->out_h;
+    a.n_samples = (int)d->n_samples;
+    a.tile_w = tw;
+    a.tile_h = th;
+    a.tiles_x = (a.out_w + tw - 1) / tw;
+    const long long ntiles = (long long)a.tiles_x * ((a.out_h + th - 1) / th);
+    if (ntiles > 0x7fffffffLL) return set_error(ctx, FG_EINVAL, "too many tiles");
+    a.n_tiles = (int)ntiles;
+    a.lds_bytes = lds;
+    a.inv_zoom = 1.0f / d->zoom;
+    a.rm = d->rm;
+    a.delta = d->delta;
+    a.uscale = fg::uniform_f32_scale(0.0f, d->delta);
+    float rc = d->radius_mean;  // RadiusProfile::sample for Const, then the rm clamp
+    if (rc > d->rm) rc = d->rm;
+    a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
+    a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
+    a.off_range = ctx->d_range;
+    hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    e = fg::launch_pixelwise(a, (int)d->dist_kind, stats, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    return FG_OK;
+}
+
+int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
+                 float *d_out, hipStream_t s, bool stats)
+{
+    int tw, th, lds, gpt;
+    grain_config(ctx, d, &tw, &th, &lds, &gpt);
+    if (lds > ctx->max_lds)
+        return set_error(ctx, FG_EINVAL, "sample count too large for one grain-wise tile");
+    fg::GrainArgs a{};
+    a.lambda = d_lambda;
+    a.offs = reinterpret_cast<const float2 *>(d_offs);
+    a.out = d_out;
+    a.status = ctx->d_status;
+    a.stats = ctx->d_stats;
+    a.seed = d->seed;
+    a.log_mu = (double)d->log_mu;
+    a.log_sigma = (double)d->log_sigma;
+    a.in_w = (int)d->in_w;
+    a.in_h = (int)d->in_h;
+    a.out_w = (int)d->out_w;
+    a.out_h = (int)d->out_h;
+    a.n_samples = (int)d->n_samples;
+    a.words = (int)((d->n_samples + 31) / 32);
+    a.tile_w = tw;
+    a.tile_h = th;
+    a.tiles_x = (a.out_w + tw - 1) / tw;
+    const long long ntiles = (long long)a.tiles_x * ((a.out_h + th - 1) / th);
+    if (ntiles > 0x7fffffffLL) return set_error(ctx, FG_EINVAL, "too many tiles");
+    a.n_tiles = (int)ntiles;
+    a.grains_per_thread = gpt;
+    a.lds_bytes = lds;
+    a.zoom = d->zoom;
+    a.rm = d->rm;
+    a.r_const = d->radius_mean;
+    a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
+    a.r_out_max = d->rm * d->zoom;
+    a.off_range = ctx->d_range;
+    hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    e = fg::launch_grainwise(a, (int)d->dist_kind, stats, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    return FG_OK;
+}
+
+int read_status(fg_ctx *ctx, hipStream_t s, const char *what)
+{
+    int status = 0;
+    hipError_t e = hipMemcpyAsync(&status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if (status != 0) {
+        e = hipMemsetAsync(ctx->d_status, 0, sizeof(int), s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_error(ctx, e, what);
+    }
+    if (status & fg::kStatusNeighbourhood)
+        return set_error(ctx, FG_ENEIGHBOURHOOD,
+                         std::string(what) +
+                             ": the cell window of one output pixel does not fit in LDS; "
+                             "use a larger cell size");
+    return FG_OK;
+}
+
+bool finite_pairs(const float *xy, uint32_t n)
+{
+    for (uint32_t k = 0; k < 2 * n; ++k)
+        if (!std::isfinite(xy[k])) return false;
+    return true;
+}
+
+using DeviceFn = int (*)(fg_ctx *, const fg_plane_desc *, const float *, const float *, float *,
+                         hipStream_t, bool);
+
+int render_host(fg_ctx *ctx, const fg_plane_desc *d, const float *lambda, const float *offs,
+                float *out, fg_cancel_fn cancel, void *user, DeviceFn fn, const char *what)
+{
+    int rc = validate(ctx, d);
+    if (rc != FG_OK) return rc;
+    if (!lambda || !offs || !out) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (!finite_pairs(offs, d->n_samples))
+        return set_error(ctx, FG_EINVAL, "offsets must be finite");
+    if (cancel && cancel(user)) return set_error(ctx, FG_ECANCELLED, "cancelled");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    const size_t nin = (size_t)d->in_w * d->in_h, nout = (size_t)d->out_w * d->out_h;
+    if ((rc = ensure(ctx, &ctx->d_lambda, &ctx->cap_lambda, nin, what)) != FG_OK) return rc;
+    if ((rc = ensure(ctx, &ctx->d_offs, &ctx->cap_offs, 2 * (size_t)d->n_samples, what)) != FG_OK)
+        return rc;
+    if ((rc = ensure(ctx, &ctx->d_out, &ctx->cap_out, nout, what)) != FG_OK) return rc;
+    hipStream_t s = ctx->stream;
+    e = hipMemcpyAsync(ctx->d_lambda, lambda, nin * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ctx->d_offs, offs, (size_t)d->n_samples * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    rc = fn(ctx, d, ctx->d_lambda, ctx->d_offs, ctx->d_out, s, false);
+    if (rc != FG_OK) return rc;
+    e = hipMemcpyAsync(out, ctx->d_out, nout * 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    return read_status(ctx, s, what);
+}
+
+int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
+                 float *d_out, uint64_t counters[3], void *stream, DeviceFn fn, const char *what)
+{
+    int rc = validate(ctx, d);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offs || !d_out || !counters) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->d_stats, 0, 4 * sizeof(unsigned long long), s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    rc = fn(ctx, d, d_lambda, d_offs, d_out, s, true);
+    if (rc != FG_OK) return rc;
+    unsigned long long h[4] = {0, 0, 0, 0};
+    e = hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    rc = read_status(ctx, s, what);
+    for (int i = 0; i < 3; ++i) counters[i] = h[i];
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fg_ctx_create(int device_ordinal, fg_ctx **out)
+{
+    if (!out) return FG_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return FG_ENODEV;
+    }
+    if (device_ordinal < 0 || device_ordinal >= count) return FG_ENODEV;
+    fg_ctx *ctx = new fg_ctx();
+    ctx->device = device_ordinal;
+    hipError_t e = hipSetDevice(device_ordinal);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device_ordinal);
+    if (e == hipSuccess) {
+        ctx->max_lds = (int)prop.sharedMemPerBlock;
+        ctx->num_cus = prop.multiProcessorCount;
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    }
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_status, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_range, sizeof(float4));
+    if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, sizeof(int));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        fg_ctx_destroy(ctx);
+        return FG_EHIP;
+    }
+    *out = ctx;
+    return FG_OK;
+}
+
+void fg_ctx_destroy(fg_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_buffers(ctx);
+    (void)hipFree(ctx->d_status);
+    (void)hipFree(ctx->d_stats);
+    (void)hipFree(ctx->d_range);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *fg_last_error(const fg_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int fg_render_pixelwise(fg_ctx *ctx, const fg_plane_desc *desc, const float *lambda,
+                        const float *offsets_input_xy, float *out, fg_cancel_fn cancel,
+                        void *cancel_user)
+{
+    return render_host(ctx, desc, lambda, offsets_input_xy, out, cancel, cancel_user, do_pixelwise,
+                       "Pixel renderer");
+}
+
+int fg_render_grainwise(fg_ctx *ctx, const fg_plane_desc *desc, const float *lambda,
+                        const float *offsets_xy, float *out, fg_cancel_fn cancel, void *cancel_user)
+{
+    return render_host(ctx, desc, lambda, offsets_xy, out, cancel, cancel_user, do_grainwise,
+                       "Grain renderer");
+}
+
+int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                               const float *d_offsets_input_xy, float *d_out, void *hip_stream)
+{
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offsets_input_xy || !d_out) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    return do_pixelwise(ctx, desc, d_lambda, d_offsets_input_xy, d_out,
+                        reinterpret_cast<hipStream_t>(hip_stream), false);
+}
+
+int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                               const float *d_offsets_xy, float *d_out, void *hip_stream)
+{
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offsets_xy || !d_out) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    return do_grainwise(ctx, desc, d_lambda, d_offsets_xy, d_out,
+                        reinterpret_cast<hipStream_t>(hip_stream), false);
+}
+
+int fg_stream_status(fg_ctx *ctx, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "status");
+    return read_status(ctx, reinterpret_cast<hipStream_t>(hip_stream), "render");
+}
+
+int fg_pixelwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                              const float *d_offsets_input_xy, float *d_out, uint64_t counters[3],
+                              void *hip_stream)
+{
+    return stats_device(ctx, desc, d_lambda, d_offsets_input_xy, d_out, counters, hip_stream,
+                        do_pixelwise, "Pixel renderer");
+}
+
+int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                              const float *d_offsets_xy, float *d_out, uint64_t counters[3],
+                              void *hip_stream)
+{
+    return stats_device(ctx, desc, d_lambda, d_offsets_xy, d_out, counters, hip_stream,
+                        do_grainwise, "Grain renderer");
+}
+
+int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg)
+{
+    if (!ctx) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (cfg) {
+        if (cfg->pix_tile_w < 0 || cfg->pix_tile_h < 0 ||
+            cfg->pix_tile_w * cfg->pix_tile_h > fg::kMaxTilePixels || cfg->grain_tile_w < 0 ||
+            cfg->grain_tile_h < 0 || cfg->grain_grains_per_thread < 0 ||
+            cfg->grain_grains_per_thread > 64 || cfg->pix_lds_bytes < 0 || cfg->grain_lds_bytes < 0)
+            return set_error(ctx, FG_EINVAL, "invalid launch configuration");
+        ctx->cfg = *cfg;
+    } else {
+        ctx->cfg = fg_launch_config{};
+    }
+    return FG_OK;
+}
+
+int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out)
+{
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!out) return set_error(ctx, FG_EINVAL, "null output");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = fg_launch_config{};
+    if (algo == 1) {
+        grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes,
+                     &out->grain_grains_per_thread);
+    } else {
+        pixel_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes);
+    }
+    return FG_OK;
+}
+
+const char *fg_version(void) { return "film_grain_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

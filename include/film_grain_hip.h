@@ -1,0 +1,139 @@
+/*
+ * film_grain_hip.h -- C ABI of the MI355X (gfx950) film-grain backend.
+ *
+ * This is the drop-in boundary for the reference's per-plane integrator
+ * dispatch (src/lib.rs:151-165): the two CPU integrators
+ *     render_pixelwise(lambda, params, derived, cancel) -> Plane   (src/pixelwise.rs:11-16)
+ *     render_grainwise(lambda, params, derived, cancel) -> Plane   (src/grainwise.rs:12-17)
+ * and the GPU backend they sit beside
+ *     wgpu::context() -> Arc<GpuContext>                           (src/wgpu/mod.rs:84)
+ *     render_pixelwise_gpu(ctx, lambda, params, derived) -> Plane  (src/wgpu/mod.rs:336-341)
+ *     render_grainwise_gpu(ctx, lambda, params, derived) -> Plane  (src/wgpu/mod.rs:473-478)
+ * A Rust FFI crate (or ctypes, cgo, JNI ...) binds these declarations
+ * unchanged: plain pointers and sizes, POD structs, int status codes.
+ *
+ * Results equal the reference's CPU path on the same seed (not its wgpu path,
+ * whose random streams differ): the same cell/pixel hashes, Xoshiro256++
+ * streams, Poisson counts and grain geometry, and the same window and disk
+ * tests in f32 without contraction.
+ *
+ * Data contract (src/wgpu/mod.rs:17-38, 353-358, 490-495):
+ *   lambda   in_h x in_w f32, row-major: the lambda_plane of one colour plane
+ *   offsets  n_samples (x, y) f32 pairs: Derived::offsets_input (input pixels)
+ *            for the pixel-wise path, Derived::offsets (output pixels) for the
+ *            grain-wise path
+ *   out      out_h x out_w f32, written completely
+ */
+#ifndef FILM_GRAIN_HIP_H
+#define FILM_GRAIN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; each maps to a RenderError variant (src/lib.rs:28-44). */
+#define FG_OK 0
+#define FG_EINVAL 1         /* bad argument, e.g. offsets count != N (src/wgpu/mod.rs:353-358) -> Gpu(..) */
+#define FG_ENODEV 2         /* no HIP device ("no compatible GPU adapter found", src/wgpu/mod.rs:101) */
+#define FG_ENOMEM 3         /* device allocation failed (fatal; the context's buffers are dropped,
+                               src/wgpu/mod.rs:727-736) */
+#define FG_ECANCELLED 4     /* the cancel callback returned non-zero -> Cancelled */
+#define FG_EHIP 5           /* a HIP runtime call or launch failed -> Gpu(..) */
+#define FG_ENEIGHBOURHOOD 6 /* one output pixel's cell window does not fit in LDS (cell size
+                               far below the grain radius) -> Gpu(..) */
+
+/* Opaque per-device context (~ GpuContext, src/wgpu/mod.rs:40-49).  All entry
+ * points taking a context are thread-safe (an internal mutex guards the cached
+ * device buffers). */
+typedef struct fg_ctx fg_ctx;
+
+/* One colour plane's render description (~ Uniforms, src/wgpu/mod.rs:17-38,
+ * built like build_uniforms :661-692, but the seed keeps all 64 bits as
+ * Params::seed does on the CPU path). */
+typedef struct {
+    uint32_t in_w, in_h;     /* lambda plane size */
+    uint32_t out_w, out_h;   /* Derived::output_width/height */
+    uint32_t n_samples;      /* Params::n_samples (>= 1) */
+    uint32_t dist_kind;      /* 0 = RadiusDist::Const, 1 = RadiusDist::Lognorm */
+    uint64_t seed;           /* Params::seed */
+    float zoom;              /* Params::zoom */
+    float delta;             /* Derived::delta (pixel-wise cell size) */
+    float rm;                /* Derived::rm (radius clamp) */
+    float radius_mean;       /* Params::radius_mean (the Const radius) */
+    float log_mu;            /* Params::radius_log_mu (f32; widened on device like RadiusProfile::new) */
+    float log_sigma;         /* Params::radius_log_sigma */
+} fg_plane_desc;
+
+/* Cancellation poll (~ the cancel closure, src/lib.rs:114-123); non-zero cancels. */
+typedef int (*fg_cancel_fn)(void *user);
+
+/* Creates the context for HIP device `device_ordinal` (~ wgpu::context() / init,
+ * src/wgpu/mod.rs:84-118).  FG_ENODEV when no device exists. */
+int fg_ctx_create(int device_ordinal, fg_ctx **out);
+void fg_ctx_destroy(fg_ctx *ctx);
+/* Text of the last error on this context (~ RenderError::Gpu(String)). */
+const char *fg_last_error(const fg_ctx *ctx);
+
+/* Pixel-wise render of one plane from host memory; synchronous
+ * (~ render_pixelwise_gpu, src/wgpu/mod.rs:336-471; CPU twin render_pixelwise,
+ * src/pixelwise.rs:11-45).  `cancel` (may be NULL) is polled before the launch,
+ * as the reference's GPU arm does (src/lib.rs:155). */
+int fg_render_pixelwise(fg_ctx *ctx, const fg_plane_desc *desc, const float *lambda,
+                        const float *offsets_input_xy, float *out, fg_cancel_fn cancel,
+                        void *cancel_user);
+
+/* Grain-wise render of one plane from host memory; synchronous
+ * (~ render_grainwise_gpu, src/wgpu/mod.rs:473-659; CPU twin render_grainwise,
+ * src/grainwise.rs:12-124). */
+int fg_render_grainwise(fg_ctx *ctx, const fg_plane_desc *desc, const float *lambda,
+                        const float *offsets_xy, float *out, fg_cancel_fn cancel,
+                        void *cancel_user);
+
+/* Device-resident variants: all three arrays are device pointers on the
+ * context's device; the work is enqueued on `hip_stream` (a hipStream_t, NULL
+ * for the default stream) and the call returns without waiting.  Errors found
+ * on the device are reported by fg_stream_status.  No counterpart in the
+ * reference (its GPU arm always copies through host memory); these are the
+ * entry points for callers whose planes already live in HBM. */
+int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                               const float *d_offsets_input_xy, float *d_out, void *hip_stream);
+int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                               const float *d_offsets_xy, float *d_out, void *hip_stream);
+
+/* Waits for `hip_stream` and returns the device-side status of the work
+ * enqueued by the device-resident entry points since the previous call. */
+int fg_stream_status(fg_ctx *ctx, void *hip_stream);
+
+/* Instrumented pixel-wise render (device pointers, synchronous): also returns
+ * counters[0] = (pixel, sample) points, counters[1] = disk tests the reference
+ * performs in its own loop order with its early exit (the T of the FP32 FLOP
+ * count 16 + 6T), counters[2] = covered points.  No reference counterpart. */
+int fg_pixelwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                              const float *d_offsets_input_xy, float *d_out, uint64_t counters[3],
+                              void *hip_stream);
+
+/* Instrumented grain-wise render: counters[0] = grains, counters[1] = (grain,
+ * sample) pairs tested, counters[2] = bits set.  No reference counterpart. */
+int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                              const float *d_offsets_xy, float *d_out, uint64_t counters[3],
+                              void *hip_stream);
+
+/* Launch shape used by the next renders on this context; 0 keeps the
+ * automatic choice.  No reference counterpart (the autotuner sets these). */
+typedef struct {
+    int pix_tile_w, pix_tile_h, pix_lds_bytes;
+    int grain_tile_w, grain_tile_h, grain_lds_bytes, grain_grains_per_thread;
+} fg_launch_config;
+int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg);
+int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out);
+
+/* Library version string. */
+const char *fg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,72 @@
+"""Shared builders for parity cases: a seeded intensity plane, the oracle's
+derived parameters, offsets and lambda plane, and the matching C-ABI plane
+description.  The workloads mirror the benchmark configurations at sizes the
+CPU oracle finishes in seconds."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from film_grain_amd import synth
+
+
+@dataclass
+class Case:
+    name: str
+    u: np.ndarray             # intensity plane (h, w) f32
+    lam: np.ndarray           # lambda plane (oracle lambda_plane)
+    offsets: np.ndarray       # Derived::offsets (output px), (N, 2) f32
+    offsets_input: np.ndarray  # Derived::offsets_input (input px)
+    derived: object           # oracle FgoDerived
+    plane: object             # oracle FgoPlane
+    desc: object              # capi PlaneDesc
+    n_samples: int
+    seed: int
+    zoom: float
+    dist: str
+    radius: float
+
+
+def build_case(O, name, u, radius, stddev=0.0, dist="const", zoom=1.0, sigma=0.8, n_samples=64,
+               seed=42, cell_delta=None, max_radius=("quantile", 0.999)) -> Case:
+    from film_grain_amd import capi
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    h, w = u.shape
+    d = O.derive(radius, stddev, dist, zoom, sigma, n_samples, max_radius=max_radius,
+                 cell_delta=cell_delta, seed=seed, in_w=w, in_h=h)
+    norm, _ = O.normalize_plane(u)
+    lam = O.lambda_plane(norm, d.inv_e_pi_r2)
+    offs = O.make_offsets(seed, n_samples, sigma)
+    offs_in = (offs / np.float32(zoom)).astype(np.float32)
+    p = O.plane(d, n_samples, seed, zoom, dist, radius)
+    desc = capi.make_desc(d.in_w, d.in_h, d.out_w, d.out_h, n_samples, 1 if dist == "lognorm" else 0,
+                          seed, zoom, d.delta, d.rm, radius, d.log_mu, d.log_sigma)
+    return Case(name, u, lam, offs, offs_in, d, p, desc, n_samples, seed, zoom, dist, radius)
+
+
+def c1_case(O, size=64, n_samples=64):
+    """Plain pixel-wise: r = 0.12 const, sigma 0.8, zoom 1, seed 42."""
+    return build_case(O, f"c1_{size}", synth.c1_luma(42, size), 0.12, n_samples=n_samples, seed=42)
+
+
+def c2_case(O, w=96, h=64, n_samples=256, channel=0):
+    """Pixel-wise lognormal: mean 0.1 / sd 0.05, sigma 0.8, N=256, zoom 2, seed 7."""
+    u = synth.c2_rgb(7, w, h)[channel]
+    return build_case(O, f"c2_{w}x{h}_c{channel}", u, 0.1, 0.05, "lognorm", zoom=2.0,
+                      n_samples=n_samples, seed=7)
+
+
+def c3_case(O, size=96, n_samples=128):
+    """Grain-wise: r = 0.05 const, ramp + 5% noise, N=128, zoom 1, seed 11."""
+    return build_case(O, f"c3_{size}", synth.c3_ramp(11, size), 0.05, n_samples=n_samples, seed=11)
+
+
+def tiles(rng, out_w, out_h, tile, count):
+    """Random tile origins (x0, y0, x1, y1) inside the output."""
+    out = []
+    for _ in range(count):
+        x0 = int(rng.integers(0, max(1, out_w - tile + 1)))
+        y0 = int(rng.integers(0, max(1, out_h - tile + 1)))
+        out.append((x0, y0, min(x0 + tile, out_w), min(y0 + tile, out_h)))
+    return out
