@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X film-grain hot path.
+
+Default workload (the headline metric): pixel-wise render of a 1920x1080 RGB
+frame at zoom 2 (3840x2160 output), lognormal radii mean 0.1 / sd 0.05, filter
+sigma 0.8, N = 256, seed 7, with the lambda planes already resident in HBM.
+One step renders the three colour planes.  The value is
+Mpix*samples/s = 3840 * 2160 * 3 * 256 * steps / seconds / 1e6.
+
+Also reported on the same JSON line:
+  roofline      FP32 FLOP/s of the pixel-wise kernel against MI355X's FP32 peak;
+                FLOPs per (pixel, sample) = 16 + 6*T, T = disk tests in the
+                reference's loop order with its early exit, counted by the
+                kernel's instrumented twin on the same frame; duration per
+                launch from HIP events on the launch stream.
+  cpu_baseline  the CPU oracle (a C restatement of the reference's CPU path)
+                on a bounded random sample of 16x16 output tiles of the same
+                frame, on the host's cores; those tiles also spot-check parity.
+
+    python bench.py [--steps K] [--warmup W] [--workload pixel-c2|grain-c3|pixel-c1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    "pixel-c2": dict(w=1920, h=1080, rgb=True, radius=0.1, sd=0.05, dist="lognorm", zoom=2.0,
+                     iters=256, seed=7, algo="pixel",
+                     desc="pixel-wise 1920x1080 RGB -> 3840x2160, lognormal 0.1/0.05, sigma 0.8, N=256, seed 7"),
+    "grain-c3": dict(w=4096, h=4096, rgb=False, radius=0.05, sd=0.0, dist="const", zoom=1.0,
+                     iters=128, seed=11, algo="grain",
+                     desc="grain-wise 4096x4096 luma ramp+5% noise, r=0.05 const, sigma 0.8, N=128, seed 11"),
+    "pixel-c1": dict(w=256, h=256, rgb=False, radius=0.12, sd=0.0, dist="const", zoom=1.0,
+                     iters=64, seed=42, algo="pixel",
+                     desc="pixel-wise 256x256 luma, r=0.12 const, sigma 0.8, N=64, seed 42"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(wl):
+    from film_grain_amd import synth
+    if wl is WORKLOADS["pixel-c2"]:
+        planes = synth.c2_rgb(wl["seed"], wl["w"], wl["h"])
+    elif wl is WORKLOADS["grain-c3"]:
+        planes = synth.c3_ramp(wl["seed"], wl["w"])[None]
+    else:
+        planes = synth.c1_luma(wl["seed"], wl["w"])[None]
+    return planes
+
+
+def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
+    """Times the oracle on random 16x16 output tiles of plane 0 and checks the
+    GPU frame on the same tiles."""
+    from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    od = O.derive(wl["radius"], wl["sd"], wl["dist"], wl["zoom"], 0.8, wl["iters"], seed=wl["seed"],
+                  in_w=wl["w"], in_h=wl["h"])
+    p = O.plane(od, wl["iters"], wl["seed"], wl["zoom"], wl["dist"], wl["radius"])
+    rng = np.random.default_rng(2024)
+    tile = 16
+    n_px = 0
+    mism = 0
+    t_used = 0.0
+    n_tiles = 0
+    if wl["algo"] == "grain":
+        # the grain-wise oracle works on whole frames: time a crop of the input
+        crop = min(wl["w"], 256)
+        lam_c = np.ascontiguousarray(lam0[:crop, :crop])
+        od_c = O.derive(wl["radius"], wl["sd"], wl["dist"], wl["zoom"], 0.8, wl["iters"],
+                        seed=wl["seed"], in_w=crop, in_h=crop)
+        pc = O.plane(od_c, wl["iters"], wl["seed"], wl["zoom"], wl["dist"], wl["radius"])
+        t0 = time.perf_counter()
+        O.render_grainwise(pc, lam_c, offs, threads=threads)
+        t_used = time.perf_counter() - t0
+        n_px = pc.out_w * pc.out_h
+        sample = f"grain-wise oracle on the top-left {crop}x{crop} input crop of the frame"
+        return dict(value=n_px * wl["iters"] / t_used / 1e6, unit="Mpix·samples/s", cores=threads,
+                    kind="port", sample=sample, seconds=round(t_used, 3)), None
+    while t_used < budget_s:
+        x0 = int(rng.integers(0, od.out_w - tile + 1))
+        y0 = int(rng.integers(0, od.out_h - tile + 1))
+        t0 = time.perf_counter()
+        want = O.render_pixelwise(p, lam0, offs, region=(x0, y0, x0 + tile, y0 + tile), threads=threads)
+        t_used += time.perf_counter() - t0
+        n_px += tile * tile
+        n_tiles += 1
+        got = gpu_plane0[y0:y0 + tile, x0:x0 + tile]
+        mism += int(np.sum(got != want))
+        if n_tiles >= 400:
+            break
+    value = n_px * wl["iters"] / t_used / 1e6
+    sample = (f"{n_tiles} random {tile}x{tile} output tiles of colour plane 0 "
+              f"({n_px * wl['iters']:.3g} pixel-samples, {t_used:.1f} s)")
+    return dict(value=round(value, 3), unit="Mpix·samples/s", cores=threads, kind="port",
+                sample=sample), dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from film_grain_amd import capi, host
+
+    wl = WORKLOADS[args.workload]
+    log(f"[bench] workload {args.workload}: {wl['desc']}")
+    planes = make_inputs(wl)
+    cli = host.CliArgs(radius=wl["radius"], radius_stddev=wl["sd"], radius_dist=wl["dist"], zoom=wl["zoom"],
+                       iters=wl["iters"], seed=wl["seed"], algo=wl["algo"], sigma=0.8,
+                       color_mode="rgb" if wl["rgb"] else "luma")
+    d, offs, offs_in = host.derive(cli, wl["w"], wl["h"])
+    lams = []
+    for pl in planes:
+        nrm, _ = host.normalize_plane(pl)
+        lams.append(host.lambda_plane(nrm, d.inv_e_pi_r2))
+    algo = wl["algo"]
+    offsets = offs_in if algo == "pixel" else offs
+    desc = capi.make_desc(d.in_w, d.in_h, d.out_w, d.out_h, d.n_samples, d.dist_kind, d.seed, d.zoom,
+                          d.delta, d.rm, d.radius_mean, d.log_mu, d.log_sigma)
+
+    dev = torch.device("cuda:0")
+    ctx = capi.Context(0)
+    lam_d = [torch.from_numpy(l).to(dev) for l in lams]
+    offs_d = torch.from_numpy(np.ascontiguousarray(offsets)).to(dev)
+    out_d = torch.empty((len(lams), d.out_h, d.out_w), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    cfg = ctx.launch_config(desc, algo)
+    log(f"[bench] launch config {cfg}")
+
+    # instrumented pass: algorithmic work of the frame
+    counters = []
+    for i, l in enumerate(lam_d):
+        counters.append(ctx.stats_device(algo, desc, l, offs_d, out_d[i], stream))
+    torch.cuda.synchronize()
+    log(f"[bench] counters per plane {counters}")
+
+    def step():
+        for i, l in enumerate(lam_d):
+            ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.stream_status(stream)
+    torch.cuda.synchronize()
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps * len(lam_d))]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k = 0
+    for _ in range(args.steps):
+        for i, l in enumerate(lam_d):
+            ev[k][0].record(stream)
+            ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
+            ev[k][1].record(stream)
+            k += 1
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.stream_status(stream)
+    secs = t1 - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    avg_launch_ms = float(np.mean(launch_ms))
+    log(f"[bench] {args.steps} steps in {secs:.4f} s; launch ms mean {avg_launch_ms:.3f} "
+        f"min {min(launch_ms):.3f} max {max(launch_ms):.3f}")
+
+    px_samples = d.out_w * d.out_h * len(lam_d) * d.n_samples
+    value = px_samples * args.steps / secs / 1e6
+    if algo == "pixel":
+        flops = [16.0 * c[0] + 6.0 * c[1] for c in counters]
+        achieved = float(np.mean(flops)) / (avg_launch_ms * 1e-3) / 1e12
+        roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                        frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
+                        kernel="pixelwise_tiles<lognormal>",
+                        flops_per_launch=float(np.mean(flops)),
+                        tests_per_point=round(sum(c[1] for c in counters) / sum(c[0] for c in counters), 4))
+    else:
+        # algorithmic bytes per output pixel: lambda read 4 + output write 4 + N/4 (bitset clear + read)
+        bytes_per_launch = d.out_w * d.out_h * (8 + d.n_samples / 4.0)
+        achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+        roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
+                        frac=round(achieved / PEAK_HBM_GBS, 6), traffic=None,
+                        kernel="grainwise_tiles<const>", bytes_per_launch=bytes_per_launch,
+                        pairs_per_launch=float(np.mean([c[1] for c in counters])))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        plane0 = out_d[0].cpu().numpy()
+        cpu, parity = cpu_baseline(wl, planes, lams[0], d, offsets, plane0, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu} parity {parity}")
+
+    line = {
+        "metric": "Mpix·samples/s at 4K RGB N=256" if args.workload == "pixel-c2"
+        else f"Mpix·samples/s ({args.workload})",
+        "value": round(value, 2),
+        "unit": "Mpix·samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "planes": len(lam_d), "out": [d.out_w, d.out_h],
+                   "n_samples": d.n_samples, "launch": cfg},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if parity is not None:
+        line["parity_spot_check"] = parity
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -62,7 +62,7 @@ def build(verbose: bool = False) -> Path:
     for src in HOST_SOURCES:
         obj = BUILD / ("host_" + src + ".o")
         objs.append(obj)
-        jobs.append([HIPCC, *COMMON, "-x", "hip", *DEVICE, "-c", str(CSRC / "host" / src), "-o", str(obj)])
+        jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "host" / src), "-o", str(obj)])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB.with_suffix(".so.tmp")
