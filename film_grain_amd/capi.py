@@ -47,7 +47,7 @@ class LaunchConfig(C.Structure):
     _fields_ = [
         ("pix_tile_w", C.c_int), ("pix_tile_h", C.c_int), ("pix_lds_bytes", C.c_int),
         ("grain_tile_w", C.c_int), ("grain_tile_h", C.c_int), ("grain_lds_bytes", C.c_int),
-        ("grain_grains_per_thread", C.c_int),
+        ("grain_grains_per_thread", C.c_int), ("pix_variant", C.c_int),
     ]
 
 

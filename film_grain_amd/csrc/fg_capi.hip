@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 
@@ -37,6 +38,7 @@ struct fg_ctx {
     unsigned long long *d_stats = nullptr;
     float4 *d_range = nullptr;
     fg_launch_config cfg{};
+    std::map<uint32_t, bool> div_checked;  // cell size -> reciprocal division verified
 };
 
 namespace {
@@ -107,12 +109,11 @@ int validate(fg_ctx *ctx, const fg_plane_desc *d)
     return FG_OK;
 }
 
-// Largest tile (from a fixed ladder) whose cell neighbourhood index fits in
-// about half of the LDS budget, for an assumed offset spread of +-3.5 px
-// (sigma = 0.8 output px, a few hundred samples).  Tiles whose actual
-// neighbourhood is larger are split inside the kernel, so this is a speed
-// choice, not a correctness one.
-void choose_pixel_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
+// Gather kernel: largest tile (from a fixed ladder) whose cell neighbourhood
+// index fits in about half of the LDS budget, for an assumed offset spread of
+// +-3.5 output px.  Tiles whose actual neighbourhood is larger are split inside
+// the kernel, so this is a speed choice, not a correctness one.
+void choose_gather_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
 {
     static const int ladder[][2] = {{32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}};
     const double iz = 1.0 / d->zoom;
@@ -130,17 +131,60 @@ void choose_pixel_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
     *th = 4;
 }
 
-void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
+// Splat kernel: largest tile whose N-bit sample sets take at most half of the
+// LDS budget, leaving room for the offsets table and at least 2048 grains.
+void choose_splat_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
 {
-    const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : 64 * 1024;
+    static const int ladder[][2] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8},
+                                    {8, 8},   {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
+    const size_t words = (d->n_samples + 31) / 32;
+    for (const auto &t : ladder) {
+        const size_t bits = (size_t)t[0] * t[1] * words * 4;
+        if (bits <= (size_t)lds / 2 && bits + 2048 * 32 <= (size_t)lds + 16384) {
+            *tw = t[0];
+            *th = t[1];
+            return;
+        }
+    }
+    *tw = 1;
+    *th = 1;
+}
+
+int pixel_variant(fg_ctx *ctx, const fg_plane_desc *d)
+{
+    if (ctx->cfg.pix_variant == 1 || d->n_samples > 65535) return 1;
+    return 0;
+}
+
+void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int variant, int *tw, int *th, int *lds)
+{
+    const int def = variant == 1 ? 64 * 1024 : 72 * 1024;
+    const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : def;
     *lds = std::min(budget, ctx->max_lds - 6 * 1024);
+    const int max_px = variant == 1 ? fg::kMaxTilePixels : 256 * 256;
     if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 &&
-        ctx->cfg.pix_tile_w * ctx->cfg.pix_tile_h <= fg::kMaxTilePixels) {
+        ctx->cfg.pix_tile_w * ctx->cfg.pix_tile_h <= max_px && ctx->cfg.pix_tile_w <= 256 &&
+        ctx->cfg.pix_tile_h <= 256) {
         *tw = ctx->cfg.pix_tile_w;
         *th = ctx->cfg.pix_tile_h;
+    } else if (variant == 1) {
+        choose_gather_tile(d, *lds, tw, th);
     } else {
-        choose_pixel_tile(d, *lds, tw, th);
+        choose_splat_tile(d, *lds, tw, th);
     }
+}
+
+// floor(a / delta) through the reciprocal sequence is used only for cell
+// sizes where the host has checked it against IEEE division.
+bool division_fast_ok(fg_ctx *ctx, float delta, float rdelta)
+{
+    uint32_t key;
+    std::memcpy(&key, &delta, 4);
+    auto it = ctx->div_checked.find(key);
+    if (it != ctx->div_checked.end()) return it->second;
+    const bool ok = fg::reciprocal_division_exact(delta, rdelta);
+    ctx->div_checked[key] = ok;
+    return ok;
 }
 
 void grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, int *gpt)
@@ -174,8 +218,9 @@ void grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *ld
 int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
                  float *d_out, hipStream_t s, bool stats)
 {
+    const int variant = stats ? 1 : pixel_variant(ctx, d);
     int tw, th, lds;
-    pixel_config(ctx, d, &tw, &th, &lds);
+    pixel_config(ctx, d, variant, &tw, &th, &lds);
     fg::PixelArgs a{};
     a.lambda = d_lambda;
     a.offs = reinterpret_cast<const float2 *>(d_offs);
@@ -183,6 +228,7 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.status = ctx->d_status;
     a.stats = ctx->d_stats;
     a.tile_list = nullptr;
+    a.off_range = ctx->d_range;
     a.seed = d->seed;
     a.log_mu = (double)d->log_mu;
     a.log_sigma = (double)d->log_sigma;
@@ -198,18 +244,27 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     if (ntiles > 0x7fffffffLL) return set_error(ctx, FG_EINVAL, "too many tiles");
     a.n_tiles = (int)ntiles;
     a.lds_bytes = lds;
+    a.words = (int)((d->n_samples + 31) / 32);
+    const size_t bits = (((size_t)tw * th * a.words * 4) + 15) & ~(size_t)15;
+    a.offs_in_lds = (variant == 0 && d->n_samples <= 2048 &&
+                     bits + (size_t)d->n_samples * 8 + 256 * 32 <= (size_t)lds) ? 1 : 0;
+    if (variant == 0 && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + 256 * 32 > (size_t)lds)
+        return set_error(ctx, FG_EINVAL, "sample count too large for one pixel-wise tile");
     a.inv_zoom = 1.0f / d->zoom;
+    a.zoom = d->zoom;
     a.rm = d->rm;
     a.delta = d->delta;
+    a.rdelta = 1.0f / d->delta;
+    a.div_fast = division_fast_ok(ctx, a.delta, a.rdelta) ? 1 : 0;
     a.uscale = fg::uniform_f32_scale(0.0f, d->delta);
     float rc = d->radius_mean;  // RadiusProfile::sample for Const, then the rm clamp
     if (rc > d->rm) rc = d->rm;
+    a.r_const = rc;
     a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
-    a.off_range = ctx->d_range;
     hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
-    e = fg::launch_pixelwise(a, (int)d->dist_kind, stats, s);
+    e = fg::launch_pixelwise(a, (int)d->dist_kind, variant, stats, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     return FG_OK;
 }
@@ -460,9 +515,10 @@ int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg)
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (cfg) {
         if (cfg->pix_tile_w < 0 || cfg->pix_tile_h < 0 ||
-            cfg->pix_tile_w * cfg->pix_tile_h > fg::kMaxTilePixels || cfg->grain_tile_w < 0 ||
+            cfg->pix_tile_w > 256 || cfg->pix_tile_h > 256 || cfg->grain_tile_w < 0 ||
             cfg->grain_tile_h < 0 || cfg->grain_grains_per_thread < 0 ||
-            cfg->grain_grains_per_thread > 64 || cfg->pix_lds_bytes < 0 || cfg->grain_lds_bytes < 0)
+            cfg->grain_grains_per_thread > 64 || cfg->pix_lds_bytes < 0 || cfg->grain_lds_bytes < 0 ||
+            cfg->pix_variant < 0 || cfg->pix_variant > 1)
             return set_error(ctx, FG_EINVAL, "invalid launch configuration");
         ctx->cfg = *cfg;
     } else {
@@ -482,7 +538,9 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
         grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes,
                      &out->grain_grains_per_thread);
     } else {
-        pixel_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes);
+        out->pix_variant = pixel_variant(ctx, desc);
+        pixel_config(ctx, desc, out->pix_variant, &out->pix_tile_w, &out->pix_tile_h,
+                     &out->pix_lds_bytes);
     }
     return FG_OK;
 }

@@ -1,0 +1,100 @@
+// fg_device.h -- device helpers shared by the film-grain kernels.
+#ifndef FG_DEVICE_H
+#define FG_DEVICE_H
+
+#include "fg_kernels.h"
+#include "fg_rng.h"
+
+namespace fg {
+
+static __constant__ double d_zig_x[257] = {FG_ZIG_NORM_X_VALUES};
+static __constant__ double d_zig_f[257] = {FG_ZIG_NORM_F_VALUES};
+
+// floor(RN(a / d)) cast like Rust's `as i32`.  With fast set, the quotient
+// comes from q0 = a*rd corrected by one FMA residual step, which the host has
+// verified equals the IEEE quotient for this d over a whole binade; outside
+// [2^-60, 2^60] (where intermediates could leave the normal range) the IEEE
+// division is used.
+__device__ __forceinline__ int floor_div(float a, float d, float rd, int fast)
+{
+    float q;
+    const float aa = fabsf(a);
+    if (fast && aa >= 8.67361737988404e-19f && aa <= 1.152921504606847e18f) {
+        const float q0 = a * rd;
+        const float r = fmaf(-q0, d, a);
+        q = fmaf(r, rd, q0);
+    } else {
+        q = __fdiv_rn(a, d);
+    }
+    return sat_i32(floorf(q));
+}
+
+// Workgroup exclusive scan of one u32 per thread (kBlock threads); returns the
+// exclusive prefix and the workgroup total.  Contains two barriers.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave,
+                                                         uint32_t &total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t t = s_wave[w];
+        before += (w < wid) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
+// Poisson threshold of one lambda value for the pixel-wise cells
+// (src/pixelwise.rs:69-82): 2.0 when the cell is skipped (lambda <= 0 or
+// lambda*delta*delta <= 0), exp(-expected) for the Knuth branch (expected < 12),
+// and -expected for the rejection branch.
+__device__ __forceinline__ double cell_threshold(float lam, float delta)
+{
+    if (!(lam > 0.0f)) return 2.0;
+    const float expected = lam * delta * delta;
+    if (!(expected > 0.0f)) return 2.0;
+    if (expected < 12.0f) return exp(-(double)expected);
+    return -(double)expected;
+}
+
+// Poisson count of a cell given its hash and threshold, leaving rng positioned
+// after the count draws (as Poisson::sample leaves it).  The common empty cell
+// costs only the two seed words the first Xoshiro output depends on.
+__device__ __forceinline__ uint32_t cell_count(uint64_t h, double e, Xoshiro &rng)
+{
+    if (e > 1.5) return 0u;
+    if (e >= 0.0) {
+        const uint64_t s0 = splitmix64(h), s3 = splitmix64(h + 3 * kGolden);
+        const double u1 = standard_f64(rotl64(s0 + s3, 23) + s0);
+        if (!(u1 > e)) return 0u;  // Knuth: p = u1 <= exp(-lambda) stops at once
+        rng.seed(h);
+        (void)rng.next();
+        double p = u1;
+        uint32_t n = 1;
+        while (p > e) {
+            p = p * standard_f64(rng.next());
+            ++n;
+        }
+        return n - 1u;
+    }
+    rng.seed(h);
+    const double r = poisson_rejection(rng, -e);
+    if (!(r > 0.0)) return 0u;
+    if (r >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)r;
+}
+
+}  // namespace fg
+
+#endif

@@ -18,13 +18,9 @@
 // slotted grains with lanes over samples k, so the 64 lanes of a wave test one
 // grain against 64 different offsets and the grain is a uniform (broadcast)
 // LDS read.
-#include "fg_kernels.h"
-#include "fg_rng.h"
+#include "fg_device.h"
 
 namespace fg {
-
-static __constant__ double g_zx[257] = {FG_ZIG_NORM_X_VALUES};
-static __constant__ double g_zf[257] = {FG_ZIG_NORM_F_VALUES};
 
 // src/grainwise.rs:126-142, with the image limit; returns false for None.
 __device__ __forceinline__ bool ref_bounds(float center, float radius, int limit, int &lo, int &hi)
@@ -108,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
             while (n < gpt && done < q) {
                 const float cx = fx + uniform_f32(rng, uscale1);
                 const float cy = fy + uniform_f32(rng, uscale1);
-                float rad = KIND == 0 ? a.r_const : lognormal_f32(rng, a.log_mu, a.log_sigma, g_zx, g_zf);
+                float rad = KIND == 0 ? a.r_const : lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
                 ++done;
                 if (rad > a.rm) rad = a.rm;
                 if (rad <= 0.0f) continue;

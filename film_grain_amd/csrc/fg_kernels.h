@@ -8,12 +8,14 @@
 
 namespace fg {
 
-constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
-constexpr int kMaxTilePixels = 1024;  // largest output tile one workgroup owns
+constexpr int kBlock = 256;           // threads per workgroup (4 wave64)
+constexpr int kMaxTilePixels = 1024;  // largest output tile of the gather kernel
+constexpr int kQueueCap = 128;        // per-wave candidate queue of the splat kernel
 
 // Device status word bits (OR-ed by kernels, read back by the host).
 constexpr int kStatusNeighbourhood = 1;  // one output pixel's cell window exceeds LDS
 constexpr int kStatusCancelled = 2;
+constexpr int kStatusRegion = 4;         // a tile's cell region exceeds 65535 cells per side
 
 // Pixel-wise integrator launch (src/pixelwise.rs:11-106).
 struct PixelArgs {
@@ -23,13 +25,16 @@ struct PixelArgs {
     int *status;                // device status word
     unsigned long long *stats;  // optional counters: points, tests (T), hits
     const int *tile_list;       // optional tile indices (batching / resubmission)
+    const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
     uint64_t seed;
     double log_mu, log_sigma;
     int in_w, in_h, out_w, out_h, n_samples;
     int tile_w, tile_h, tiles_x, n_tiles;
     int lds_bytes;
-    float inv_zoom, rm, delta, uscale, r2_const, inv_samples;
-    const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
+    int words;                  // ceil(n_samples / 32): bitset words per pixel (splat)
+    int offs_in_lds;            // splat: stage the offsets table in LDS
+    int div_fast;               // 1: floor((a)/delta) via the checked reciprocal sequence
+    float inv_zoom, zoom, rm, delta, rdelta, uscale, r_const, r2_const, inv_samples;
 };
 
 // Grain-wise integrator launch (src/grainwise.rs:12-124).
@@ -39,6 +44,7 @@ struct GrainArgs {
     float *out;
     int *status;
     unsigned long long *stats;  // optional counters: grains, (grain, sample) pairs, hits
+    const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
     uint64_t seed;
     double log_mu, log_sigma;
     int in_w, in_h, out_w, out_h, n_samples, words;
@@ -46,13 +52,19 @@ struct GrainArgs {
     int grains_per_thread;
     int lds_bytes;
     float zoom, rm, r_const, inv_samples, r_out_max;
-    const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
 };
 
 // Writes (min x, max x, min y, max y) of n (x, y) pairs to *range (one workgroup).
 hipError_t launch_offsets_range(const float2 *offs, int n, float4 *range, hipStream_t s);
-hipError_t launch_pixelwise(const PixelArgs &a, int dist_kind, bool with_stats, hipStream_t s);
+// variant: 0 splat (default), 1 gather.  Stats always use the gather kernel.
+hipError_t launch_pixelwise(const PixelArgs &a, int dist_kind, int variant, bool with_stats,
+                            hipStream_t s);
 hipError_t launch_grainwise(const GrainArgs &a, int dist_kind, bool with_stats, hipStream_t s);
+
+// Host check that q0 = a*rd, r = fma(-q0, d, a), q = fma(r, rd, q0) equals the
+// IEEE quotient a/d for every a of one binade (hence, by scaling, for every a
+// whose intermediates stay normal).  Cached per d by the caller.
+bool reciprocal_division_exact(float d, float rd);
 
 }  // namespace fg
 

@@ -4,71 +4,38 @@
 // (src/pixelwise.rs:11-106): for every output pixel and Monte-Carlo offset k,
 // the point (xg, yg) = ((x+0.5)/zoom - off_k.x, (y+0.5)/zoom - off_k.y) is
 // covered iff some grain of some cell (i, j) in the reference's window
-// [floor((xg-rm)/delta), floor((xg+rm)/delta)] x [...] contains it; the pixel
-// value is hits * (1/N).
+// [floor((xg-rm)/delta), floor((xg+rm)/delta)] x [...] contains it
+// (dx*dx + dy*dy <= r*r in f32); the pixel value is hits * (1/N).
 //
-// The reference regenerates a cell's grains (cell hash, Xoshiro seed, Poisson
-// count, centres, radii) at every (pixel, sample, cell) visit.  A cell's grains
-// depend only on (seed, i, j, lambda(cell)), so here each workgroup owns an
-// output tile, generates every cell of the tile's neighbourhood once into LDS,
-// and the coverage tests become pure FP32 work:
-//   phase 1  cells in column-major order (i outer, j inner, the reference's
-//            loop order), one cell per thread per chunk; Poisson counts are
-//            prefix-summed across the workgroup and each thread appends its
-//            cell's grains (centre, r^2) at its offset, so the grains of cells
-//            (i, j0..j1) are one contiguous LDS range;
-//   phase 2  each thread owns (pixel, sample-slice) work items, walks the
-//            window's columns and tests the contiguous grain ranges, stopping
-//            at the first hit exactly like the reference's early exit;
-//   phase 3  per-pixel hit counts are combined in LDS and stored once.
-// If a tile's neighbourhood does not fit in LDS it is split in halves and each
-// half processed in turn (down to a single pixel).
+// The reference regenerates a cell's grains at every (pixel, sample, cell)
+// visit.  A cell's grains depend only on (seed, i, j, lambda(cell)), so each
+// workgroup here owns an output tile and generates every cell of the tile's
+// neighbourhood once, in LDS.  The neighbourhood bound is exact: each step from
+// a pixel index to its window bound is a monotone rounding function, so the
+// tile's extreme pixel and extreme offset bound every point's window.
 //
-// Tile neighbourhood bounds are exact: every step from x to the window bound is
-// a monotone rounding function, so evaluating it at the tile's extreme pixel and
-// extreme offset bounds every point's window.
-#include "fg_kernels.h"
-#include "fg_rng.h"
+// Two formulations, bit-identical by construction:
+//   splat   (default) grains are tested per sample: for grain g and offset k the
+//           output pixels whose sample point can lie in g's disk form a box of at
+//           most a few pixels; candidate (g, k, box) entries go to a per-wave LDS
+//           queue and are drained 64 at a time through the exact reference test
+//           (disk test plus window membership of g's cell), setting bit k of
+//           the pixel in an LDS bitset; a popcount epilogue stores the tile.
+//   gather  each (pixel, sample) walks its window's cells in the reference's
+//           order and stops at the first hit; kept as the instrumented twin that
+//           counts the reference's disk tests T (the FP32 FLOP count 16 + 6T).
+#include <cmath>
+#include <cstring>
+
+#include "fg_device.h"
 
 namespace fg {
 
-static __constant__ double c_zx[257] = {FG_ZIG_NORM_X_VALUES};
-static __constant__ double c_zf[257] = {FG_ZIG_NORM_F_VALUES};
-
-// floor(RN(a / delta)) as Rust's `as i32` (IEEE round-to-nearest division).
-__device__ __forceinline__ int floor_div(float a, float delta)
-{
-    return sat_i32(floorf(__fdiv_rn(a, delta)));
-}
-
-// Workgroup exclusive scan of one u32 per thread; returns the exclusive prefix
-// and the workgroup total.  Contains two barriers.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave,
-                                                         uint32_t &total)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wave[wid] = x;
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-        const uint32_t t = s_wave[w];
-        before += (w < wid) ? t : 0u;
-        tot += t;
-    }
-    __syncthreads();
-    total = tot;
-    return before + x - v;
-}
-
+// ---------------------------------------------------------------------------
+// gather formulation
+// ---------------------------------------------------------------------------
 template <int KIND, bool STATS>
-__global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
+__global__ __launch_bounds__(kBlock) void pixelwise_gather(PixelArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_cnt[kMaxTilePixels];
@@ -81,7 +48,8 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
     if (a.tile_list) tile = a.tile_list[tile];
     if (tile >= a.n_tiles) return;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-    const float rm = a.rm, delta = a.delta, iz = a.inv_zoom;
+    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
+    const int fast = a.div_fast;
     const float4 orange = *a.off_range;
 
     for (int p = tid; p < kMaxTilePixels; p += kBlock) s_cnt[p] = 0;
@@ -101,13 +69,13 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
         __syncthreads();
         if (tid == 0) s_sp = sp - 1;
 
-        // ---- neighbourhood of the rect (exact bounds, see header) ----
+        // ---- neighbourhood of the rect (exact bounds) ----
         const float px_lo = ((float)r.x + 0.5f) * iz, px_hi = ((float)(r.z - 1) + 0.5f) * iz;
         const float py_lo = ((float)r.y + 0.5f) * iz, py_hi = ((float)(r.w - 1) + 0.5f) * iz;
-        const int I0 = floor_div((px_lo - orange.y) - rm, delta);
-        const int I1 = floor_div((px_hi - orange.x) + rm, delta);
-        const int J0 = floor_div((py_lo - orange.w) - rm, delta);
-        const int J1 = floor_div((py_hi - orange.z) + rm, delta);
+        const int I0 = floor_div((px_lo - orange.y) - rm, delta, rd, fast);
+        const int I1 = floor_div((px_hi - orange.x) + rm, delta, rd, fast);
+        const int J0 = floor_div((py_lo - orange.w) - rm, delta, rd, fast);
+        const int J1 = floor_div((py_hi - orange.z) + rm, delta, rd, fast);
         const long long W = (long long)I1 - I0 + 1, H = (long long)J1 - J0 + 1;
         const long long C64 = W * H;
         const int gsize = KIND == 0 ? 8 : 12;
@@ -126,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
             gr2 = reinterpret_cast<float *>(lds + prefix_bytes + 8LL * cap_g);
         }
 
-        // ---- phase 1: generate the neighbourhood's cells into LDS ----
+        // ---- phase 1: cells in column-major order (i outer, j inner) ----
         unsigned long long running = 0;
         for (int cb = 0; fits && cb < C; cb += kBlock) {
             const int c = cb + tid;
@@ -137,18 +105,14 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
             if (c < C) {
                 const int ci = c / Hc, cj = c - ci * Hc;
                 const int i = I0 + ci, j = J0 + cj;
-                rng.seed(mix3(a.seed, kCellStream, i, j));
                 sx = (float)i * delta;
                 sy = (float)j * delta;
                 // Plane::get_clamped(floor(sx), floor(sy)) (src/model.rs:60-67)
                 const float fx = floorf(sx), fy = floorf(sy);
                 const int ix = fx < 0.0f ? 0 : (fx > (float)(a.in_w - 1) ? a.in_w - 1 : (int)fx);
                 const int iy = fy < 0.0f ? 0 : (fy > (float)(a.in_h - 1) ? a.in_h - 1 : (int)fy);
-                const float lam = a.lambda[(size_t)iy * a.in_w + ix];
-                if (lam > 0.0f) {
-                    const float expected = lam * delta * delta;
-                    if (expected > 0.0f) q = poisson_u32(rng, (double)expected);
-                }
+                const double e = cell_threshold(a.lambda[(size_t)iy * a.in_w + ix], delta);
+                q = cell_count(mix3(a.seed, kCellStream, i, j), e, rng);
             }
             uint32_t total;
             const uint32_t excl = block_exclusive_scan(q, s_wave, total);
@@ -163,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
                 const float cy = sy + uniform_f32(rng, a.uscale);
                 gpos[base + t] = make_float2(cx, cy);
                 if (KIND == 1) {
-                    float rad = lognormal_f32(rng, a.log_mu, a.log_sigma, c_zx, c_zf);
+                    float rad = lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
                     if (rad > rm) rad = rm;
                     gr2[base + t] = rad <= 0.0f ? -1.0f : rad * rad;
                 }
@@ -196,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
         if (tid == 0) prefix[C] = (uint32_t)running;
         __syncthreads();
 
-        // ---- phase 2: coverage tests ----
+        // ---- phase 2: coverage tests in the reference's order ----
         const int rw = r.z - r.x, rh = r.w - r.y, P = rw * rh;
         const int S = P >= kBlock ? 1 : kBlock / P;
         const int items = P * S;
@@ -210,10 +174,10 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
             for (int k = sl; k < a.n_samples; k += S) {
                 const float2 o = a.offs[k];
                 const float xg = pxx - o.x, yg = pyy - o.y;
-                const int i0 = max(floor_div(xg - rm, delta), I0);
-                const int i1 = min(floor_div(xg + rm, delta), I1);
-                const int j0 = max(floor_div(yg - rm, delta), J0);
-                const int j1 = min(floor_div(yg + rm, delta), J1);
+                const int i0 = max(floor_div(xg - rm, delta, rd, fast), I0);
+                const int i1 = min(floor_div(xg + rm, delta, rd, fast), I1);
+                const int j0 = max(floor_div(yg - rm, delta, rd, fast), J0);
+                const int j1 = min(floor_div(yg + rm, delta, rd, fast), J1);
                 bool hit = false;
                 // j0 <= j1 always holds for finite offsets; the guard keeps the
                 // prefix reads in range for any input.
@@ -253,6 +217,268 @@ __global__ __launch_bounds__(kBlock) void pixelwise_tiles(PixelArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// splat formulation
+// ---------------------------------------------------------------------------
+struct SplatTile {
+    int X0, Y0, X1, Y1;  // output rect
+    int I0, J0;          // region origin (cell indices)
+    int TW;              // bitset row pitch (tile_w)
+    float fx0, fx1, fy0, fy1;  // X0, X1-1, Y0, Y1-1 as floats
+};
+
+// Drains the first n (<= 64) entries of this wave's queue through the exact
+// reference test and moves entries [64, count) down to the front.
+__device__ __forceinline__ void splat_drain(const PixelArgs &a, const SplatTile &t, uint2 *queue,
+                                            int n, int count, const float4 *gA,
+                                            const float2 *offs, uint32_t *bits)
+{
+    const int lane = threadIdx.x & 63;
+    uint2 e = make_uint2(0u, 0u);
+    if (lane < n) e = queue[lane];
+    if (lane + 64 < count) queue[lane] = queue[lane + 64];
+    if (lane >= n) return;
+    const uint32_t g = e.x & 0xFFFFu;
+    const int xhi = t.X0 + (int)((e.x >> 16) & 0xFFu), yhi = t.Y0 + (int)(e.x >> 24);
+    const int k = (int)(e.y & 0xFFFFu);
+    const int xlo = t.X0 + (int)((e.y >> 16) & 0xFFu), ylo = t.Y0 + (int)(e.y >> 24);
+    const float4 G = gA[g];
+    const uint32_t cell = __float_as_uint(G.w);
+    const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
+    const float2 o = offs[k];
+    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
+    const int fast = a.div_fast;
+    for (int y = ylo; y <= yhi; ++y) {
+        const float yg = ((float)y + 0.5f) * iz - o.y;
+        const float dy = yg - G.y;
+        const float dy2 = dy * dy;
+        for (int x = xlo; x <= xhi; ++x) {
+            const float xg = ((float)x + 0.5f) * iz - o.x;
+            const float dx = xg - G.x;
+            if (dx * dx + dy2 <= G.z) {
+                // the grain's cell must lie in the point's window
+                if (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
+                    floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)) {
+                    atomicOr(&bits[((y - t.Y0) * t.TW + (x - t.X0)) * a.words + (k >> 5)], 1u << (k & 31));
+                }
+            }
+        }
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void pixelwise_splat(PixelArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint2 s_queue[kBlock / 64][kQueueCap];
+    __shared__ uint32_t s_wave[kBlock / 64];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int tile = blockIdx.x;
+    if (a.tile_list) tile = a.tile_list[tile];
+    if (tile >= a.n_tiles) return;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    SplatTile t;
+    t.X0 = txi * a.tile_w;
+    t.Y0 = tyi * a.tile_h;
+    t.X1 = min(t.X0 + a.tile_w, a.out_w);
+    t.Y1 = min(t.Y0 + a.tile_h, a.out_h);
+    t.TW = a.tile_w;
+    t.fx0 = (float)t.X0;
+    t.fx1 = (float)(t.X1 - 1);
+    t.fy0 = (float)t.Y0;
+    t.fy1 = (float)(t.Y1 - 1);
+    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
+    const int fast = a.div_fast;
+    const int N = a.n_samples, words = a.words;
+    const float4 orange = *a.off_range;
+
+    // ---- neighbourhood of the tile (exact bounds) ----
+    const float px_lo = (t.fx0 + 0.5f) * iz, px_hi = (t.fx1 + 0.5f) * iz;
+    const float py_lo = (t.fy0 + 0.5f) * iz, py_hi = (t.fy1 + 0.5f) * iz;
+    const int I0 = floor_div((px_lo - orange.y) - rm, delta, rd, fast);
+    const int I1 = floor_div((px_hi - orange.x) + rm, delta, rd, fast);
+    const int J0 = floor_div((py_lo - orange.w) - rm, delta, rd, fast);
+    const int J1 = floor_div((py_hi - orange.z) + rm, delta, rd, fast);
+    t.I0 = I0;
+    t.J0 = J0;
+    const long long W = (long long)I1 - I0 + 1, H = (long long)J1 - J0 + 1;
+    if (W <= 0 || H <= 0 || W > 65535 || H > 65535 || W * H > 0x7fffffffLL) {
+        if (tid == 0) atomicOr(a.status, kStatusRegion);
+        return;
+    }
+    const int C = (int)(W * H);
+
+    // ---- LDS layout ----
+    size_t off = 0;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
+    const int tile_words = a.tile_w * a.tile_h * words;
+    off += ((size_t)tile_words * 4 + 15) & ~(size_t)15;
+    const float2 *offs = a.offs;
+    if (a.offs_in_lds) {
+        float2 *ol = reinterpret_cast<float2 *>(lds + off);
+        for (int k = tid; k < N; k += kBlock) ol[k] = a.offs[k];
+        offs = ol;
+        off += ((size_t)N * 8 + 15) & ~(size_t)15;
+    }
+    // lambda -> Poisson threshold of the region's input pixels
+    const int PX0 = (int)fminf(fmaxf(floorf((float)I0 * delta), 0.0f), (float)(a.in_w - 1));
+    const int PX1 = (int)fminf(fmaxf(floorf((float)I1 * delta), 0.0f), (float)(a.in_w - 1));
+    const int PY0 = (int)fminf(fmaxf(floorf((float)J0 * delta), 0.0f), (float)(a.in_h - 1));
+    const int PY1 = (int)fminf(fmaxf(floorf((float)J1 * delta), 0.0f), (float)(a.in_h - 1));
+    const int ETW = PX1 - PX0 + 1, ETH = PY1 - PY0 + 1;
+    const size_t et_bytes = (((size_t)ETW * ETH * 8) + 15) & ~(size_t)15;
+    const int gsize = 32;
+    const bool use_et = off + et_bytes + 256 * gsize <= (size_t)a.lds_bytes;
+    double *etab = reinterpret_cast<double *>(lds + off);
+    if (use_et) {
+        for (int p = tid; p < ETW * ETH; p += kBlock) {
+            const int px = PX0 + p % ETW, py = PY0 + p / ETW;
+            etab[p] = cell_threshold(a.lambda[(size_t)py * a.in_w + px], delta);
+        }
+        off += et_bytes;
+    }
+    const int gcap = (int)min((long long)((a.lds_bytes - (long long)off) / gsize), 65535LL);
+    if (gcap < kBlock) {  // the host sizes LDS so that one round of grains always fits
+        if (tid == 0) atomicOr(a.status, kStatusRegion);
+        return;
+    }
+    float4 *gA = reinterpret_cast<float4 *>(lds + off);
+    float4 *gB = gA + gcap;
+    for (int w = tid; w < tile_words; w += kBlock) bits[w] = 0u;
+
+    // sample mapping: lanes over samples k, groups over grains
+    const int KL = N >= kBlock ? kBlock : N;
+    const int GS = kBlock / KL;
+    const int kl = tid % KL, gsub = tid / KL;
+    const bool tester = tid < KL * GS;
+    const int KM = (N + KL - 1) / KL;  // samples per thread
+    float oxz0 = 0.0f, oyz0 = 0.0f;
+    if (KM == 1 && tester) {
+        const float2 o = a.offs[kl];
+        oxz0 = o.x * z;
+        oyz0 = o.y * z;
+    }
+    // candidate box slack: covers the rounding of xg, dx and of the box itself
+    const float slack = 4e-3f + 2e-6f * (float)max(a.out_w, a.out_h);
+    uint2 *queue = s_queue[wid];
+    int qcount = 0;  // wave-uniform
+    __syncthreads();
+
+    auto flush = [&](int fill) {
+        __syncthreads();
+        const int TT = (fill + GS - 1) / GS;
+        for (int it = 0; it < TT; ++it) {
+            const int g = gsub + it * GS;
+            const bool gv = tester && g < fill;
+            const float4 B = gB[gv ? g : 0];
+            for (int m = 0; m < KM; ++m) {
+                const int k = kl + m * KL;
+                bool cand = false;
+                uint32_t e0 = 0, e1 = 0;
+                if (gv && k < N) {
+                    float oxz = oxz0, oyz = oyz0;
+                    if (KM > 1) {
+                        const float2 o = offs[k];
+                        oxz = o.x * z;
+                        oyz = o.y * z;
+                    }
+                    const float cx = B.x + oxz, cy = B.y + oyz;
+                    const float xl = fmaxf(ceilf(cx - B.z), t.fx0), xh = fminf(floorf(cx + B.z), t.fx1);
+                    const float yl = fmaxf(ceilf(cy - B.z), t.fy0), yh = fminf(floorf(cy + B.z), t.fy1);
+                    cand = xl <= xh && yl <= yh;
+                    if (cand) {
+                        const int ixl = (int)xl - t.X0, ixh = (int)xh - t.X0;
+                        const int iyl = (int)yl - t.Y0, iyh = (int)yh - t.Y0;
+                        e0 = (uint32_t)g | ((uint32_t)ixh << 16) | ((uint32_t)iyh << 24);
+                        e1 = (uint32_t)k | ((uint32_t)ixl << 16) | ((uint32_t)iyl << 24);
+                    }
+                }
+                const unsigned long long mask = __ballot(cand);
+                if (cand) {
+                    const int pos = qcount + __popcll(mask & ((1ull << lane) - 1ull));
+                    queue[pos] = make_uint2(e0, e1);
+                }
+                qcount += __popcll(mask);
+                if (qcount >= 64) {
+                    splat_drain(a, t, queue, 64, qcount, gA, offs, bits);
+                    qcount -= 64;
+                }
+            }
+        }
+        if (qcount > 0) {
+            splat_drain(a, t, queue, qcount, qcount, gA, offs, bits);
+            qcount = 0;
+        }
+        __syncthreads();
+    };
+
+    // ---- generation: cells in row-major order, grains batched through LDS ----
+    const int Wi = (int)W;
+    const float zs = z;
+    int fill = 0;  // uniform
+    // grains one thread may add per round, so that one round always fits the
+    // buffer (gcap >= kBlock is guaranteed by the host's LDS sizing)
+    const uint32_t kRound = (uint32_t)max(1, min(8, gcap / kBlock));
+    for (int cb = 0; cb < C; cb += kBlock) {
+        const int c = cb + tid;
+        uint32_t rem = 0;
+        Xoshiro rng;
+        rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
+        float sx = 0.0f, sy = 0.0f;
+        uint32_t cellpk = 0;
+        if (c < C) {
+            const int cj = c / Wi, ci = c - cj * Wi;
+            const int i = I0 + ci, j = J0 + cj;
+            sx = (float)i * delta;
+            sy = (float)j * delta;
+            const float fx = floorf(sx), fy = floorf(sy);
+            const int ix = fx < 0.0f ? 0 : (fx > (float)(a.in_w - 1) ? a.in_w - 1 : (int)fx);
+            const int iy = fy < 0.0f ? 0 : (fy > (float)(a.in_h - 1) ? a.in_h - 1 : (int)fy);
+            const double e = use_et ? etab[(iy - PY0) * ETW + (ix - PX0)]
+                                    : cell_threshold(a.lambda[(size_t)iy * a.in_w + ix], delta);
+            rem = cell_count(mix3(a.seed, kCellStream, i, j), e, rng);
+            cellpk = (uint32_t)ci | ((uint32_t)cj << 16);
+        }
+        while (__syncthreads_or(rem > 0u)) {
+            const uint32_t n = rem < kRound ? rem : kRound;
+            uint32_t total;
+            const uint32_t excl = block_exclusive_scan(n, s_wave, total);
+            if (fill + (int)total > gcap) {
+                flush(fill);
+                fill = 0;
+            }
+            const int base = fill + (int)excl;
+            for (uint32_t q = 0; q < n; ++q) {
+                const float cx = sx + uniform_f32(rng, a.uscale);
+                const float cy = sy + uniform_f32(rng, a.uscale);
+                float rad = a.r_const;
+                if (KIND == 1) {
+                    rad = lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
+                    if (rad > rm) rad = rm;
+                }
+                const float r2 = rad <= 0.0f ? -1.0f : rad * rad;
+                const float rz = fmaxf(rad, 0.0f) * zs + slack;
+                gA[base + q] = make_float4(cx, cy, r2, __uint_as_float(cellpk));
+                gB[base + q] = make_float4(cx * zs - 0.5f, cy * zs - 0.5f, rz, 0.0f);
+            }
+            fill += (int)total;
+            rem -= n;
+        }
+    }
+    flush(fill);
+
+    // ---- popcount epilogue: one store per output pixel ----
+    const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
+    for (int p = tid; p < tw * th; p += kBlock) {
+        const int ox = p % tw, oy = p / tw;
+        const uint32_t *pw = bits + ((size_t)oy * a.tile_w + ox) * words;
+        uint32_t count = 0;
+        for (int w = 0; w < words; ++w) count += __popc(pw[w]);
+        a.out[(size_t)(t.Y0 + oy) * a.out_w + (t.X0 + ox)] = (float)count * a.inv_samples;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void offsets_range_kernel(const float2 *offs, int n, float4 *range)
 {
     __shared__ float4 s_r[kBlock];
@@ -287,22 +513,46 @@ hipError_t launch_offsets_range(const float2 *offs, int n, float4 *range, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_pixelwise(const PixelArgs &a, int dist_kind, bool with_stats, hipStream_t s)
+hipError_t launch_pixelwise(const PixelArgs &a, int dist_kind, int variant, bool with_stats, hipStream_t s)
 {
     const dim3 grid(a.n_tiles), block(kBlock);
     const size_t lds = (size_t)a.lds_bytes;
-    if (dist_kind == 0) {
-        if (with_stats)
-            hipLaunchKernelGGL((pixelwise_tiles<0, true>), grid, block, lds, s, a);
-        else
-            hipLaunchKernelGGL((pixelwise_tiles<0, false>), grid, block, lds, s, a);
+    if (with_stats || variant == 1) {
+        if (dist_kind == 0) {
+            if (with_stats)
+                hipLaunchKernelGGL((pixelwise_gather<0, true>), grid, block, lds, s, a);
+            else
+                hipLaunchKernelGGL((pixelwise_gather<0, false>), grid, block, lds, s, a);
+        } else {
+            if (with_stats)
+                hipLaunchKernelGGL((pixelwise_gather<1, true>), grid, block, lds, s, a);
+            else
+                hipLaunchKernelGGL((pixelwise_gather<1, false>), grid, block, lds, s, a);
+        }
     } else {
-        if (with_stats)
-            hipLaunchKernelGGL((pixelwise_tiles<1, true>), grid, block, lds, s, a);
+        if (dist_kind == 0)
+            hipLaunchKernelGGL((pixelwise_splat<0>), grid, block, lds, s, a);
         else
-            hipLaunchKernelGGL((pixelwise_tiles<1, false>), grid, block, lds, s, a);
+            hipLaunchKernelGGL((pixelwise_splat<1>), grid, block, lds, s, a);
     }
     return hipGetLastError();
+}
+
+// q0 = a*rd; r = fma(-q0, d, a); q = fma(r, rd, q0) against a/d over a in [1, 2).
+bool reciprocal_division_exact(float d, float rd)
+{
+    if (!(d > 0.0f) || !std::isfinite(d) || !(rd > 0.0f) || !std::isfinite(rd)) return false;
+    uint32_t bits = 0x3F800000u;
+    for (uint32_t m = 0; m < (1u << 23); ++m, ++bits) {
+        float a;
+        std::memcpy(&a, &bits, 4);
+        const volatile float q0 = a * rd;
+        const float r = std::fma(-(float)q0, d, a);
+        const float q = std::fma(r, rd, (float)q0);
+        const volatile float exact = a / d;
+        if (q != exact) return false;
+    }
+    return true;
 }
 
 }  // namespace fg

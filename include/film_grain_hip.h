@@ -126,6 +126,7 @@ int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const floa
 typedef struct {
     int pix_tile_w, pix_tile_h, pix_lds_bytes;
     int grain_tile_w, grain_tile_h, grain_lds_bytes, grain_grains_per_thread;
+    int pix_variant; /* 0 splat (default), 1 gather (the reference's loop order) */
 } fg_launch_config;
 int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg);
 int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out);

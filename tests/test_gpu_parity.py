@@ -3,14 +3,19 @@
 Bar: pixel values are counts * (1/N), so equality is checked bit-exactly
 (max |diff| == 0).  The north-star tolerance (at most one flipped sample on
 >= 99.99% of pixels, mean |diff| <= 1e-4) is asserted as well, so a failure
-report shows both numbers.
+report shows both numbers.  The pixel-wise path has two device formulations
+(splat, the default, and gather, the reference's loop order); both are checked
+against the oracle and, at full frame size, against each other.
 """
 import numpy as np
 import pytest
 
+from film_grain_amd import synth
 from tests import fgcases
 
 pytestmark = pytest.mark.gpu
+
+VARIANTS = {"splat": 0, "gather": 1}
 
 
 def _compare(name, got, want, n):
@@ -23,25 +28,60 @@ def _compare(name, got, want, n):
     assert diff.max() == 0.0, f"{name}: not bit-exact"
 
 
-def test_pixelwise_c1_small(hip_ctx, oracle):
-    case = fgcases.c1_case(oracle, size=64, n_samples=64)
-    want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
-    got = hip_ctx.render_host("pixel", case.desc, case.lam, case.offsets_input)
-    _compare(case.name, got, want, case.n_samples)
+def _render(ctx, variant, case, algo="pixel"):
+    ctx.set_launch_config(pix_variant=VARIANTS[variant])
+    try:
+        offs = case.offsets_input if algo == "pixel" else case.offsets
+        return ctx.render_host(algo, case.desc, case.lam, offs)
+    finally:
+        ctx.reset_launch_config()
+
+
+def _pixel_cases(O):
+    yield fgcases.c1_case(O, size=64, n_samples=64)
+    yield fgcases.c2_case(O, w=48, h=32, n_samples=256)
+    u = synth.sinusoid_field(33, 21, 5, n_waves=3)
+    yield fgcases.build_case(O, "zoom1.5_lognorm_wide", u, 0.15, 0.15, "lognorm", zoom=1.5,
+                             n_samples=100, seed=99)
+    yield fgcases.build_case(O, "n1", u, 0.2, n_samples=1, seed=3)
+    yield fgcases.build_case(O, "n7_zoom3", u[:9, :11], 0.1, n_samples=7, zoom=3.0, seed=4)
+    yield fgcases.build_case(O, "n300", u[:12, :16], 0.1, 0.02, "lognorm", n_samples=300, seed=5)
+    yield fgcases.build_case(O, "tiny_1x1", np.full((1, 1), 0.6, np.float32), 0.1, n_samples=16,
+                             zoom=4.0, seed=6)
+    yield fgcases.build_case(O, "cell_0.05_absrm", u[:16, :16], 0.2, n_samples=32, seed=8,
+                             cell_delta=0.05, max_radius=("absolute", 0.35))
+    yield fgcases.build_case(O, "dense_0.999", np.full((12, 12), 0.999, np.float32), 0.1, 0.05,
+                             "lognorm", n_samples=64, zoom=2.0, seed=9)
+    yield fgcases.build_case(O, "sigma3_zoom0.5", u, 0.3, n_samples=24, zoom=0.5, sigma=3.0, seed=10)
+    yield fgcases.build_case(O, "big_lambda_rejection", np.full((6, 6), 0.99, np.float32), 0.05,
+                             n_samples=16, seed=12, cell_delta=1.0)
+
+
+@pytest.mark.parametrize("variant", ["splat", "gather"])
+def test_pixelwise_cases(hip_ctx, oracle, variant):
+    for case in _pixel_cases(oracle):
+        want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
+        got = _render(hip_ctx, variant, case)
+        _compare(f"{variant}:{case.name}", got, want, case.n_samples)
 
 
 def test_pixelwise_c1_full_256(hip_ctx, oracle):
     case = fgcases.c1_case(oracle, size=256, n_samples=64)
     want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
-    got = hip_ctx.render_host("pixel", case.desc, case.lam, case.offsets_input)
-    _compare(case.name, got, want, case.n_samples)
+    _compare(case.name, _render(hip_ctx, "splat", case), want, case.n_samples)
 
 
-def test_pixelwise_c2_lognormal_zoom2(hip_ctx, oracle):
-    case = fgcases.c2_case(oracle, w=48, h=32, n_samples=256)
-    want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
-    got = hip_ctx.render_host("pixel", case.desc, case.lam, case.offsets_input)
-    _compare(case.name, got, want, case.n_samples)
+def test_pixelwise_c2_full_frame_splat_equals_gather_and_oracle_tiles(hip_ctx, oracle):
+    """Full 1920x1080 -> 3840x2160 plane: the two device formulations agree on
+    every pixel, and random 16x16 tiles agree with the oracle."""
+    case = fgcases.c2_case(oracle, w=1920, h=1080, n_samples=256, channel=1)
+    a = _render(hip_ctx, "splat", case)
+    b = _render(hip_ctx, "gather", case)
+    _compare("c2_full:splat_vs_gather", a, b, case.n_samples)
+    rng = np.random.default_rng(5)
+    for (x0, y0, x1, y1) in fgcases.tiles(rng, case.derived.out_w, case.derived.out_h, 16, 24):
+        want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input, region=(x0, y0, x1, y1))
+        _compare(f"c2_full:tile{x0},{y0}", a[y0:y1, x0:x1], want, case.n_samples)
 
 
 def test_grainwise_c3_small(hip_ctx, oracle):
@@ -52,9 +92,25 @@ def test_grainwise_c3_small(hip_ctx, oracle):
 
 
 def test_grainwise_lognormal_zoom2(hip_ctx, oracle):
-    u = fgcases.synth.sinusoid_field(40, 32, 3, n_waves=4)
+    u = synth.sinusoid_field(40, 32, 3, n_waves=4)
     case = fgcases.build_case(oracle, "grain_lognorm_z2", u, 0.2, 0.1, "lognorm", zoom=2.0,
                               n_samples=96, seed=3)
     want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
     got = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
     _compare(case.name, got, want, case.n_samples)
+
+
+def test_grainwise_edge_cases(hip_ctx, oracle):
+    u = synth.sinusoid_field(23, 17, 8, n_waves=3)
+    cases = [
+        fgcases.build_case(oracle, "g_n1", u, 0.3, n_samples=1, seed=1),
+        fgcases.build_case(oracle, "g_n300_zoom1.5", u[:10, :10], 0.1, 0.05, "lognorm", n_samples=300,
+                           zoom=1.5, seed=2),
+        fgcases.build_case(oracle, "g_tiny", np.full((2, 3), 0.7, np.float32), 0.4, n_samples=40,
+                           zoom=3.0, seed=3),
+        fgcases.build_case(oracle, "g_zoom0.5", u, 0.5, n_samples=33, zoom=0.5, sigma=2.0, seed=4),
+    ]
+    for case in cases:
+        want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
+        got = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
+        _compare(case.name, got, want, case.n_samples)
