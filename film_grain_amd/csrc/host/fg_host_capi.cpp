@@ -200,6 +200,48 @@ int fgh_render_file(const fgh_cli_args *args, fgh_derived *stats, char *err, siz
     });
 }
 
+int fgh_color_roundtrip(const float *rgb, uint32_t w, uint32_t h, uint32_t color_mode, uint8_t *out_rgb8)
+{
+    return guarded(nullptr, 0, [&]() {
+        const Workspace ws = Workspace::from_rgb_f32(rgb, w, h, color_mode == 1 ? ColorMode::Rgb : ColorMode::Luma);
+        const RgbImage img = ws.into_rgb_image();
+        std::memcpy(out_rgb8, img.data.data(), img.data.size());
+        return FGH_OK;
+    });
+}
+
+int fgh_load_image(const char *path, uint32_t *w, uint32_t *h, float *out_rgb, size_t cap, char *err,
+                   size_t errlen)
+{
+    return guarded(err, errlen, [&]() {
+        size_t iw = 0, ih = 0;
+        const std::vector<float> rgb = load_image_rgb_f32(path, &iw, &ih);
+        *w = (uint32_t)iw;
+        *h = (uint32_t)ih;
+        if (out_rgb) {
+            if (cap < rgb.size()) {
+                put_err(err, errlen, "image buffer too small");
+                return FGH_ECAPACITY;
+            }
+            std::memcpy(out_rgb, rgb.data(), rgb.size() * sizeof(float));
+        }
+        return FGH_OK;
+    });
+}
+
+int fgh_save_rgb8(const char *path, const uint8_t *rgb8, uint32_t w, uint32_t h, const char *format, char *err,
+                  size_t errlen)
+{
+    return guarded(err, errlen, [&]() {
+        RgbImage img;
+        img.width = w;
+        img.height = h;
+        img.data.assign(rgb8, rgb8 + (size_t)w * h * 3);
+        save_image(img, path, format ? format : "png");
+        return FGH_OK;
+    });
+}
+
 int fgh_dry_run(const fgh_cli_args *args, fgh_derived *stats, char *err, size_t errlen)
 {
     return guarded(err, errlen, [&]() {

@@ -25,7 +25,8 @@ FGH_EMESSAGE = 16
 FGH_ECAPACITY = 17
 
 EXPORTED = ["fgh_derive", "fgh_make_offsets", "fgh_normalize_plane", "fgh_lambda_plane",
-            "fgh_inverse_normal_cdf", "fgh_render_rgb", "fgh_render_file", "fgh_dry_run"]
+            "fgh_inverse_normal_cdf", "fgh_render_rgb", "fgh_render_file", "fgh_dry_run",
+            "fgh_color_roundtrip", "fgh_load_image", "fgh_save_rgb8"]
 
 
 class CliArgsC(C.Structure):
@@ -123,8 +124,51 @@ def _setup():
     L.fgh_render_file.restype = C.c_int
     L.fgh_dry_run.argtypes = [C.POINTER(CliArgsC), C.POINTER(DerivedC), C.c_char_p, C.c_size_t]
     L.fgh_dry_run.restype = C.c_int
+    L.fgh_color_roundtrip.argtypes = [fp, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]
+    L.fgh_color_roundtrip.restype = C.c_int
+    L.fgh_load_image.argtypes = [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), fp, C.c_size_t,
+                                 C.c_char_p, C.c_size_t]
+    L.fgh_load_image.restype = C.c_int
+    L.fgh_save_rgb8.argtypes = [C.c_char_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32, C.c_char_p,
+                                C.c_char_p, C.c_size_t]
+    L.fgh_save_rgb8.restype = C.c_int
     L._fgh_ready = True
     return L
+
+
+def color_roundtrip(rgb: np.ndarray, color_mode: str) -> np.ndarray:
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    out = np.zeros((h, w, 3), dtype=np.uint8)
+    rc = _setup().fgh_color_roundtrip(_f32(rgb), w, h, 1 if color_mode == "rgb" else 0,
+                                      out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != FGH_OK:
+        raise RenderError(rc, "colour round trip failed")
+    return out
+
+
+def load_image(path: str) -> np.ndarray:
+    L = _setup()
+    w, h = C.c_uint32(0), C.c_uint32(0)
+    err = C.create_string_buffer(512)
+    rc = L.fgh_load_image(path.encode(), C.byref(w), C.byref(h), None, 0, err, 512)
+    if rc != FGH_OK:
+        raise RenderError(rc, err.value.decode())
+    out = np.zeros((h.value, w.value, 3), dtype=np.float32)
+    rc = L.fgh_load_image(path.encode(), C.byref(w), C.byref(h), _f32(out), out.size, err, 512)
+    if rc != FGH_OK:
+        raise RenderError(rc, err.value.decode())
+    return out
+
+
+def save_rgb8(path: str, rgb8: np.ndarray, fmt: str = "png") -> None:
+    rgb8 = np.ascontiguousarray(rgb8, dtype=np.uint8)
+    h, w = rgb8.shape[:2]
+    err = C.create_string_buffer(512)
+    rc = _setup().fgh_save_rgb8(path.encode(), rgb8.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, fmt.encode(),
+                                err, 512)
+    if rc != FGH_OK:
+        raise RenderError(rc, err.value.decode())
 
 
 def _f32(a):

@@ -86,6 +86,19 @@ int fgh_render_file(const fgh_cli_args *args, fgh_derived *stats, char *err, siz
 /* ~ dry_run (src/lib.rs:95-98). */
 int fgh_dry_run(const fgh_cli_args *args, fgh_derived *stats, char *err, size_t errlen);
 
+/* Colour conversions alone: Workspace from an RGB f32 image (luma: BT.709
+ * Y/Cb/Cr split; rgb: three planes) then into_rgb_image, with no rendering in
+ * between (src/color.rs:66-114, 158-213).  out_rgb8: w*h*3 bytes. */
+int fgh_color_roundtrip(const float *rgb, uint32_t w, uint32_t h, uint32_t color_mode,
+                        uint8_t *out_rgb8);
+/* Reads PNG / PGM / PPM / PFM as RGB f32 (image::to_rgb32f semantics).  With
+ * out_rgb NULL only the size is returned. */
+int fgh_load_image(const char *path, uint32_t *w, uint32_t *h, float *out_rgb, size_t cap, char *err,
+                   size_t errlen);
+/* Writes an RGB8 image as "png" or "ppm". */
+int fgh_save_rgb8(const char *path, const uint8_t *rgb8, uint32_t w, uint32_t h, const char *format,
+                  char *err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif
