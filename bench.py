@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
+    ap.add_argument("--pix-variant", type=int, default=None,
+                    help="pixel-wise formulation: 1 gather, 2 tile splat, 3 HBM splat")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -125,7 +128,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from film_grain_amd import capi, host
+    from film_grain_amd import autotune, capi, host
 
     wl = WORKLOADS[args.workload]
     log(f"[bench] workload {args.workload}: {wl['desc']}")
@@ -149,8 +152,6 @@ def main():
     offs_d = torch.from_numpy(np.ascontiguousarray(offsets)).to(dev)
     out_d = torch.empty((len(lams), d.out_h, d.out_w), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    cfg = ctx.launch_config(desc, algo)
-    log(f"[bench] launch config {cfg}")
 
     # instrumented pass: algorithmic work of the frame
     counters = []
@@ -158,6 +159,13 @@ def main():
         counters.append(ctx.stats_device(algo, desc, l, offs_d, out_d[i], stream))
     torch.cuda.synchronize()
     log(f"[bench] counters per plane {counters}")
+
+    tuned = None if args.no_tuning else autotune.apply_table(ctx, algo, desc)
+    if args.pix_variant is not None:
+        ctx.set_launch_config(pix_variant=args.pix_variant)
+        tuned = None
+    cfg = ctx.launch_config(desc, algo)
+    log(f"[bench] launch config {cfg} (tuned table entry: {tuned is not None})")
 
     def step():
         for i, l in enumerate(lam_d):
@@ -188,23 +196,40 @@ def main():
     log(f"[bench] {args.steps} steps in {secs:.4f} s; launch ms mean {avg_launch_ms:.3f} "
         f"min {min(launch_ms):.3f} max {max(launch_ms):.3f}")
 
+    # kernel timing pass: HIP events recorded by the library on the launch stream
+    # around the generation kernels and around the coverage-test kernel
+    ctx.set_timing(True)
+    kt = []
+    for _ in range(max(2, args.steps)):
+        for i, l in enumerate(lam_d):
+            ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
+            kt.append(ctx.timings())
+    ctx.set_timing(False)
+    ctx.stream_status(stream)
+    kt = np.array(kt)
+    render_ms, gen_ms, test_ms = (float(v) for v in kt.mean(axis=0))
+    log(f"[bench] per plane: render {render_ms:.3f} ms = generation {gen_ms:.3f} + test kernel {test_ms:.3f}")
+
     px_samples = d.out_w * d.out_h * len(lam_d) * d.n_samples
     value = px_samples * args.steps / secs / 1e6
+    kernel_name = {1: "pixelwise_gather", 2: "pixelwise_splat", 3: "splat_hbm_kernel"}.get(cfg.get("pix_variant"), "")
     if algo == "pixel":
         flops = [16.0 * c[0] + 6.0 * c[1] for c in counters]
-        achieved = float(np.mean(flops)) / (avg_launch_ms * 1e-3) / 1e12
+        achieved = float(np.mean(flops)) / (test_ms * 1e-3) / 1e12
         roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                         frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
-                        kernel="pixelwise_tiles<lognormal>",
+                        kernel=kernel_name, kernel_ms=round(test_ms, 3),
+                        generation_ms=round(gen_ms, 3), render_ms=round(render_ms, 3),
                         flops_per_launch=float(np.mean(flops)),
                         tests_per_point=round(sum(c[1] for c in counters) / sum(c[0] for c in counters), 4))
     else:
         # algorithmic bytes per output pixel: lambda read 4 + output write 4 + N/4 (bitset clear + read)
         bytes_per_launch = d.out_w * d.out_h * (8 + d.n_samples / 4.0)
-        achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+        achieved = bytes_per_launch / (test_ms * 1e-3) / 1e9
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
                         frac=round(achieved / PEAK_HBM_GBS, 6), traffic=None,
-                        kernel="grainwise_tiles<const>", bytes_per_launch=bytes_per_launch,
+                        kernel="grainwise_tiles", kernel_ms=round(test_ms, 3),
+                        bytes_per_launch=bytes_per_launch,
                         pairs_per_launch=float(np.mean([c[1] for c in counters])))
 
     cpu = None

@@ -30,7 +30,7 @@ EXPORTED = [
     "fg_ctx_create", "fg_ctx_destroy", "fg_last_error", "fg_render_pixelwise",
     "fg_render_grainwise", "fg_render_pixelwise_device", "fg_render_grainwise_device",
     "fg_stream_status", "fg_pixelwise_stats_device", "fg_grainwise_stats_device",
-    "fg_set_launch_config", "fg_get_launch_config", "fg_version",
+    "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings", "fg_version",
 ]
 
 
@@ -47,7 +47,7 @@ class LaunchConfig(C.Structure):
     _fields_ = [
         ("pix_tile_w", C.c_int), ("pix_tile_h", C.c_int), ("pix_lds_bytes", C.c_int),
         ("grain_tile_w", C.c_int), ("grain_tile_h", C.c_int), ("grain_lds_bytes", C.c_int),
-        ("grain_grains_per_thread", C.c_int), ("pix_variant", C.c_int),
+        ("grain_grains_per_thread", C.c_int), ("pix_variant", C.c_int), ("pix_block", C.c_int),
     ]
 
 
@@ -99,6 +99,10 @@ def lib() -> C.CDLL:
     L.fg_set_launch_config.restype = C.c_int
     L.fg_get_launch_config.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchConfig)]
     L.fg_get_launch_config.restype = C.c_int
+    L.fg_set_timing.argtypes = [vp, C.c_int]
+    L.fg_set_timing.restype = C.c_int
+    L.fg_get_timings.argtypes = [vp, C.POINTER(C.c_float)]
+    L.fg_get_timings.restype = C.c_int
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
     _lib = L
@@ -176,6 +180,16 @@ class Context:
                 C.c_void_p(out.data_ptr()), cnt, C.c_void_p(_stream_handle(stream)))
         self._check(rc)
         return [int(v) for v in cnt]
+
+    def set_timing(self, enable: bool):
+        self._check(self._L.fg_set_timing(self.handle, 1 if enable else 0))
+
+    def timings(self):
+        """(whole render, generation kernels, coverage-test kernel) in ms for
+        the last render since timing was enabled."""
+        ms = (C.c_float * 3)()
+        self._check(self._L.fg_get_timings(self.handle, ms))
+        return tuple(float(v) for v in ms)
 
     def set_launch_config(self, **kw):
         cfg = LaunchConfig()

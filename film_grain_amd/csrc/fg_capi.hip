@@ -17,7 +17,14 @@
 
 #include "../../include/film_grain_hip.h"
 #include "fg_kernels.h"
+#include "fg_pixelwise_hbm.h"
 #include "fg_rng.h"
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
 
 struct fg_ctx {
     int device = 0;
@@ -37,6 +44,15 @@ struct fg_ctx {
     int *d_status = nullptr;
     unsigned long long *d_stats = nullptr;
     float4 *d_range = nullptr;
+    // scratch of the per-plane grain generation: Poisson thresholds, bin
+    // counts and starts, scan workspace, grain records
+    DevBuf thresholds, bin_count, bin_start, scan_temp, grains;
+    hipEvent_t scratch_done = nullptr;  // recorded after the last kernel that reads the scratch
+    bool scratch_pending = false;
+    // optional kernel timing: start, generation done, test kernel start, end
+    bool timing = false;
+    bool timed = false;
+    hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
     fg_launch_config cfg{};
     std::map<uint32_t, bool> div_checked;  // cell size -> reciprocal division verified
 };
@@ -59,6 +75,13 @@ int hip_error(fg_ctx *ctx, hipError_t e, const char *what)
     return set_error(ctx, FG_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void free_buf(DevBuf &b)
+{
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
 void free_buffers(fg_ctx *ctx)
 {
     (void)hipFree(ctx->d_lambda);
@@ -66,6 +89,53 @@ void free_buffers(fg_ctx *ctx)
     (void)hipFree(ctx->d_out);
     ctx->d_lambda = ctx->d_offs = ctx->d_out = nullptr;
     ctx->cap_lambda = ctx->cap_offs = ctx->cap_out = 0;
+    free_buf(ctx->thresholds);
+    free_buf(ctx->bin_count);
+    free_buf(ctx->bin_start);
+    free_buf(ctx->scan_temp);
+    free_buf(ctx->grains);
+}
+
+int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
+{
+    if (b.cap >= bytes) return FG_OK;
+    free_buf(b);
+    const size_t want = std::max<size_t>(bytes + bytes / 8, 256);  // headroom for the next plane
+    const hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        free_buffers(ctx);
+        (void)hipGetLastError();
+        return hip_error(ctx, e, what);
+    }
+    b.cap = want;
+    return FG_OK;
+}
+
+// The generation scratch is shared by every call on the context; a call waits
+// until the previous user's last kernel has finished before overwriting it.
+int scratch_acquire(fg_ctx *ctx, const char *what)
+{
+    if (!ctx->scratch_pending) return FG_OK;
+    const hipError_t e = hipEventSynchronize(ctx->scratch_done);
+    ctx->scratch_pending = false;
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, what);
+}
+
+void mark(fg_ctx *ctx, int i, hipStream_t s)
+{
+    if (!ctx->timing) return;
+    if (!ctx->tev[i]) (void)hipEventCreate(&ctx->tev[i]);
+    (void)hipEventRecord(ctx->tev[i], s);
+    if (i == 3) ctx->timed = true;
+}
+
+int scratch_release(fg_ctx *ctx, hipStream_t s, const char *what)
+{
+    const hipError_t e = hipEventRecord(ctx->scratch_done, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    ctx->scratch_pending = true;
+    return FG_OK;
 }
 
 int ensure(fg_ctx *ctx, float **buf, size_t *cap, size_t n_floats, const char *what)
@@ -140,7 +210,7 @@ void choose_splat_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
     const size_t words = (d->n_samples + 31) / 32;
     for (const auto &t : ladder) {
         const size_t bits = (size_t)t[0] * t[1] * words * 4;
-        if (bits <= (size_t)lds / 2 && bits + 2048 * 32 <= (size_t)lds + 16384) {
+        if (bits <= (size_t)lds / 2 && bits + 1024 * 32 <= (size_t)lds) {
             *tw = t[0];
             *th = t[1];
             return;
@@ -150,15 +220,46 @@ void choose_splat_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
     *th = 1;
 }
 
+// Pixel-wise formulation: 1 gather (the reference's loop order), 2 splat with
+// in-tile generation, 3 splat from grains generated once per plane in HBM.
+// The automatic choice (0) is 3; the caller falls back to 2 when the plane's
+// cell grid is too large to pack cell indices in 16 bits.
 int pixel_variant(fg_ctx *ctx, const fg_plane_desc *d)
 {
     if (ctx->cfg.pix_variant == 1 || d->n_samples > 65535) return 1;
-    return 0;
+    if (ctx->cfg.pix_variant == 2) return 2;
+    return 3;
 }
+
+int splat_hbm_block(const fg_ctx *ctx) { return ctx->cfg.pix_block == 256 ? 256 : 512; }
 
 void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int variant, int *tw, int *th, int *lds)
 {
-    const int def = variant == 1 ? 64 * 1024 : 72 * 1024;
+    if (variant == 3 && ctx->cfg.pix_tile_w == 0 && ctx->cfg.pix_lds_bytes == 0) {
+        // Measured on MI355X (4K RGB, N = 256): occupancy decides, so keep the
+        // workgroup's LDS small -- a bitset of at most 16 KB, the offsets table
+        // and one grain record per thread.
+        static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
+                                        {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
+        const size_t words = (d->n_samples + 31) / 32;
+        *tw = 1;
+        *th = 1;
+        for (const auto &t : ladder) {
+            if ((size_t)t[0] * t[1] * words * 4 <= 16384) {
+                *tw = t[0];
+                *th = t[1];
+                break;
+            }
+        }
+        const size_t bits = (((size_t)(*tw) * (*th) * words * 4) + 15) & ~(size_t)15;
+        const size_t offs = d->n_samples <= 2048 ? (((size_t)d->n_samples * 8 + 15) & ~(size_t)15) : 0;
+        *lds = (int)std::min<size_t>(bits + offs + (size_t)splat_hbm_block(ctx) * 32 + 512,
+                                     (size_t)ctx->max_lds - 6 * 1024);
+        return;
+    }
+    // the HBM splat kernel holds no generation state, so smaller LDS blocks
+    // buy more resident workgroups
+    const int def = variant == 1 ? 64 * 1024 : (variant == 3 ? 48 * 1024 : 72 * 1024);
     const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : def;
     *lds = std::min(budget, ctx->max_lds - 6 * 1024);
     const int max_px = variant == 1 ? fg::kMaxTilePixels : 256 * 256;
@@ -215,10 +316,98 @@ void grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *ld
     *lds = (int)(bits + slots);
 }
 
+// floor(RN(a / d)) as the kernels compute it (they use the reciprocal
+// sequence only where it equals IEEE division).
+int host_floor_div(float a, float d)
+{
+    const float q = std::floor(a / d);
+    if (!(q == q)) return 0;
+    if (q >= 2147483647.0f) return 2147483647;
+    if (q <= -2147483648.0f) return (int)0x80000000u;
+    return (int)q;
+}
+
+// Extended cell grid of a plane: the union of all windows (every step from a
+// pixel index to a window bound is monotone, so the first and last pixel with
+// the extreme offsets bound it).  Same float operations as the kernels.
+bool plane_geometry(const fg_plane_desc *d, const float4 &r, fg::PixGeom *g)
+{
+    const float iz = 1.0f / d->zoom, rm = d->rm, delta = d->delta;
+    const float px_lo = (0.0f + 0.5f) * iz, px_hi = ((float)(d->out_w - 1) + 0.5f) * iz;
+    const float py_lo = (0.0f + 0.5f) * iz, py_hi = ((float)(d->out_h - 1) + 0.5f) * iz;
+    const long long I0 = host_floor_div((px_lo - r.y) - rm, delta);
+    const long long I1 = host_floor_div((px_hi - r.x) + rm, delta);
+    const long long J0 = host_floor_div((py_lo - r.w) - rm, delta);
+    const long long J1 = host_floor_div((py_hi - r.z) + rm, delta);
+    const long long W = I1 - I0 + 1, H = J1 - J0 + 1;
+    if (W <= 0 || H <= 0 || W > 65535 || H > 65535) return false;
+    const long long nbx = (W + fg::kBinSide - 1) / fg::kBinSide, nby = (H + fg::kBinSide - 1) / fg::kBinSide;
+    if (nbx * nby >= 0x7fffffffLL) return false;
+    g->Ix0 = (int)I0;
+    g->Jy0 = (int)J0;
+    g->W = (int)W;
+    g->H = (int)H;
+    g->nbx = (int)nbx;
+    g->nby = (int)nby;
+    return true;
+}
+
+// Per-plane generation into HBM, then the splat-from-HBM kernel.
+int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, int dist_kind, hipStream_t s)
+{
+    const char *what = "Pixel renderer";
+    int rc = scratch_acquire(ctx, what);
+    if (rc != FG_OK) return rc;
+    const long long nbins = (long long)g.nbx * g.nby;
+    const size_t npx = (size_t)a.in_w * a.in_h;
+    if ((rc = ensure_bytes(ctx, ctx->thresholds, npx * sizeof(double), what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->bin_count, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->bin_start, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
+    size_t temp_bytes = 0;
+    hipError_t e = fg::scan_bins(nullptr, &temp_bytes, nullptr, nullptr, (int)(nbins + 1), s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = ensure_bytes(ctx, ctx->scan_temp, temp_bytes, what)) != FG_OK) return rc;
+    double *T = static_cast<double *>(ctx->thresholds.p);
+    uint32_t *cnt = static_cast<uint32_t *>(ctx->bin_count.p);
+    uint32_t *start = static_cast<uint32_t *>(ctx->bin_start.p);
+    mark(ctx, 0, s);
+    e = fg::launch_threshold(a, T, s);
+    if (e == hipSuccess) e = hipMemsetAsync(cnt + nbins, 0, 4, s);
+    if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, s);
+    temp_bytes = ctx->scan_temp.cap;
+    if (e == hipSuccess) e = fg::scan_bins(ctx->scan_temp.p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
+    uint32_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, start + nbins, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = ensure_bytes(ctx, ctx->grains, std::max<size_t>(total, 1) * sizeof(float4), what)) != FG_OK)
+        return rc;
+    float4 *grains = static_cast<float4 *>(ctx->grains.p);
+    e = fg::launch_bin_grains(a, g, dist_kind, T, start, grains, s);
+    mark(ctx, 1, s);
+    mark(ctx, 2, s);
+    const int block = splat_hbm_block(ctx);
+    if (e == hipSuccess) e = fg::launch_splat_hbm(a, g, start, grains, block, s);
+    mark(ctx, 3, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    return scratch_release(ctx, s, what);
+}
+
 int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
                  float *d_out, hipStream_t s, bool stats)
 {
-    const int variant = stats ? 1 : pixel_variant(ctx, d);
+    int variant = stats ? 1 : pixel_variant(ctx, d);
+    fg::PixGeom geom{};
+    hipError_t e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offs), (int)d->n_samples,
+                                            ctx->d_range, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    if (variant == 3) {
+        float4 r;
+        e = hipMemcpyAsync(&r, ctx->d_range, sizeof r, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+        if (!plane_geometry(d, r, &geom)) variant = 2;
+    }
     int tw, th, lds;
     pixel_config(ctx, d, variant, &tw, &th, &lds);
     fg::PixelArgs a{};
@@ -246,9 +435,12 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.lds_bytes = lds;
     a.words = (int)((d->n_samples + 31) / 32);
     const size_t bits = (((size_t)tw * th * a.words * 4) + 15) & ~(size_t)15;
-    a.offs_in_lds = (variant == 0 && d->n_samples <= 2048 &&
-                     bits + (size_t)d->n_samples * 8 + 256 * 32 <= (size_t)lds) ? 1 : 0;
-    if (variant == 0 && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + 256 * 32 > (size_t)lds)
+    const bool splat = variant != 1;
+    // the splat kernels need room for one grain record (32 B) per thread
+    const size_t min_grains = variant == 3 ? (size_t)splat_hbm_block(ctx) : 256;
+    a.offs_in_lds = (splat && d->n_samples <= 2048 &&
+                     bits + (size_t)d->n_samples * 8 + min_grains * 32 <= (size_t)lds) ? 1 : 0;
+    if (splat && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + min_grains * 32 > (size_t)lds)
         return set_error(ctx, FG_EINVAL, "sample count too large for one pixel-wise tile");
     a.inv_zoom = 1.0f / d->zoom;
     a.zoom = d->zoom;
@@ -262,9 +454,12 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.r_const = rc;
     a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
-    hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
-    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
-    e = fg::launch_pixelwise(a, (int)d->dist_kind, variant, stats, s);
+    if (variant == 3) return run_pixel_hbm(ctx, a, geom, (int)d->dist_kind, s);
+    mark(ctx, 0, s);
+    mark(ctx, 1, s);
+    mark(ctx, 2, s);
+    e = fg::launch_pixelwise(a, (int)d->dist_kind, variant == 1 ? 1 : 0, stats, s);
+    mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     return FG_OK;
 }
@@ -307,7 +502,11 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.off_range = ctx->d_range;
     hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    mark(ctx, 0, s);
+    mark(ctx, 1, s);
+    mark(ctx, 2, s);
     e = fg::launch_grainwise(a, (int)d->dist_kind, stats, s);
+    mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
     return FG_OK;
 }
@@ -418,6 +617,7 @@ int fg_ctx_create(int device_ordinal, fg_ctx **out)
     if (e == hipSuccess) e = hipMalloc(&ctx->d_status, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_range, sizeof(float4));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, sizeof(int));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -437,6 +637,9 @@ void fg_ctx_destroy(fg_ctx *ctx)
     (void)hipFree(ctx->d_status);
     (void)hipFree(ctx->d_stats);
     (void)hipFree(ctx->d_range);
+    if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
+    for (hipEvent_t ev : ctx->tev)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -518,7 +721,8 @@ int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg)
             cfg->pix_tile_w > 256 || cfg->pix_tile_h > 256 || cfg->grain_tile_w < 0 ||
             cfg->grain_tile_h < 0 || cfg->grain_grains_per_thread < 0 ||
             cfg->grain_grains_per_thread > 64 || cfg->pix_lds_bytes < 0 || cfg->grain_lds_bytes < 0 ||
-            cfg->pix_variant < 0 || cfg->pix_variant > 1)
+            cfg->pix_variant < 0 || cfg->pix_variant > 3 ||
+            (cfg->pix_block != 0 && cfg->pix_block != 256 && cfg->pix_block != 512))
             return set_error(ctx, FG_EINVAL, "invalid launch configuration");
         ctx->cfg = *cfg;
     } else {
@@ -539,9 +743,32 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
                      &out->grain_grains_per_thread);
     } else {
         out->pix_variant = pixel_variant(ctx, desc);
+        out->pix_block = splat_hbm_block(ctx);
         pixel_config(ctx, desc, out->pix_variant, &out->pix_tile_w, &out->pix_tile_h,
                      &out->pix_lds_bytes);
     }
+    return FG_OK;
+}
+
+int fg_set_timing(fg_ctx *ctx, int enable)
+{
+    if (!ctx) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->timing = enable != 0;
+    ctx->timed = false;
+    return FG_OK;
+}
+
+int fg_get_timings(fg_ctx *ctx, float ms[3])
+{
+    if (!ctx || !ms) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->timed) return set_error(ctx, FG_EINVAL, "no timed render on this context");
+    hipError_t e = hipEventSynchronize(ctx->tev[3]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ctx->tev[0], ctx->tev[3]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ctx->tev[0], ctx->tev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], ctx->tev[2], ctx->tev[3]);
+    if (e != hipSuccess) return hip_error(ctx, e, "timing");
     return FG_OK;
 }
 

@@ -95,6 +95,81 @@ __device__ __forceinline__ uint32_t cell_count(uint64_t h, double e, Xoshiro &rn
     return (uint32_t)r;
 }
 
+// ---------------------------------------------------------------------------
+// splat drain, shared by the tile and HBM splat kernels
+// ---------------------------------------------------------------------------
+struct SplatTile {
+    int X0, Y0, X1, Y1;  // output rect
+    int I0, J0;          // region origin (cell indices)
+    int TW;              // bitset row pitch (tile_w)
+    float fx0, fx1, fy0, fy1;  // X0, X1-1, Y0, Y1-1 as floats
+};
+
+// Candidate box of grain record B = (cx*zoom - 0.5, cy*zoom - 0.5, r*zoom +
+// slack) for sample k with scaled offset (oxz, oyz): the output pixels whose
+// sample point can lie in the grain's disk.  The phase that queues candidates
+// and the drain that tests them evaluate this with the same operations, so
+// they agree on every box.
+struct SplatBox {
+    float xl, xh, yl, yh;
+};
+__device__ __forceinline__ SplatBox splat_box(const float4 &B, float oxz, float oyz)
+{
+    const float cx = B.x + oxz, cy = B.y + oyz;
+    SplatBox b;
+    b.xl = ceilf(cx - B.z);
+    b.xh = floorf(cx + B.z);
+    b.yl = ceilf(cy - B.z);
+    b.yh = floorf(cy + B.z);
+    return b;
+}
+// The box is non-empty and meets the tile.
+__device__ __forceinline__ bool splat_box_hits(const SplatBox &b, const SplatTile &t)
+{
+    return b.xl <= b.xh && b.xl <= t.fx1 && b.xh >= t.fx0 && b.yl <= b.yh && b.yl <= t.fy1 &&
+           b.yh >= t.fy0;
+}
+
+// Drains the first n (<= 64) entries (grain | k << 16) of this wave's queue
+// through the exact reference test and moves entries [64, count) to the front.
+__device__ __forceinline__ void splat_drain(const PixelArgs &a, const SplatTile &t, uint32_t *queue,
+                                            int n, int count, const float4 *gA, const float4 *gB,
+                                            const float2 *offs, uint32_t *bits)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t e = 0;
+    if (lane < n) e = queue[lane];
+    if (lane + 64 < count) queue[lane] = queue[lane + 64];
+    if (lane >= n) return;
+    const uint32_t g = e & 0xFFFFu;
+    const int k = (int)(e >> 16);
+    const float2 o = offs[k];
+    const SplatBox b = splat_box(gB[g], o.x * a.zoom, o.y * a.zoom);
+    const int xlo = (int)fmaxf(b.xl, t.fx0), xhi = (int)fminf(b.xh, t.fx1);
+    const int ylo = (int)fmaxf(b.yl, t.fy0), yhi = (int)fminf(b.yh, t.fy1);
+    const float4 G = gA[g];
+    const uint32_t cell = __float_as_uint(G.w);
+    const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
+    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
+    const int fast = a.div_fast;
+    for (int y = ylo; y <= yhi; ++y) {
+        const float yg = ((float)y + 0.5f) * iz - o.y;
+        const float dy = yg - G.y;
+        const float dy2 = dy * dy;
+        for (int x = xlo; x <= xhi; ++x) {
+            const float xg = ((float)x + 0.5f) * iz - o.x;
+            const float dx = xg - G.x;
+            if (dx * dx + dy2 <= G.z) {
+                // the grain's cell must lie in the point's window
+                if (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
+                    floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)) {
+                    atomicOr(&bits[((y - t.Y0) * t.TW + (x - t.X0)) * a.words + (k >> 5)], 1u << (k & 31));
+                }
+            }
+        }
+    }
+}
+
 }  // namespace fg
 
 #endif

@@ -220,57 +220,11 @@ __global__ __launch_bounds__(kBlock) void pixelwise_gather(PixelArgs a)
 // ---------------------------------------------------------------------------
 // splat formulation
 // ---------------------------------------------------------------------------
-struct SplatTile {
-    int X0, Y0, X1, Y1;  // output rect
-    int I0, J0;          // region origin (cell indices)
-    int TW;              // bitset row pitch (tile_w)
-    float fx0, fx1, fy0, fy1;  // X0, X1-1, Y0, Y1-1 as floats
-};
-
-// Drains the first n (<= 64) entries of this wave's queue through the exact
-// reference test and moves entries [64, count) down to the front.
-__device__ __forceinline__ void splat_drain(const PixelArgs &a, const SplatTile &t, uint2 *queue,
-                                            int n, int count, const float4 *gA,
-                                            const float2 *offs, uint32_t *bits)
-{
-    const int lane = threadIdx.x & 63;
-    uint2 e = make_uint2(0u, 0u);
-    if (lane < n) e = queue[lane];
-    if (lane + 64 < count) queue[lane] = queue[lane + 64];
-    if (lane >= n) return;
-    const uint32_t g = e.x & 0xFFFFu;
-    const int xhi = t.X0 + (int)((e.x >> 16) & 0xFFu), yhi = t.Y0 + (int)(e.x >> 24);
-    const int k = (int)(e.y & 0xFFFFu);
-    const int xlo = t.X0 + (int)((e.y >> 16) & 0xFFu), ylo = t.Y0 + (int)(e.y >> 24);
-    const float4 G = gA[g];
-    const uint32_t cell = __float_as_uint(G.w);
-    const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
-    const float2 o = offs[k];
-    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
-    const int fast = a.div_fast;
-    for (int y = ylo; y <= yhi; ++y) {
-        const float yg = ((float)y + 0.5f) * iz - o.y;
-        const float dy = yg - G.y;
-        const float dy2 = dy * dy;
-        for (int x = xlo; x <= xhi; ++x) {
-            const float xg = ((float)x + 0.5f) * iz - o.x;
-            const float dx = xg - G.x;
-            if (dx * dx + dy2 <= G.z) {
-                // the grain's cell must lie in the point's window
-                if (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
-                    floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)) {
-                    atomicOr(&bits[((y - t.Y0) * t.TW + (x - t.X0)) * a.words + (k >> 5)], 1u << (k & 31));
-                }
-            }
-        }
-    }
-}
-
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void pixelwise_splat(PixelArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint2 s_queue[kBlock / 64][kQueueCap];
+    __shared__ uint32_t s_queue[kBlock / 64][kQueueCap];
     __shared__ uint32_t s_wave[kBlock / 64];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -361,9 +315,22 @@ __global__ __launch_bounds__(kBlock) void pixelwise_splat(PixelArgs a)
     }
     // candidate box slack: covers the rounding of xg, dx and of the box itself
     const float slack = 4e-3f + 2e-6f * (float)max(a.out_w, a.out_h);
-    uint2 *queue = s_queue[wid];
+    uint32_t *queue = s_queue[wid];
     int qcount = 0;  // wave-uniform
+    const bool one_k = KM == 1;
+    const bool kvalid = tester && kl < N;
+    const uint32_t klane = (uint32_t)kl << 16;
     __syncthreads();
+
+    auto push = [&](bool cand, uint32_t entry) {
+        const unsigned long long mask = __ballot(cand);
+        if (cand) queue[qcount + __popcll(mask & ((1ull << lane) - 1ull))] = entry;
+        qcount += __popcll(mask);
+        if (qcount >= 64) {
+            splat_drain(a, t, queue, 64, qcount, gA, gB, offs, bits);
+            qcount -= 64;
+        }
+    };
 
     auto flush = [&](int fill) {
         __syncthreads();
@@ -372,42 +339,20 @@ __global__ __launch_bounds__(kBlock) void pixelwise_splat(PixelArgs a)
             const int g = gsub + it * GS;
             const bool gv = tester && g < fill;
             const float4 B = gB[gv ? g : 0];
-            for (int m = 0; m < KM; ++m) {
-                const int k = kl + m * KL;
-                bool cand = false;
-                uint32_t e0 = 0, e1 = 0;
-                if (gv && k < N) {
-                    float oxz = oxz0, oyz = oyz0;
-                    if (KM > 1) {
-                        const float2 o = offs[k];
-                        oxz = o.x * z;
-                        oyz = o.y * z;
-                    }
-                    const float cx = B.x + oxz, cy = B.y + oyz;
-                    const float xl = fmaxf(ceilf(cx - B.z), t.fx0), xh = fminf(floorf(cx + B.z), t.fx1);
-                    const float yl = fmaxf(ceilf(cy - B.z), t.fy0), yh = fminf(floorf(cy + B.z), t.fy1);
-                    cand = xl <= xh && yl <= yh;
-                    if (cand) {
-                        const int ixl = (int)xl - t.X0, ixh = (int)xh - t.X0;
-                        const int iyl = (int)yl - t.Y0, iyh = (int)yh - t.Y0;
-                        e0 = (uint32_t)g | ((uint32_t)ixh << 16) | ((uint32_t)iyh << 24);
-                        e1 = (uint32_t)k | ((uint32_t)ixl << 16) | ((uint32_t)iyl << 24);
-                    }
-                }
-                const unsigned long long mask = __ballot(cand);
-                if (cand) {
-                    const int pos = qcount + __popcll(mask & ((1ull << lane) - 1ull));
-                    queue[pos] = make_uint2(e0, e1);
-                }
-                qcount += __popcll(mask);
-                if (qcount >= 64) {
-                    splat_drain(a, t, queue, 64, qcount, gA, offs, bits);
-                    qcount -= 64;
+            if (one_k) {
+                const SplatBox b = splat_box(B, oxz0, oyz0);
+                push(gv && kvalid && splat_box_hits(b, t), (uint32_t)g | klane);
+            } else {
+                for (int m = 0; m < KM; ++m) {
+                    const int k = kl + m * KL;
+                    const float2 o = offs[k < N ? k : 0];
+                    const SplatBox b = splat_box(B, o.x * z, o.y * z);
+                    push(gv && k < N && splat_box_hits(b, t), (uint32_t)g | ((uint32_t)k << 16));
                 }
             }
         }
         if (qcount > 0) {
-            splat_drain(a, t, queue, qcount, qcount, gA, offs, bits);
+            splat_drain(a, t, queue, qcount, qcount, gA, gB, offs, bits);
             qcount = 0;
         }
         __syncthreads();

@@ -126,10 +126,22 @@ int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const floa
 typedef struct {
     int pix_tile_w, pix_tile_h, pix_lds_bytes;
     int grain_tile_w, grain_tile_h, grain_lds_bytes, grain_grains_per_thread;
-    int pix_variant; /* 0 splat (default), 1 gather (the reference's loop order) */
+    int pix_variant; /* pixel-wise formulation: 0 automatic (= 3), 1 gather in the reference's
+                        loop order, 2 splat with per-tile grain generation, 3 splat from grains
+                        generated once per plane in HBM; all give identical results */
+    int pix_block;   /* threads per workgroup of the HBM splat kernel: 512 (default) or 256 */
 } fg_launch_config;
 int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg);
 int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out);
+
+/* Kernel timing of the following renders on this context: with enable set,
+ * HIP events are recorded on the launch stream around the grain-generation
+ * kernels and around the coverage-test kernel of each render, and
+ * fg_get_timings waits for the last render and returns milliseconds:
+ * ms[0] whole render, ms[1] generation kernels, ms[2] coverage-test kernel.
+ * No reference counterpart (benchmarking support). */
+int fg_set_timing(fg_ctx *ctx, int enable);
+int fg_get_timings(fg_ctx *ctx, float ms[3]);
 
 /* Library version string. */
 const char *fg_version(void);

@@ -4,7 +4,8 @@ Bar: pixel values are counts * (1/N), so equality is checked bit-exactly
 (max |diff| == 0).  The north-star tolerance (at most one flipped sample on
 >= 99.99% of pixels, mean |diff| <= 1e-4) is asserted as well, so a failure
 report shows both numbers.  The pixel-wise path has two device formulations
-(splat, the default, and gather, the reference's loop order); both are checked
+(splat from grains generated per plane in HBM, the default; splat
+with per-tile generation; gather in the reference's loop order); both are checked
 against the oracle and, at full frame size, against each other.
 """
 import numpy as np
@@ -15,7 +16,7 @@ from tests import fgcases
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = {"splat": 0, "gather": 1}
+VARIANTS = {"auto": 0, "gather": 1, "tile": 2, "hbm": 3}
 
 
 def _compare(name, got, want, n):
@@ -57,7 +58,7 @@ def _pixel_cases(O):
                              n_samples=16, seed=12, cell_delta=1.0)
 
 
-@pytest.mark.parametrize("variant", ["splat", "gather"])
+@pytest.mark.parametrize("variant", ["hbm", "tile", "gather"])
 def test_pixelwise_cases(hip_ctx, oracle, variant):
     for case in _pixel_cases(oracle):
         want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
@@ -68,16 +69,18 @@ def test_pixelwise_cases(hip_ctx, oracle, variant):
 def test_pixelwise_c1_full_256(hip_ctx, oracle):
     case = fgcases.c1_case(oracle, size=256, n_samples=64)
     want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
-    _compare(case.name, _render(hip_ctx, "splat", case), want, case.n_samples)
+    _compare(case.name, _render(hip_ctx, "auto", case), want, case.n_samples)
 
 
 def test_pixelwise_c2_full_frame_splat_equals_gather_and_oracle_tiles(hip_ctx, oracle):
     """Full 1920x1080 -> 3840x2160 plane: the two device formulations agree on
     every pixel, and random 16x16 tiles agree with the oracle."""
     case = fgcases.c2_case(oracle, w=1920, h=1080, n_samples=256, channel=1)
-    a = _render(hip_ctx, "splat", case)
+    a = _render(hip_ctx, "hbm", case)
     b = _render(hip_ctx, "gather", case)
-    _compare("c2_full:splat_vs_gather", a, b, case.n_samples)
+    _compare("c2_full:hbm_vs_gather", a, b, case.n_samples)
+    c = _render(hip_ctx, "tile", case)
+    _compare("c2_full:tile_vs_gather", c, b, case.n_samples)
     rng = np.random.default_rng(5)
     for (x0, y0, x1, y1) in fgcases.tiles(rng, case.derived.out_w, case.derived.out_h, 16, 24):
         want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input, region=(x0, y0, x1, y1))
