@@ -47,6 +47,8 @@ struct fg_ctx {
     // scratch of the per-plane grain generation: Poisson thresholds, bin
     // counts and starts, scan workspace, grain records
     DevBuf thresholds, bin_count, bin_start, scan_temp, grains;
+    DevBuf samples;  // sample order of the splat kernel
+
     hipEvent_t scratch_done = nullptr;  // recorded after the last kernel that reads the scratch
     bool scratch_pending = false;
     // optional kernel timing: start, generation done, test kernel start, end
@@ -94,6 +96,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->bin_start);
     free_buf(ctx->scan_temp);
     free_buf(ctx->grains);
+    free_buf(ctx->samples);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -224,42 +227,62 @@ void choose_splat_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
 // in-tile generation, 3 splat from grains generated once per plane in HBM.
 // The automatic choice (0) is 3; the caller falls back to 2 when the plane's
 // cell grid is too large to pack cell indices in 16 bits.
+// Largest sample count whose sorted sample table the HBM splat kernel keeps
+// in LDS (18 bytes per sample plus the bucket index).
+constexpr uint32_t kHbmMaxSamples = 2048;
+
 int pixel_variant(fg_ctx *ctx, const fg_plane_desc *d)
 {
     if (ctx->cfg.pix_variant == 1 || d->n_samples > 65535) return 1;
-    if (ctx->cfg.pix_variant == 2) return 2;
+    if (ctx->cfg.pix_variant == 2 || d->n_samples > kHbmMaxSamples) return 2;
     return 3;
 }
 
 int splat_hbm_block(const fg_ctx *ctx) { return ctx->cfg.pix_block == 256 ? 256 : 512; }
 
-void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int variant, int *tw, int *th, int *lds)
+// Buckets of the sample order's fraction index: the next power of two >= N,
+// within [64, 4096].
+int sample_buckets(uint32_t n)
 {
-    if (variant == 3 && ctx->cfg.pix_tile_w == 0 && ctx->cfg.pix_lds_bytes == 0) {
-        // Measured on MI355X (4K RGB, N = 256): occupancy decides, so keep the
-        // workgroup's LDS small -- a bitset of at most 16 KB, the offsets table
-        // and one grain record per thread.
-        static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
-                                        {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
-        const size_t words = (d->n_samples + 31) / 32;
-        *tw = 1;
-        *th = 1;
+    int q = 64;
+    while (q < (int)n && q < 4096) q <<= 1;
+    return q;
+}
+
+// HBM splat kernel: the configured tile, else the largest tile of the ladder
+// whose bitset takes at most 16 KB (measured on MI355X, 4K RGB, N = 256:
+// occupancy decides); the LDS is exactly what the tile and the sample table
+// need.  Returns false when no tile fits the workgroup LDS limit.
+bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
+{
+    static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
+                                    {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
+    const int q = sample_buckets(d->n_samples);
+    const size_t words = (d->n_samples + 31) / 32;
+    const size_t limit = (size_t)ctx->max_lds - (size_t)splat_hbm_block(ctx) / 64 * fg::kQueueCap * 4;
+    auto fits = [&](int w, int h) { return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) <= limit; };
+    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 && ctx->cfg.pix_tile_w <= 256 &&
+        ctx->cfg.pix_tile_h <= 256 && fits(ctx->cfg.pix_tile_w, ctx->cfg.pix_tile_h)) {
+        *tw = ctx->cfg.pix_tile_w;
+        *th = ctx->cfg.pix_tile_h;
+    } else {
+        *tw = 0;
         for (const auto &t : ladder) {
-            if ((size_t)t[0] * t[1] * words * 4 <= 16384) {
+            if ((size_t)t[0] * t[1] * words * 4 <= 16384 && fits(t[0], t[1])) {
                 *tw = t[0];
                 *th = t[1];
                 break;
             }
         }
-        const size_t bits = (((size_t)(*tw) * (*th) * words * 4) + 15) & ~(size_t)15;
-        const size_t offs = d->n_samples <= 2048 ? (((size_t)d->n_samples * 8 + 15) & ~(size_t)15) : 0;
-        *lds = (int)std::min<size_t>(bits + offs + (size_t)splat_hbm_block(ctx) * 32 + 512,
-                                     (size_t)ctx->max_lds - 6 * 1024);
-        return;
+        if (*tw == 0) return false;
     }
-    // the HBM splat kernel holds no generation state, so smaller LDS blocks
-    // buy more resident workgroups
-    const int def = variant == 1 ? 64 * 1024 : (variant == 3 ? 48 * 1024 : 72 * 1024);
+    *lds = (int)fg::splat_hbm_lds(*tw, *th, (int)d->n_samples, q);
+    return true;
+}
+
+void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int variant, int *tw, int *th, int *lds)
+{
+    const int def = variant == 1 ? 64 * 1024 : 72 * 1024;
     const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : def;
     *lds = std::min(budget, ctx->max_lds - 6 * 1024);
     const int max_px = variant == 1 ? fg::kMaxTilePixels : 256 * 256;
@@ -352,12 +375,47 @@ bool plane_geometry(const fg_plane_desc *d, const float4 &r, fg::PixGeom *g)
     return true;
 }
 
+// Bounds the splat kernel relies on, from the plane's extended cell grid and
+// the offsets range r (input pixels):
+//   eps     every coordinate of a candidate box (output pixels) is below
+//           C_out in magnitude, so each f32 operation on it is off by at most
+//           ulp(C_out)/2 <= C_out * 2^-24; the box and the fraction query take
+//           fewer than ten such steps, and eps = C_out * 2^-20 + 2^-16 covers
+//           them with room to spare.
+//   safe_r  a grain that passes the f32 disk test of a point lies within
+//           r (1 + 2^-23) + ulp(C_in) of it, and its centre lies in
+//           [i delta, (i + 1) delta) up to ulp(C_in); with rm - r above
+//           sixteen ulp(C_in) >= C_in * 2^-19, the points whose disk test it
+//           passes have windows (f32 floor((x -+ rm) / delta), rounding
+//           included) that contain its cell.  C_in bounds every coordinate
+//           in input pixels.
+void splat_bounds(const fg::PixelArgs &a, const fg::PixGeom &g, const float4 &r, float *eps, float *safe_r)
+{
+    const double maxo = std::max(std::max(std::fabs((double)r.x), std::fabs((double)r.y)),
+                                 std::max(std::fabs((double)r.z), std::fabs((double)r.w)));
+    const double cells = (double)std::max(std::max(std::llabs((long long)g.Ix0), std::llabs((long long)g.Ix0 + g.W)),
+                                          std::max(std::llabs((long long)g.Jy0), std::llabs((long long)g.Jy0 + g.H)));
+    const double pix = (double)std::max(a.out_w, a.out_h) * (double)a.inv_zoom;
+    const double c_in = std::max(cells * (double)a.delta, pix) + (double)a.rm + maxo + 4.0;
+    const double c_out = (c_in + maxo + (double)a.rm) * (double)a.zoom + 8.0;
+    *eps = (float)(c_out * std::ldexp(1.0, -20) + std::ldexp(1.0, -16));
+    const double lim = (double)a.rm - c_in * std::ldexp(1.0, -19);
+    float sr = (float)lim;
+    if ((double)sr > lim) sr = std::nextafter(sr, -INFINITY);
+    *safe_r = sr > 0.0f ? sr : -1.0f;
+}
+
 // Per-plane generation into HBM, then the splat-from-HBM kernel.
-int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, int dist_kind, hipStream_t s)
+int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
+                  hipStream_t s)
 {
     const char *what = "Pixel renderer";
     int rc = scratch_acquire(ctx, what);
     if (rc != FG_OK) return rc;
+    const int q = sample_buckets((uint32_t)a.n_samples);
+    if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes(a.n_samples, q), what)) != FG_OK) return rc;
+    fg::SampleTable st{};
+    splat_bounds(a, g, range, &st.eps, &st.safe_r);
     const long long nbins = (long long)g.nbx * g.nby;
     const size_t npx = (size_t)a.in_w * a.in_h;
     if ((rc = ensure_bytes(ctx, ctx->thresholds, npx * sizeof(double), what)) != FG_OK) return rc;
@@ -384,10 +442,11 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, int dist_
         return rc;
     float4 *grains = static_cast<float4 *>(ctx->grains.p);
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, grains, s);
+    if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     const int block = splat_hbm_block(ctx);
-    if (e == hipSuccess) e = fg::launch_splat_hbm(a, g, start, grains, block, s);
+    if (e == hipSuccess) e = fg::launch_splat_hbm(a, g, st, start, grains, block, s);
     mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     return scratch_release(ctx, s, what);
@@ -398,18 +457,19 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
 {
     int variant = stats ? 1 : pixel_variant(ctx, d);
     fg::PixGeom geom{};
+    float4 range{};
     hipError_t e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offs), (int)d->n_samples,
                                             ctx->d_range, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     if (variant == 3) {
-        float4 r;
-        e = hipMemcpyAsync(&r, ctx->d_range, sizeof r, hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(&range, ctx->d_range, sizeof range, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
-        if (!plane_geometry(d, r, &geom)) variant = 2;
+        if (!plane_geometry(d, range, &geom)) variant = 2;
     }
     int tw, th, lds;
-    pixel_config(ctx, d, variant, &tw, &th, &lds);
+    if (variant == 3 && !hbm_config(ctx, d, &tw, &th, &lds)) variant = 2;
+    if (variant != 3) pixel_config(ctx, d, variant, &tw, &th, &lds);
     fg::PixelArgs a{};
     a.lambda = d_lambda;
     a.offs = reinterpret_cast<const float2 *>(d_offs);
@@ -435,9 +495,9 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.lds_bytes = lds;
     a.words = (int)((d->n_samples + 31) / 32);
     const size_t bits = (((size_t)tw * th * a.words * 4) + 15) & ~(size_t)15;
-    const bool splat = variant != 1;
-    // the splat kernels need room for one grain record (32 B) per thread
-    const size_t min_grains = variant == 3 ? (size_t)splat_hbm_block(ctx) : 256;
+    // the tile splat kernel needs room for one grain record (32 B) per thread
+    const bool splat = variant == 2;
+    const size_t min_grains = 256;
     a.offs_in_lds = (splat && d->n_samples <= 2048 &&
                      bits + (size_t)d->n_samples * 8 + min_grains * 32 <= (size_t)lds) ? 1 : 0;
     if (splat && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + min_grains * 32 > (size_t)lds)
@@ -454,7 +514,7 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.r_const = rc;
     a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
-    if (variant == 3) return run_pixel_hbm(ctx, a, geom, (int)d->dist_kind, s);
+    if (variant == 3) return run_pixel_hbm(ctx, a, geom, range, (int)d->dist_kind, s);
     mark(ctx, 0, s);
     mark(ctx, 1, s);
     mark(ctx, 2, s);
@@ -744,8 +804,12 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
     } else {
         out->pix_variant = pixel_variant(ctx, desc);
         out->pix_block = splat_hbm_block(ctx);
-        pixel_config(ctx, desc, out->pix_variant, &out->pix_tile_w, &out->pix_tile_h,
-                     &out->pix_lds_bytes);
+        if (out->pix_variant == 3 &&
+            !hbm_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes))
+            out->pix_variant = 2;
+        if (out->pix_variant != 3)
+            pixel_config(ctx, desc, out->pix_variant, &out->pix_tile_w, &out->pix_tile_h,
+                         &out->pix_lds_bytes);
     }
     return FG_OK;
 }

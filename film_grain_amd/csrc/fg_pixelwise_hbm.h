@@ -16,6 +16,29 @@ struct PixGeom {
     int nbx, nby;  // bins per row / column
 };
 
+// The N samples ordered by the fractional part of their scaled x offset
+// X = ox * zoom, with a bucket index over that fraction.  For a grain, the
+// samples whose candidate box has a column in it are those whose fraction
+// lies in one cyclic interval, so they form one cyclic run of this order.
+struct SampleTable {
+    const float2 *xy;      // (ox * zoom, oy * zoom), sorted by frac(ox * zoom), ties by index
+    const float2 *o;       // (ox, oy) in input pixels, same order
+    const uint16_t *k;     // sample index, same order
+    const uint16_t *qidx;  // qbuckets + 1 entries: first position whose fraction >= q / qbuckets
+    const float4 *ext;     // (min X, max X, min Y, max Y) of the scaled offsets
+    const float *key;      // the fractions, same order
+    int qbuckets;          // power of two
+    float eps;             // widening of the fraction interval (covers f32 rounding of the box)
+    float safe_r;          // a grain of radius <= safe_r that passes the disk test of a point
+                           // lies in that point's cell window
+};
+
+// Bytes of the device buffer behind a SampleTable for n samples.
+size_t sample_table_bytes(int n, int qbuckets);
+// Builds the table in `buf` (sample_table_bytes bytes) and points *t at it.
+hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
+                               hipStream_t s);
+
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s);
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
                             hipStream_t s);
@@ -23,7 +46,9 @@ hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *
 hipError_t scan_bins(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, int n, hipStream_t s);
 hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind, const double *T,
                              const uint32_t *bin_start, float4 *grains, hipStream_t s);
-hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const uint32_t *bin_start,
+// LDS bytes the splat kernel needs beyond its static queues for one tile.
+size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
+hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &t, const uint32_t *bin_start,
                             const float4 *grains, int block, hipStream_t s);
 
 }  // namespace fg

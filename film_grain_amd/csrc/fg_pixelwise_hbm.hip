@@ -12,10 +12,15 @@
 //   bin_grains  each bin's grains written contiguously at bin_start[bin]; the
 //               non-empty cells are compacted first so the divergent grain
 //               draws run on lanes that all have work
-//   splat       per output tile: the grains of the bins covering the tile's
-//               cell region are streamed from HBM into LDS batches and tested
-//               per sample through candidate boxes and the exact reference test
-//               (as the tile splat kernel does), into an LDS bitset.
+//   samples     the N sample offsets ordered by the fraction of their scaled
+//               x offset, with a bucket index over that fraction
+//   splat       per output tile, every wave streams 64-grain chunks of the
+//               tile's bins on its own; for each grain the lanes walk the one
+//               cyclic run of the sample order whose x fraction can put a
+//               pixel column in the grain's candidate box, queue the (grain,
+//               sample) pairs whose box meets the tile, and drain the queue 64
+//               pairs at a time through the exact reference test into an LDS
+//               bitset of (pixel, sample) coverage.
 // A grain record is (cx, cy, r, cell) with cell = (i - Ix0) | (j - Jy0) << 16.
 #include <hipcub/hipcub.hpp>
 
@@ -120,11 +125,144 @@ __global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom
 }
 
 // ---------------------------------------------------------------------------
+// sample order
+// ---------------------------------------------------------------------------
+// Fraction of a scaled offset in [0, 1); 1.0 (rounded up from just below) and
+// NaN map to 0.  The sample order and the grain queries only need the two to
+// agree up to the query's widening, which covers this rounding.
+__device__ __forceinline__ float sample_frac(float X)
+{
+    const float f = X - floorf(X);
+    return f < 1.0f ? f : 0.0f;
+}
+
+// Rank sort of the samples by (fraction, index): thread i writes sample i to
+// its rank.  O(N^2) compares, N <= 65535, run once per render.
+__global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs, int n, float zoom,
+                                                             SampleTable t)
+{
+    __shared__ float s_key[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    float2 oi = make_float2(0.0f, 0.0f);
+    float f = 0.0f;
+    if (i < n) {
+        oi = offs[i];
+        f = sample_frac(oi.x * zoom);
+    }
+    int rank = 0;
+    for (int base = 0; base < n; base += kBlock) {
+        const int j = base + (int)threadIdx.x;
+        __syncthreads();
+        s_key[threadIdx.x] = j < n ? sample_frac(offs[j].x * zoom) : 0.0f;
+        __syncthreads();
+        const int m = min(kBlock, n - base);
+        for (int jj = 0; jj < m; ++jj) {
+            const float fj = s_key[jj];
+            rank += (fj < f || (fj == f && base + jj < i)) ? 1 : 0;
+        }
+    }
+    if (i < n) {
+        const_cast<float2 *>(t.xy)[rank] = make_float2(oi.x * zoom, oi.y * zoom);
+        const_cast<float2 *>(t.o)[rank] = oi;
+        const_cast<uint16_t *>(t.k)[rank] = (uint16_t)i;
+        const_cast<float *>(t.key)[rank] = f;
+    }
+}
+
+// Bucket index (first sorted position with fraction >= q / Q, by binary
+// search) and the extents of the scaled offsets; one workgroup.
+__global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable t)
+{
+    __shared__ float4 s_r[kBlock];
+    const int Q = t.qbuckets;
+    for (int q = threadIdx.x; q <= Q; q += kBlock) {
+        const float v = (float)q / (float)Q;  // exact: Q is a power of two
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (t.key[mid] < v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const_cast<uint16_t *>(t.qidx)[q] = (uint16_t)lo;
+    }
+    float4 r = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        const float2 X = t.xy[k];
+        r.x = fminf(r.x, X.x);
+        r.y = fmaxf(r.y, X.x);
+        r.z = fminf(r.z, X.y);
+        r.w = fmaxf(r.w, X.y);
+    }
+    s_r[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const float4 b = s_r[threadIdx.x + w];
+            float4 c = s_r[threadIdx.x];
+            c.x = fminf(c.x, b.x);
+            c.y = fmaxf(c.y, b.y);
+            c.z = fminf(c.z, b.z);
+            c.w = fmaxf(c.w, b.w);
+            s_r[threadIdx.x] = c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *const_cast<float4 *>(t.ext) = s_r[0];
+}
+
+__host__ __device__ static inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+size_t sample_table_bytes(int n, int qbuckets)
+{
+    return 16 + align16((size_t)n * 8) * 2 + align16((size_t)n * 4) + align16((size_t)n * 2) +
+           align16((size_t)(qbuckets + 1) * 2);
+}
+
+hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
+                               hipStream_t s)
+{
+    unsigned char *p = static_cast<unsigned char *>(buf);
+    t->ext = reinterpret_cast<float4 *>(p);
+    p += 16;
+    t->xy = reinterpret_cast<float2 *>(p);
+    p += align16((size_t)n * 8);
+    t->o = reinterpret_cast<float2 *>(p);
+    p += align16((size_t)n * 8);
+    t->key = reinterpret_cast<float *>(p);
+    p += align16((size_t)n * 4);
+    t->k = reinterpret_cast<uint16_t *>(p);
+    p += align16((size_t)n * 2);
+    t->qidx = reinterpret_cast<uint16_t *>(p);
+    t->qbuckets = qbuckets;
+    hipLaunchKernelGGL(sample_sort_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, offs,
+                       n, zoom, *t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sample_index_kernel, dim3(1), dim3(kBlock), 0, s, n, *t);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // splat from HBM grains
 // ---------------------------------------------------------------------------
+size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
+{
+    const size_t words = ((size_t)n_samples + 31) / 32;
+    return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 8) * 2 +
+           align16((size_t)n_samples * 2) + align16((size_t)(qbuckets + 1) * 2);
+}
+
+__device__ __forceinline__ float lane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom geo, const uint32_t *bin_start,
-                                                          const float4 *grains)
+__global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom geo, SampleTable st,
+                                                          const uint32_t *bin_start, const float4 *grains)
 {
     constexpr int NW = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -149,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     t.J0 = geo.Jy0;
     const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
     const int fast = a.div_fast;
-    const int N = a.n_samples, words = a.words;
+    const int N = a.n_samples, words = a.words, Q = st.qbuckets;
     const float4 orange = *a.off_range;
 
     // cell region of the tile (exact), then the bins covering it
@@ -164,136 +302,176 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     // region in packed-cell coordinates, for culling grains of partly covered bins
     const int rci0 = I0 - geo.Ix0, rci1 = I1 - geo.Ix0, rcj0 = J0 - geo.Jy0, rcj1 = J1 - geo.Jy0;
 
-    // ---- LDS layout ----
+    // ---- LDS: bitset | xy[N] | o[N] | k[N] | qidx[Q + 1] ----
+    const int tile_words = a.tile_w * a.tile_h * words;
     size_t off = 0;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
-    const int tile_words = a.tile_w * a.tile_h * words;
-    off += ((size_t)tile_words * 4 + 15) & ~(size_t)15;
-    const float2 *offs = a.offs;
-    if (a.offs_in_lds) {
-        float2 *ol = reinterpret_cast<float2 *>(lds + off);
-        for (int k = tid; k < N; k += BLOCK) ol[k] = a.offs[k];
-        offs = ol;
-        off += ((size_t)N * 8 + 15) & ~(size_t)15;
-    }
-    const int gcap = (int)min((long long)((a.lds_bytes - (long long)off) / 32), 65535LL);
-    if (gcap < BLOCK) {
+    off += align16((size_t)tile_words * 4);
+    float2 *sxy = reinterpret_cast<float2 *>(lds + off);
+    off += align16((size_t)N * 8);
+    float2 *so = reinterpret_cast<float2 *>(lds + off);
+    off += align16((size_t)N * 8);
+    uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
+    off += align16((size_t)N * 2);
+    uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
+    off += align16((size_t)(Q + 1) * 2);
+    if (off > (size_t)a.lds_bytes) {
         if (tid == 0) atomicOr(a.status, kStatusRegion);
         return;
     }
-    float4 *gA = reinterpret_cast<float4 *>(lds + off);
-    float4 *gB = gA + gcap;
     for (int w = tid; w < tile_words; w += BLOCK) bits[w] = 0u;
+    for (int k = tid; k < N; k += BLOCK) {
+        sxy[k] = st.xy[k];
+        so[k] = st.o[k];
+        sk[k] = st.k[k];
+    }
+    for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
+    const float4 ext = *st.ext;
+    __syncthreads();
 
-    // sample mapping: lanes over samples k, groups over grains
-    const int KL = N >= BLOCK ? BLOCK : N;
-    const int GS = BLOCK / KL;
-    const int kl = tid % KL, gsub = tid / KL;
-    const bool tester = tid < KL * GS;
-    const int KM = (N + KL - 1) / KL;
-    float oxz0 = 0.0f, oyz0 = 0.0f;
-    // the wave's extent of scaled offsets, for whole-wave grain culling
-    float ox_lo = INFINITY, ox_hi = -INFINITY, oy_lo = INFINITY, oy_hi = -INFINITY;
-    if (tester) {
-        for (int m = 0; m < KM; ++m) {
-            const int k = kl + m * KL;
-            if (k >= N) break;
-            const float2 o = a.offs[k];
-            const float ox = o.x * z, oy = o.y * z;
-            if (m == 0) {
-                oxz0 = ox;
-                oyz0 = oy;
-            }
-            ox_lo = fminf(ox_lo, ox);
-            ox_hi = fmaxf(ox_hi, ox);
-            oy_lo = fminf(oy_lo, oy);
-            oy_hi = fmaxf(oy_hi, oy);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        ox_lo = fminf(ox_lo, __shfl_xor(ox_lo, o, 64));
-        ox_hi = fmaxf(ox_hi, __shfl_xor(ox_hi, o, 64));
-        oy_lo = fminf(oy_lo, __shfl_xor(oy_lo, o, 64));
-        oy_hi = fmaxf(oy_hi, __shfl_xor(oy_hi, o, 64));
-    }
     const float slack = 4e-3f + 2e-6f * (float)max(a.out_w, a.out_h);
+    const float eps = st.eps, safe_r = st.safe_r, Qf = (float)Q;
     uint32_t *queue = s_queue[wid];
     int qcount = 0;
-    const bool one_k = KM == 1;  // N <= BLOCK: one sample per lane, offset held in registers
-    const bool kvalid = tester && kl < N;
-    const uint32_t klane = (uint32_t)kl << 16;
 
-    // queue the (grain, sample) pairs whose candidate box meets the tile
+    // Per-lane state of the wave's current 64-grain chunk: lane l holds grain
+    // l's box origin and half-size, centre, signed radius (negative: the cell
+    // window must be checked) and packed cell.
+    float gbx = 0.0f, gby = 0.0f, grz = 0.0f, gcx = 0.0f, gcy = 0.0f, gr = 0.0f;
+    uint32_t gcell = 0;
+
+    // Tests queue entries [0, n) (grain lane | sorted position << 8) against
+    // the reference's disk test and cell window, and moves entries [64,
+    // qcount) to the front.  Called by the whole wave.
+    auto drain = [&](int n) {
+        uint32_t e = 0;
+        if (lane < n) e = queue[lane];
+        if (lane + 64 < qcount) queue[lane] = queue[lane + 64];
+        const int l = (int)(e & 0xFFu), p = (int)(e >> 8);
+        const float bx = __shfl(gbx, l, 64), by = __shfl(gby, l, 64), rz = __shfl(grz, l, 64);
+        const float cx = __shfl(gcx, l, 64), cy = __shfl(gcy, l, 64), rs = __shfl(gr, l, 64);
+        const uint32_t cell = (uint32_t)__shfl((int)gcell, l, 64);
+        if (lane >= n) return;
+        const float2 X = sxy[p];
+        const float2 O = so[p];
+        const int k = sk[p];
+        const SplatBox b = splat_box(make_float4(bx, by, rz, 0.0f), X.x, X.y);
+        const int xlo = (int)fmaxf(b.xl, t.fx0), xhi = (int)fminf(b.xh, t.fx1);
+        const int ylo = (int)fmaxf(b.yl, t.fy0), yhi = (int)fminf(b.yh, t.fy1);
+        const float r = fabsf(rs);
+        const float r2 = r * r;
+        const bool safe = rs > 0.0f;
+        const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
+        const uint32_t bit = 1u << (k & 31);
+        const int kw = k >> 5;
+        for (int y = ylo; y <= yhi; ++y) {
+            const float yg = ((float)y + 0.5f) * iz - O.y;
+            const float dy = yg - cy;
+            const float dy2 = dy * dy;
+            for (int x = xlo; x <= xhi; ++x) {
+                const float xg = ((float)x + 0.5f) * iz - O.x;
+                const float dx = xg - cx;
+                if (dx * dx + dy2 <= r2 &&
+                    (safe || (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
+                              floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast))))
+                    atomicOr(&bits[((y - t.Y0) * t.TW + (x - t.X0)) * words + kw], bit);
+            }
+        }
+    };
+
     auto push = [&](bool cand, uint32_t entry) {
         const unsigned long long mask = __ballot(cand);
         if (cand) queue[qcount + __popcll(mask & ((1ull << lane) - 1ull))] = entry;
         qcount += __popcll(mask);
         if (qcount >= 64) {
-            splat_drain(a, t, queue, 64, qcount, gA, gB, offs, bits);
+            drain(64);
             qcount -= 64;
         }
     };
 
-    auto flush = [&](int fill) {
-        __syncthreads();
-        const int TT = (fill + GS - 1) / GS;
-        for (int it = 0; it < TT; ++it) {
-            const int g = gsub + it * GS;
-            const bool gv = tester && g < fill;
-            const float4 B = gB[gv ? g : 0];
-            // whole-wave cull: no sample of this wave puts the grain's box in the tile
-            const bool reach = gv && B.x + ox_hi + B.z >= t.fx0 && B.x + ox_lo - B.z <= t.fx1 &&
-                               B.y + oy_hi + B.z >= t.fy0 && B.y + oy_lo - B.z <= t.fy1;
-            if (!__any(reach)) continue;
-            if (one_k) {
-                const SplatBox b = splat_box(B, oxz0, oyz0);
-                push(reach && kvalid && splat_box_hits(b, t), (uint32_t)g | klane);
+    // ---- each wave walks the region's bin rows in 64-grain chunks ----
+    int row = by0, cbase = 0, nch = 0;
+    uint32_t rs_ = 0, re_ = 0;
+    auto row_load = [&](int by) {
+        rs_ = bin_start[by * geo.nbx + bx0];
+        re_ = bin_start[by * geo.nbx + bx1 + 1];
+        nch = re_ > rs_ ? (int)((re_ - rs_ + 63u) / 64u) : 0;
+    };
+    if (bx0 <= bx1 && row <= by1) row_load(row);
+    else row = by1 + 1;
+    int c = wid;
+    while (row <= by1) {
+        if (c >= cbase + nch) {
+            cbase += nch;
+            if (++row <= by1) row_load(row);
+            continue;
+        }
+        const uint32_t q = rs_ + (uint32_t)(c - cbase) * 64u + (uint32_t)lane;
+        c += NW;
+        float4 G = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const bool in = q < re_;
+        if (in) G = grains[q];
+        const uint32_t pk = __float_as_uint(G.w);
+        const int ci = (int)(pk & 0xFFFFu), cj = (int)(pk >> 16);
+        const float rad = G.z;
+        const float bx = G.x * z - 0.5f, by = G.y * z - 0.5f, rz = rad * z + slack;
+        // a grain outside the region enters no window of the tile; one whose
+        // boxes miss the tile for every sample cannot cover any of its points
+        bool keep = in && rad > 0.0f && ci >= rci0 && ci <= rci1 && cj >= rcj0 && cj <= rcj1;
+        keep = keep && floorf((bx + ext.y) + rz) >= t.fx0 && ceilf((bx + ext.x) - rz) <= t.fx1 &&
+               floorf((by + ext.w) + rz) >= t.fy0 && ceilf((by + ext.z) - rz) <= t.fy1;
+        // The box has a column iff frac(X) lies in [A - 2rz, A] (mod 1) with
+        // A = frac(rz - bx); widened by eps and rounded out to whole buckets.
+        int p0 = 0, len = 0;
+        if (keep) {
+            const float nc = rz - bx;
+            const float A = nc - floorf(nc);
+            const float lo = (A - (rz + rz)) - eps, hi = A + eps;
+            int lq = (int)floorf(lo * Qf), hq = (int)floorf(hi * Qf) + 1;
+            if (!(lo * Qf > -2.0f * Qf) || !(hi * Qf < 2.0f * Qf) || hq - lq >= Q) {
+                len = N;
             } else {
-                for (int m = 0; m < KM; ++m) {
-                    const int k = kl + m * KL;
-                    const float2 o = offs[k < N ? k : 0];
-                    const SplatBox b = splat_box(B, o.x * z, o.y * z);
-                    push(reach && k < N && splat_box_hits(b, t), (uint32_t)g | ((uint32_t)k << 16));
+                if (lq < 0) {
+                    lq += Q;
+                    hq += Q;
                 }
+                lq = min(max(lq, 0), Q - 1);
+                hq = min(max(hq, lq + 1), 2 * Q);
+                p0 = sq[lq];
+                const int end = hq <= Q ? (int)sq[hq] : N + (int)sq[hq - Q];
+                len = end - p0;
             }
         }
+        gbx = bx;
+        gby = by;
+        grz = rz;
+        gcx = G.x;
+        gcy = G.y;
+        gr = rad <= safe_r ? rad : -rad;
+        gcell = pk;
+        unsigned long long m = __ballot(keep);
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            const float ubx = lane_f(bx, l), uby = lane_f(by, l), urz = lane_f(rz, l);
+            const int up0 = lane_i(p0, l), ulen = lane_i(len, l);
+            const float4 B = make_float4(ubx, uby, urz, 0.0f);
+            for (int i0 = 0; i0 < ulen; i0 += 64) {
+                const int i = i0 + lane;
+                int p = up0 + i;
+                if (p >= N) p -= N;
+                const bool iv = i < ulen;
+                const float2 X = sxy[iv ? p : 0];
+                const SplatBox b = splat_box(B, X.x, X.y);
+                push(iv && splat_box_hits(b, t), (uint32_t)l | ((uint32_t)p << 8));
+            }
+        }
+        // the queue refers to this chunk's lanes: empty it before the next
         if (qcount > 0) {
-            splat_drain(a, t, queue, qcount, qcount, gA, gB, offs, bits);
+            drain(qcount);
             qcount = 0;
         }
-        __syncthreads();
-    };
-
-    // ---- stream the region's grains through LDS batches ----
-    const float zs = z;
-    int fill = 0;  // uniform
-    for (int by = by0; by <= by1; ++by) {
-        uint32_t s = bin_start[by * geo.nbx + bx0];
-        const uint32_t e = bin_start[by * geo.nbx + bx1 + 1];
-        while (s < e) {
-            const int n = (int)min((uint32_t)(gcap - fill), e - s);
-            for (int q = tid; q < n; q += BLOCK) {
-                const float4 G = grains[s + q];
-                const uint32_t pk = __float_as_uint(G.w);
-                const int ci = (int)(pk & 0xFFFFu), cj = (int)(pk >> 16);
-                const bool inreg = ci >= rci0 && ci <= rci1 && cj >= rcj0 && cj <= rcj1;
-                const float rad = G.z;
-                // a grain outside the region can enter no window of the tile:
-                // give it an empty box so it is culled
-                const float r2 = (rad <= 0.0f || !inreg) ? -1.0f : rad * rad;
-                const float rz = inreg ? fmaxf(rad, 0.0f) * zs + slack : -1.0e30f;
-                gA[fill + q] = make_float4(G.x, G.y, r2, G.w);
-                gB[fill + q] = make_float4(G.x * zs - 0.5f, G.y * zs - 0.5f, rz, 0.0f);
-            }
-            fill += n;
-            s += (uint32_t)n;
-            if (fill == gcap) {
-                flush(fill);
-                fill = 0;
-            }
-        }
     }
-    if (fill > 0) flush(fill);
     __syncthreads();
 
     // ---- popcount epilogue: one store per output pixel ----
@@ -339,15 +517,15 @@ hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind
     return hipGetLastError();
 }
 
-hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const uint32_t *bin_start,
+hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &st, const uint32_t *bin_start,
                             const float4 *grains, int block, hipStream_t s)
 {
     const dim3 grid(a.n_tiles);
     const size_t lds = (size_t)a.lds_bytes;
     if (block == 512)
-        hipLaunchKernelGGL((splat_hbm_kernel<512>), grid, dim3(512), lds, s, a, g, bin_start, grains);
+        hipLaunchKernelGGL((splat_hbm_kernel<512>), grid, dim3(512), lds, s, a, g, st, bin_start, grains);
     else
-        hipLaunchKernelGGL((splat_hbm_kernel<256>), grid, dim3(256), lds, s, a, g, bin_start, grains);
+        hipLaunchKernelGGL((splat_hbm_kernel<256>), grid, dim3(256), lds, s, a, g, st, bin_start, grains);
     return hipGetLastError();
 }
 
