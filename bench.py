@@ -121,6 +121,11 @@ def main():
     ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
     ap.add_argument("--pix-variant", type=int, default=None,
                     help="pixel-wise formulation: 1 gather, 2 tile splat, 3 HBM splat")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB, so that counter "
+                         "runs (FETCH_SIZE, WRITE_SIZE) can be scaled by a copy of known size")
+    ap.add_argument("--no-stats", action="store_true",
+                    help="skip the instrumented pass (the roofline then has no FLOP count)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -153,12 +158,23 @@ def main():
     out_d = torch.empty((len(lams), d.out_h, d.out_w), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.copy_calibration_mib} MiB done")
+        del src, dst
+
     # instrumented pass: algorithmic work of the frame
     counters = []
-    for i, l in enumerate(lam_d):
-        counters.append(ctx.stats_device(algo, desc, l, offs_d, out_d[i], stream))
-    torch.cuda.synchronize()
-    log(f"[bench] counters per plane {counters}")
+    if not args.no_stats:
+        for i, l in enumerate(lam_d):
+            counters.append(ctx.stats_device(algo, desc, l, offs_d, out_d[i], stream))
+        torch.cuda.synchronize()
+        log(f"[bench] counters per plane {counters}")
 
     tuned = None if args.no_tuning else autotune.apply_table(ctx, algo, desc)
     if args.pix_variant is not None:
@@ -213,7 +229,9 @@ def main():
     px_samples = d.out_w * d.out_h * len(lam_d) * d.n_samples
     value = px_samples * args.steps / secs / 1e6
     kernel_name = {1: "pixelwise_gather", 2: "pixelwise_splat", 3: "splat_hbm_kernel"}.get(cfg.get("pix_variant"), "")
-    if algo == "pixel":
+    if not counters:
+        roofline = None
+    elif algo == "pixel":
         flops = [16.0 * c[0] + 6.0 * c[1] for c in counters]
         achieved = float(np.mean(flops)) / (test_ms * 1e-3) / 1e12
         roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",

@@ -252,7 +252,9 @@ int sample_buckets(uint32_t n)
 // HBM splat kernel: the configured tile, else the largest tile of the ladder
 // whose bitset takes at most 16 KB (measured on MI355X, 4K RGB, N = 256:
 // occupancy decides); the LDS is exactly what the tile and the sample table
-// need.  Returns false when no tile fits the workgroup LDS limit.
+// need.  Returns false when no tile fits the workgroup LDS limit.  Queue
+// entries hold a tile column in 7 bits and a row in 8 bits, so tiles are at
+// most 128 x 256.
 bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
 {
     static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
@@ -261,7 +263,7 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
     const size_t words = (d->n_samples + 31) / 32;
     const size_t limit = (size_t)ctx->max_lds - (size_t)splat_hbm_block(ctx) / 64 * fg::kQueueCap * 4;
     auto fits = [&](int w, int h) { return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) <= limit; };
-    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 && ctx->cfg.pix_tile_w <= 256 &&
+    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 && ctx->cfg.pix_tile_w <= 128 &&
         ctx->cfg.pix_tile_h <= 256 && fits(ctx->cfg.pix_tile_w, ctx->cfg.pix_tile_h)) {
         *tw = ctx->cfg.pix_tile_w;
         *th = ctx->cfg.pix_tile_h;

@@ -335,53 +335,46 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     int qcount = 0;
 
     // Per-lane state of the wave's current 64-grain chunk: lane l holds grain
-    // l's box origin and half-size, centre, signed radius (negative: the cell
-    // window must be checked) and packed cell.
-    float gbx = 0.0f, gby = 0.0f, grz = 0.0f, gcx = 0.0f, gcy = 0.0f, gr = 0.0f;
+    // l's centre, signed radius (negative: the cell window must be checked)
+    // and packed cell.
+    float gcx = 0.0f, gcy = 0.0f, gr = 0.0f;
     uint32_t gcell = 0;
 
-    // Tests queue entries [0, n) (grain lane | sorted position << 8) against
-    // the reference's disk test and cell window, and moves entries [64,
-    // qcount) to the front.  Called by the whole wave.
+    // A queue entry is one (grain, sample, pixel) test: grain lane (6 bits) |
+    // sorted sample position << 6 (11 bits) | tile column << 17 (7 bits) |
+    // tile row << 24 (8 bits).
+    // Tests entries [0, n) against the reference's disk test and cell window
+    // and moves entries [64, qcount) to the front.  Called by the whole wave.
     auto drain = [&](int n) {
         uint32_t e = 0;
         if (lane < n) e = queue[lane];
         if (lane + 64 < qcount) queue[lane] = queue[lane + 64];
-        const int l = (int)(e & 0xFFu), p = (int)(e >> 8);
-        const float bx = __shfl(gbx, l, 64), by = __shfl(gby, l, 64), rz = __shfl(grz, l, 64);
+        const int l = (int)(e & 63u);
         const float cx = __shfl(gcx, l, 64), cy = __shfl(gcy, l, 64), rs = __shfl(gr, l, 64);
         const uint32_t cell = (uint32_t)__shfl((int)gcell, l, 64);
         if (lane >= n) return;
-        const float2 X = sxy[p];
+        const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
         const float2 O = so[p];
         const int k = sk[p];
-        const SplatBox b = splat_box(make_float4(bx, by, rz, 0.0f), X.x, X.y);
-        const int xlo = (int)fmaxf(b.xl, t.fx0), xhi = (int)fminf(b.xh, t.fx1);
-        const int ylo = (int)fmaxf(b.yl, t.fy0), yhi = (int)fminf(b.yh, t.fy1);
+        const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
+        const float yg = ((float)(t.Y0 + ty) + 0.5f) * iz - O.y;
+        const float dx = xg - cx, dy = yg - cy;
         const float r = fabsf(rs);
-        const float r2 = r * r;
-        const bool safe = rs > 0.0f;
-        const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
-        const uint32_t bit = 1u << (k & 31);
-        const int kw = k >> 5;
-        for (int y = ylo; y <= yhi; ++y) {
-            const float yg = ((float)y + 0.5f) * iz - O.y;
-            const float dy = yg - cy;
-            const float dy2 = dy * dy;
-            for (int x = xlo; x <= xhi; ++x) {
-                const float xg = ((float)x + 0.5f) * iz - O.x;
-                const float dx = xg - cx;
-                if (dx * dx + dy2 <= r2 &&
-                    (safe || (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
-                              floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast))))
-                    atomicOr(&bits[((y - t.Y0) * t.TW + (x - t.X0)) * words + kw], bit);
-            }
+        if (dx * dx + dy * dy <= r * r) {
+            const int gi = t.I0 + (int)(cell & 0xFFFFu), gj = t.J0 + (int)(cell >> 16);
+            if (rs > 0.0f ||
+                (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
+                 floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)))
+                atomicOr(&bits[(ty * t.TW + tx) * words + (k >> 5)], 1u << (k & 31));
         }
     };
 
+    // Appends the entries of the lanes with cand set (at most 63 + 64 are held).
     auto push = [&](bool cand, uint32_t entry) {
         const unsigned long long mask = __ballot(cand);
-        if (cand) queue[qcount + __popcll(mask & ((1ull << lane) - 1ull))] = entry;
+        const int before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (cand) queue[qcount + before] = entry;
         qcount += __popcll(mask);
         if (qcount >= 64) {
             drain(64);
@@ -442,9 +435,6 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
                 len = end - p0;
             }
         }
-        gbx = bx;
-        gby = by;
-        grz = rz;
         gcx = G.x;
         gcy = G.y;
         gr = rad <= safe_r ? rad : -rad;
@@ -455,15 +445,43 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
             m &= m - 1ull;
             const float ubx = lane_f(bx, l), uby = lane_f(by, l), urz = lane_f(rz, l);
             const int up0 = lane_i(p0, l), ulen = lane_i(len, l);
-            const float4 B = make_float4(ubx, uby, urz, 0.0f);
             for (int i0 = 0; i0 < ulen; i0 += 64) {
                 const int i = i0 + lane;
-                int p = up0 + i;
-                if (p >= N) p -= N;
-                const bool iv = i < ulen;
-                const float2 X = sxy[iv ? p : 0];
-                const SplatBox b = splat_box(B, X.x, X.y);
-                push(iv && splat_box_hits(b, t), (uint32_t)l | ((uint32_t)p << 8));
+                const int pr = up0 + i;
+                // cyclic position: pr - N when pr >= N (as unsigned, pr - N
+                // wraps above pr otherwise); lanes past the run read the
+                // last sample and queue nothing
+                const int p = (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
+                const float2 X = sxy[p];
+                // the candidate box (splat_box): a point of the grain's disk
+                // is a pixel in it; the run makes its column range non-empty
+                // up to the widening, and the drain's exact test settles the
+                // rest, so only the rows and the tile are checked here
+                const float cxk = ubx + X.x, cyk = uby + X.y;
+                const float xl = ceilf(cxk - urz), xh = floorf(cxk + urz);
+                const float yl = ceilf(cyk - urz), yh = floorf(cyk + urz);
+                const bool cand = (i < ulen) & (yl <= yh) & (xl <= t.fx1) & (xh >= t.fx0) & (yl <= t.fy1) &
+                                  (yh >= t.fy0);
+                const uint32_t base = (uint32_t)l | ((uint32_t)p << 6);
+                const bool multi = cand & ((xh > xl) | (yh > yl));
+                if (!__any(multi)) {
+                    const int tx = (int)(xl - t.fx0), ty = (int)(yl - t.fy0);
+                    push(cand, base | ((uint32_t)tx << 17) | ((uint32_t)ty << 24));
+                } else {
+                    // boxes of more than one pixel: one entry per pixel of
+                    // the box clipped to the tile
+                    const float cxl = fmaxf(xl, t.fx0), cxh = fminf(xh, t.fx1);
+                    const float cyl = fmaxf(yl, t.fy0), cyh = fminf(yh, t.fy1);
+                    const bool c2 = cand & (cxl <= cxh);
+                    const int bw = c2 ? (int)(cxh - cxl) + 1 : 0, bh = c2 ? (int)(cyh - cyl) + 1 : 0;
+                    const int nb = bw * bh;
+                    const int tx = c2 ? (int)(cxl - t.fx0) : 0, ty = c2 ? (int)(cyl - t.fy0) : 0;
+                    for (int j = 0; __any(j < nb); ++j) {
+                        const bool c = j < nb;
+                        const int jy = c ? j / bw : 0, jx = c ? j - jy * bw : 0;
+                        push(c, base | ((uint32_t)(tx + jx) << 17) | ((uint32_t)(ty + jy) << 24));
+                    }
+                }
             }
         }
         // the queue refers to this chunk's lanes: empty it before the next
