@@ -32,35 +32,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from film_grain_amd.workloads import WORKLOADS, prepare  # noqa: E402
+
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
-
-WORKLOADS = {
-    "pixel-c2": dict(w=1920, h=1080, rgb=True, radius=0.1, sd=0.05, dist="lognorm", zoom=2.0,
-                     iters=256, seed=7, algo="pixel",
-                     desc="pixel-wise 1920x1080 RGB -> 3840x2160, lognormal 0.1/0.05, sigma 0.8, N=256, seed 7"),
-    "grain-c3": dict(w=4096, h=4096, rgb=False, radius=0.05, sd=0.0, dist="const", zoom=1.0,
-                     iters=128, seed=11, algo="grain",
-                     desc="grain-wise 4096x4096 luma ramp+5% noise, r=0.05 const, sigma 0.8, N=128, seed 11"),
-    "pixel-c1": dict(w=256, h=256, rgb=False, radius=0.12, sd=0.0, dist="const", zoom=1.0,
-                     iters=64, seed=42, algo="pixel",
-                     desc="pixel-wise 256x256 luma, r=0.12 const, sigma 0.8, N=64, seed 42"),
-}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def make_inputs(wl):
-    from film_grain_amd import synth
-    if wl is WORKLOADS["pixel-c2"]:
-        planes = synth.c2_rgb(wl["seed"], wl["w"], wl["h"])
-    elif wl is WORKLOADS["grain-c3"]:
-        planes = synth.c3_ramp(wl["seed"], wl["w"])[None]
-    else:
-        planes = synth.c1_luma(wl["seed"], wl["w"])[None]
-    return planes
 
 
 def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
@@ -133,23 +112,12 @@ def main():
         sys.exit(2)
 
     import torch
-    from film_grain_amd import autotune, capi, host
+    from film_grain_amd import autotune, capi
 
     wl = WORKLOADS[args.workload]
     log(f"[bench] workload {args.workload}: {wl['desc']}")
-    planes = make_inputs(wl)
-    cli = host.CliArgs(radius=wl["radius"], radius_stddev=wl["sd"], radius_dist=wl["dist"], zoom=wl["zoom"],
-                       iters=wl["iters"], seed=wl["seed"], algo=wl["algo"], sigma=0.8,
-                       color_mode="rgb" if wl["rgb"] else "luma")
-    d, offs, offs_in = host.derive(cli, wl["w"], wl["h"])
-    lams = []
-    for pl in planes:
-        nrm, _ = host.normalize_plane(pl)
-        lams.append(host.lambda_plane(nrm, d.inv_e_pi_r2))
-    algo = wl["algo"]
-    offsets = offs_in if algo == "pixel" else offs
-    desc = capi.make_desc(d.in_w, d.in_h, d.out_w, d.out_h, d.n_samples, d.dist_kind, d.seed, d.zoom,
-                          d.delta, d.rm, d.radius_mean, d.log_mu, d.log_sigma)
+    prep = prepare(args.workload)
+    planes, lams, offsets, d, desc, algo = prep.planes, prep.lams, prep.offsets, prep.derived, prep.desc, prep.algo
 
     dev = torch.device("cuda:0")
     ctx = capi.Context(0)

@@ -28,20 +28,17 @@ def bucket_key(algo: str, desc) -> str:
 
 
 def pixel_space(desc):
-    """Candidate pixel-wise launch shapes for the HBM splat formulation."""
+    """Candidate pixel-wise launch shapes for the HBM splat formulation: tile
+    shape and workgroup size (the kernel's LDS follows from the tile: one
+    N-bit set per pixel plus the sample tables)."""
     words = (desc.n_samples + 31) // 32
     out = []
-    for tw, th in [(16, 16), (32, 16), (32, 32), (64, 16), (64, 32)]:
-        bits = tw * th * words * 4
+    for tw, th in [(16, 16), (32, 8), (32, 16), (16, 32), (32, 32), (64, 8), (64, 16), (64, 32),
+                   (128, 16), (128, 32)]:
+        if tw * th * words * 4 > 128 * 1024:
+            continue
         for block in (256, 512):
-            for grains in (256, 512, 1024, 2048):
-                if grains < block:
-                    continue
-                lds = bits + min(desc.n_samples, 2048) * 8 + grains * 32 + 64
-                if lds > 150 * 1024:
-                    continue
-                out.append(dict(pix_variant=3, pix_tile_w=tw, pix_tile_h=th,
-                                pix_lds_bytes=(lds + 1023) // 1024 * 1024, pix_block=block))
+            out.append(dict(pix_variant=3, pix_tile_w=tw, pix_tile_h=th, pix_block=block))
     return out
 
 
@@ -117,3 +114,49 @@ def apply_table(ctx, algo, desc, table: dict | None = None) -> dict | None:
         ctx.set_launch_config(**cfg["config"])
         return cfg["config"]
     return None
+
+
+def main(argv=None):
+    """Tunes one synthetic workload on device 0 and records the winner:
+        python -m film_grain_amd.autotune --workload pixel-c2 [--save] [--reps 3]
+    """
+    import argparse
+    import sys
+
+    import numpy as np
+    import torch
+
+    from film_grain_amd import capi
+    from film_grain_amd.workloads import WORKLOADS, prepare
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--save", action="store_true", help="merge the winner into the launch table")
+    ap.add_argument("--table", default=str(TABLE_PATH))
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    prep = prepare(args.workload)
+    dev = torch.device("cuda:0")
+    ctx = capi.Context(0)
+    lam = torch.from_numpy(prep.lams[0]).to(dev)
+    offs = torch.from_numpy(np.ascontiguousarray(prep.offsets)).to(dev)
+    out = torch.empty((prep.desc.out_h, prep.desc.out_w), dtype=torch.float32, device=dev)
+    best, results = tune(ctx, prep.algo, prep.desc, lam, offs, out, reps=args.reps, log=log)
+    ctx.close()
+    if best is None:
+        log("[tune] no candidate ran")
+        return 1
+    key = bucket_key(prep.algo, prep.desc)
+    entry = {"config": best, "ms": round(results[0][0], 3), "candidates": len(results),
+             "workload": WORKLOADS[args.workload]["desc"]}
+    print(json.dumps({key: entry}))
+    if args.save:
+        table = load_table(Path(args.table))
+        table[key] = entry
+        save_table(table, Path(args.table))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -47,7 +47,8 @@ struct fg_ctx {
     // scratch of the per-plane grain generation: Poisson thresholds, bin
     // counts and starts, scan workspace, grain records
     DevBuf thresholds, bin_count, bin_start, scan_temp, grains;
-    DevBuf samples;  // sample order of the splat kernel
+    DevBuf samples;   // sample order of the splat kernel
+    DevBuf bin_mask;  // nonempty cells per bin (256 bits)
 
     hipEvent_t scratch_done = nullptr;  // recorded after the last kernel that reads the scratch
     bool scratch_pending = false;
@@ -97,6 +98,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->scan_temp);
     free_buf(ctx->grains);
     free_buf(ctx->samples);
+    free_buf(ctx->bin_mask);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -423,6 +425,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = ensure_bytes(ctx, ctx->thresholds, npx * sizeof(double), what)) != FG_OK) return rc;
     if ((rc = ensure_bytes(ctx, ctx->bin_count, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
     if ((rc = ensure_bytes(ctx, ctx->bin_start, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->bin_mask, (size_t)nbins * 32, what)) != FG_OK) return rc;
     size_t temp_bytes = 0;
     hipError_t e = fg::scan_bins(nullptr, &temp_bytes, nullptr, nullptr, (int)(nbins + 1), s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
@@ -430,10 +433,11 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     double *T = static_cast<double *>(ctx->thresholds.p);
     uint32_t *cnt = static_cast<uint32_t *>(ctx->bin_count.p);
     uint32_t *start = static_cast<uint32_t *>(ctx->bin_start.p);
+    unsigned long long *mask = static_cast<unsigned long long *>(ctx->bin_mask.p);
     mark(ctx, 0, s);
     e = fg::launch_threshold(a, T, s);
     if (e == hipSuccess) e = hipMemsetAsync(cnt + nbins, 0, 4, s);
-    if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, s);
+    if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, mask, s);
     temp_bytes = ctx->scan_temp.cap;
     if (e == hipSuccess) e = fg::scan_bins(ctx->scan_temp.p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
     uint32_t total = 0;
@@ -443,7 +447,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = ensure_bytes(ctx, ctx->grains, std::max<size_t>(total, 1) * sizeof(float4), what)) != FG_OK)
         return rc;
     float4 *grains = static_cast<float4 *>(ctx->grains.p);
-    e = fg::launch_bin_grains(a, g, dist_kind, T, start, grains, s);
+    e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
     mark(ctx, 1, s);
     mark(ctx, 2, s);

@@ -40,12 +40,14 @@ hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbucke
                                hipStream_t s);
 
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s);
+// bin_mask: four 64-bit words per bin, the nonempty cells (bin_grains reads them).
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
-                            hipStream_t s);
+                            unsigned long long *bin_mask, hipStream_t s);
 // hipCUB exclusive sum; with temp == nullptr only sets *temp_bytes.
 hipError_t scan_bins(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, int n, hipStream_t s);
 hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind, const double *T,
-                             const uint32_t *bin_start, float4 *grains, hipStream_t s);
+                             const uint32_t *bin_start, const unsigned long long *bin_mask, float4 *grains,
+                             hipStream_t s);
 // LDS bytes the splat kernel needs beyond its static queues for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
 hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &t, const uint32_t *bin_start,

@@ -36,16 +36,15 @@ __global__ __launch_bounds__(kBlock) void threshold_kernel(const float *lambda, 
     if (p < n) T[p] = cell_threshold(lambda[p], delta);
 }
 
-// Cell (i, j) of bin `bin`, thread t: returns false outside the extended grid.
-__device__ __forceinline__ bool bin_cell(const PixGeom &g, int bin, int t, int &i, int &j, uint32_t &packed)
+// Cells of a bin, one wave per bin: lane l holds the four cells of column
+// l & 15 and rows 4 (l >> 4) + q, q = 0..3, which share the column half of
+// the cell hash (src/rng.rs:40-44 hashes i, then j).  A bin's nonempty mask
+// is four 64-bit words, bit l of word q for cell (l, q).
+__device__ __forceinline__ void bin_origin(const PixGeom &g, int bin, int &ci0, int &cj0)
 {
-    const int by = bin / g.nbx, bx = bin - by * g.nbx;
-    const int ci = bx * kBinSide + (t & (kBinSide - 1)), cj = by * kBinSide + (t / kBinSide);
-    if (ci >= g.W || cj >= g.H) return false;
-    i = g.Ix0 + ci;
-    j = g.Jy0 + cj;
-    packed = (uint32_t)ci | ((uint32_t)cj << 16);
-    return true;
+    const int by = bin / g.nbx;
+    ci0 = (bin - by * g.nbx) * kBinSide;
+    cj0 = by * kBinSide;
 }
 
 __device__ __forceinline__ double cell_threshold_at(const PixelArgs &a, const double *T, int i, int j,
@@ -59,59 +58,161 @@ __device__ __forceinline__ double cell_threshold_at(const PixelArgs &a, const do
     return T[(size_t)iy * a.in_w + ix];
 }
 
-__global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                           uint32_t *bin_count)
+// mix3 split at its two halves (src/rng.rs:40-44).
+__device__ __forceinline__ uint64_t mix_col(uint64_t seed, int32_t i)
 {
-    __shared__ uint32_t s_wave[kBlock / 64];
-    const int bin = blockIdx.x;
-    int i, j;
-    uint32_t packed, q = 0;
-    if (bin_cell(g, bin, threadIdx.x, i, j, packed)) {
-        float sx, sy;
-        const double e = cell_threshold_at(a, T, i, j, sx, sy);
-        Xoshiro rng;
-        q = cell_count(mix3(a.seed, kCellStream, i, j), e, rng);
-    }
-    uint32_t total;
-    (void)block_exclusive_scan(q, s_wave, total);
-    if (threadIdx.x == 0) bin_count[bin] = total;
+    return splitmix64(rotl64((seed ^ kCellStream) + (uint64_t)(int64_t)i, 17));
+}
+__device__ __forceinline__ uint64_t mix_row(uint64_t col, int32_t j)
+{
+    return splitmix64(rotl64(col + (uint64_t)(int64_t)j, 41));
 }
 
+// Poisson count of a cell known to pass the empty-cell test (or in the
+// rejection branch), leaving rng after the count draws as Poisson::sample
+// does (rand_distr 0.4.3: Knuth multiplies uniforms until p <= exp(-lambda)).
+__device__ __forceinline__ uint32_t cell_count_full(uint64_t h, double e, Xoshiro &rng)
+{
+    if (e > 1.5) return 0u;
+    rng.seed(h);
+    if (e >= 0.0) {
+        double p = standard_f64(rng.next());
+        uint32_t n = 0;
+        while (p > e) {
+            p = p * standard_f64(rng.next());
+            ++n;
+        }
+        return n;
+    }
+    const double r = poisson_rejection(rng, -e);
+    if (!(r > 0.0)) return 0u;
+    if (r >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)r;
+}
+
+// False only for a cell whose Poisson count is 0 by the Knuth branch's first
+// draw (u1 <= exp(-lambda)), which needs just the two seed words that draw
+// depends on.  The rejection branch (e < 0) and the all-zero seed case
+// (reseeded by seed_from_u64) are left to the full count.
+__device__ __forceinline__ bool cell_maybe_nonempty(uint64_t h, double e)
+{
+    if (e > 1.5) return false;
+    if (e < 0.0) return true;
+    const uint64_t s0 = splitmix64(h), s3 = splitmix64(h + 3 * kGolden);
+    if ((s0 | s3) == 0) return true;
+    return standard_f64(rotl64(s0 + s3, 23) + s0) > e;
+}
+
+// Writes the wave's compacted list of the bin's nonempty cells (cell (l, q)
+// as q << 6 | l, in mask order) to list and returns their number.
+__device__ __forceinline__ int compact_cells(const unsigned long long m[4], uint16_t *list)
+{
+    const int lane = threadIdx.x & 63;
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m[q] >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m[q], 0u));
+        if ((m[q] >> lane) & 1ull) list[n + before] = (uint16_t)(q << 6 | lane);
+        n += __popcll(m[q]);
+    }
+    return n;
+}
+
+// Wave-wide inclusive sum.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
+// its 256 cells, the nonempty mask, and the grain count.
+__global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
+                                                           uint32_t *bin_count, unsigned long long *bin_mask)
+{
+    __shared__ uint16_t s_list[kBlock / 64][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int bin = blockIdx.x * (kBlock / 64) + wid;
+    if (bin >= g.nbx * g.nby) return;
+    int ci0, cj0;
+    bin_origin(g, bin, ci0, cj0);
+    const int ci = ci0 + (lane & 15), cjl = cj0 + 4 * (lane >> 4);
+    const uint64_t col = mix_col(a.seed, g.Ix0 + ci);
+    unsigned long long m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int cj = cjl + q;
+        bool ne = false;
+        if (ci < g.W && cj < g.H) {
+            float sx, sy;
+            const double e = cell_threshold_at(a, T, g.Ix0 + ci, g.Jy0 + cj, sx, sy);
+            ne = cell_maybe_nonempty(mix_row(col, g.Jy0 + cj), e);
+        }
+        m[q] = __ballot(ne);
+    }
+    const int n = compact_cells(m, s_list[wid]);
+    uint32_t total = 0;
+    for (int base = 0; base < n; base += 64) {
+        uint32_t c = 0;
+        if (base + lane < n) {
+            const int cell = s_list[wid][base + lane];
+            const int l = cell & 63, q = cell >> 6;
+            const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
+            float sx, sy;
+            const double e = cell_threshold_at(a, T, i, j, sx, sy);
+            Xoshiro rng;
+            c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
+        }
+        total += __shfl(wave_inclusive_sum(c), 63, 64);
+    }
+    if (lane == 0) bin_count[bin] = total;
+    if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
+}
+
+// Pass 2: per bin (one wave), the nonempty cells from the mask, compacted onto
+// the lanes; each recomputes its count (leaving its stream after the count
+// draws) and writes its grains at bin_start[bin] + the wave's prefix.
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                            const uint32_t *bin_start, float4 *grains)
+                                                            const uint32_t *bin_start,
+                                                            const unsigned long long *bin_mask, float4 *grains)
 {
-    __shared__ uint32_t s_wave[kBlock / 64];
-    __shared__ uint32_t s_excl[kBlock];
-    __shared__ uint16_t s_list[kBlock];
-    const int bin = blockIdx.x, tid = threadIdx.x;
-    int i = 0, j = 0;
-    uint32_t packed = 0, q = 0;
-    const bool valid = bin_cell(g, bin, tid, i, j, packed);
-    if (valid) {
-        float sx, sy;
-        const double e = cell_threshold_at(a, T, i, j, sx, sy);
-        Xoshiro rng;
-        q = cell_count(mix3(a.seed, kCellStream, i, j), e, rng);
-    }
-    uint32_t total, nonempty;
-    s_excl[tid] = block_exclusive_scan(q, s_wave, total);
-    const uint32_t slot = block_exclusive_scan(q > 0 ? 1u : 0u, s_wave, nonempty);
-    if (q > 0) s_list[slot] = (uint16_t)tid;
-    __syncthreads();
-    const uint32_t base = bin_start[bin];
-    for (uint32_t e = tid; e < nonempty; e += kBlock) {
-        const int t = s_list[e];
-        int ci, cj;
-        uint32_t pk;
-        bin_cell(g, bin, t, ci, cj, pk);
-        float sx, sy;
-        const double th = cell_threshold_at(a, T, ci, cj, sx, sy);
-        Xoshiro rng;
-        const uint32_t n = cell_count(mix3(a.seed, kCellStream, ci, cj), th, rng);
-        float4 *dst = grains + base + s_excl[t];
-        const float pkf = __uint_as_float(pk);
-        for (uint32_t k = 0; k < n; ++k) {
+    __shared__ uint16_t s_list[kBlock / 64][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int bin = blockIdx.x * (kBlock / 64) + wid;
+    if (bin >= g.nbx * g.nby) return;
+    unsigned long long m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = bin_mask[(size_t)bin * 4 + q];
+    const int n = compact_cells(m, s_list[wid]);
+    int ci0, cj0;
+    bin_origin(g, bin, ci0, cj0);
+    uint32_t out = bin_start[bin];
+    for (int base = 0; base < n; base += 64) {
+        uint32_t c = 0;
+        int ci = 0, cj = 0;
+        float sx = 0.0f, sy = 0.0f;
+        Xoshiro rng{};
+        if (base + lane < n) {
+            const int cell = s_list[wid][base + lane];
+            const int l = cell & 63, q = cell >> 6;
+            ci = ci0 + (l & 15);
+            cj = cj0 + 4 * (l >> 4) + q;
+            const int i = g.Ix0 + ci, j = g.Jy0 + cj;
+            const double e = cell_threshold_at(a, T, i, j, sx, sy);
+            c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
+        }
+        const uint32_t incl = wave_inclusive_sum(c);
+        float4 *dst = grains + out + (incl - c);
+        out += __shfl(incl, 63, 64);
+        const float pkf = __uint_as_float((uint32_t)ci | ((uint32_t)cj << 16));
+        for (uint32_t k = 0; k < c; ++k) {
             const float cx = sx + uniform_f32(rng, a.uscale);
             const float cy = sy + uniform_f32(rng, a.uscale);
             float rad = a.r_const;
@@ -511,10 +612,16 @@ hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
-                            hipStream_t s)
+static dim3 bin_grid(const PixGeom &g)
 {
-    hipLaunchKernelGGL(bin_count_kernel, dim3(g.nbx * g.nby), dim3(kBlock), 0, s, a, g, T, bin_count);
+    const long long nbins = (long long)g.nbx * g.nby;
+    return dim3((unsigned)((nbins + kBlock / 64 - 1) / (kBlock / 64)));
+}
+
+hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
+                            unsigned long long *bin_mask, hipStream_t s)
+{
+    hipLaunchKernelGGL(bin_count_kernel, bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_count, bin_mask);
     return hipGetLastError();
 }
 
@@ -524,13 +631,14 @@ hipError_t scan_bins(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_
 }
 
 hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind, const double *T,
-                             const uint32_t *bin_start, float4 *grains, hipStream_t s)
+                             const uint32_t *bin_start, const unsigned long long *bin_mask, float4 *grains,
+                             hipStream_t s)
 {
     if (dist_kind == 0)
-        hipLaunchKernelGGL((bin_grains_kernel<0>), dim3(g.nbx * g.nby), dim3(kBlock), 0, s, a, g, T, bin_start,
+        hipLaunchKernelGGL((bin_grains_kernel<0>), bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
                            grains);
     else
-        hipLaunchKernelGGL((bin_grains_kernel<1>), dim3(g.nbx * g.nby), dim3(kBlock), 0, s, a, g, T, bin_start,
+        hipLaunchKernelGGL((bin_grains_kernel<1>), bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
                            grains);
     return hipGetLastError();
 }

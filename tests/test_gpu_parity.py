@@ -66,6 +66,24 @@ def test_pixelwise_cases(hip_ctx, oracle, variant):
         _compare(f"{variant}:{case.name}", got, want, case.n_samples)
 
 
+@pytest.mark.parametrize("shape", [dict(pix_block=256), dict(pix_tile_w=24, pix_tile_h=8),
+                                   dict(pix_tile_w=128, pix_tile_h=4, pix_block=256),
+                                   dict(pix_tile_w=1, pix_tile_h=1), dict(pix_tile_w=40, pix_tile_h=56)])
+def test_pixelwise_hbm_launch_shapes(hip_ctx, oracle, shape):
+    """The HBM splat kernel gives the oracle's result for every launch shape
+    (workgroup size, tile widths that are not powers of two, the largest tile
+    width its queue entries can address, one-pixel tiles)."""
+    cases = list(_pixel_cases(oracle))
+    for case in (cases[1], cases[2], cases[5], cases[7]):
+        want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
+        hip_ctx.set_launch_config(pix_variant=VARIANTS["hbm"], **shape)
+        try:
+            got = hip_ctx.render_host("pixel", case.desc, case.lam, case.offsets_input)
+        finally:
+            hip_ctx.reset_launch_config()
+        _compare(f"hbm{shape}:{case.name}", got, want, case.n_samples)
+
+
 def test_pixelwise_c1_full_256(hip_ctx, oracle):
     case = fgcases.c1_case(oracle, size=256, n_samples=64)
     want = oracle.render_pixelwise(case.plane, case.lam, case.offsets_input)
