@@ -42,6 +42,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "film_grain_amd", "tuning", "traffic.json")
+
+
+def traffic_key(workload, kernel, cfg):
+    launch = ",".join(f"{k}={cfg[k]}" for k in sorted(cfg) if cfg[k])
+    return f"{workload}|{kernel}|{launch}"
+
+
+def measured_traffic(workload, kernel, cfg):
+    """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
+    WRITE_SIZE counter runs recorded for this workload and launch shape
+    (film_grain_amd.profiling); None when none were recorded."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            rec = json.load(f).get(traffic_key(workload, kernel, cfg))
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec["traffic_bytes"]
+
+
 def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     """Times the oracle on random 16x16 output tiles of plane 0 and checks the
     GPU frame on the same tiles."""
@@ -57,19 +77,28 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     t_used = 0.0
     n_tiles = 0
     if wl["algo"] == "grain":
-        # the grain-wise oracle works on whole frames: time a crop of the input
-        crop = min(wl["w"], 256)
-        lam_c = np.ascontiguousarray(lam0[:crop, :crop])
+        # the grain-wise oracle renders whole frames: time random 256x256
+        # crops of the input (the same lambda values, so the same work per
+        # pixel; crops do not reproduce the full frame's random streams, so
+        # they are timed only)
+        crop = min(wl["w"], wl["h"], 256)
         od_c = O.derive(wl["radius"], wl["sd"], wl["dist"], wl["zoom"], 0.8, wl["iters"],
                         seed=wl["seed"], in_w=crop, in_h=crop)
         pc = O.plane(od_c, wl["iters"], wl["seed"], wl["zoom"], wl["dist"], wl["radius"])
-        t0 = time.perf_counter()
-        O.render_grainwise(pc, lam_c, offs, threads=threads)
-        t_used = time.perf_counter() - t0
-        n_px = pc.out_w * pc.out_h
-        sample = f"grain-wise oracle on the top-left {crop}x{crop} input crop of the frame"
-        return dict(value=n_px * wl["iters"] / t_used / 1e6, unit="Mpix·samples/s", cores=threads,
-                    kind="port", sample=sample, seconds=round(t_used, 3)), None
+        n_crops = 0
+        while t_used < budget_s and n_crops < 64:
+            x0 = int(rng.integers(0, wl["w"] - crop + 1))
+            y0 = int(rng.integers(0, wl["h"] - crop + 1))
+            lam_c = np.ascontiguousarray(lam0[y0:y0 + crop, x0:x0 + crop])
+            t0 = time.perf_counter()
+            O.render_grainwise(pc, lam_c, offs, threads=threads)
+            t_used += time.perf_counter() - t0
+            n_px += pc.out_w * pc.out_h
+            n_crops += 1
+        sample = (f"grain-wise oracle on {n_crops} random {crop}x{crop} input crops of the frame "
+                  f"({n_px * wl['iters']:.3g} pixel-samples, {t_used:.1f} s)")
+        return dict(value=round(n_px * wl["iters"] / t_used / 1e6, 3), unit="Mpix·samples/s", cores=threads,
+                    kind="port", sample=sample), None
     while t_used < budget_s:
         x0 = int(rng.integers(0, od.out_w - tile + 1))
         y0 = int(rng.integers(0, od.out_h - tile + 1))
@@ -197,13 +226,17 @@ def main():
     px_samples = d.out_w * d.out_h * len(lam_d) * d.n_samples
     value = px_samples * args.steps / secs / 1e6
     kernel_name = {1: "pixelwise_gather", 2: "pixelwise_splat", 3: "splat_hbm_kernel"}.get(cfg.get("pix_variant"), "")
+    if algo == "grain":
+        kernel_name = "grainwise_sorted" if cfg.get("grain_variant") == 2 else "grainwise_tiles"
+    traffic = measured_traffic(args.workload, kernel_name, cfg)
+    log(f"[bench] traffic key {traffic_key(args.workload, kernel_name, cfg)} -> {traffic}")
     if not counters:
         roofline = None
     elif algo == "pixel":
         flops = [16.0 * c[0] + 6.0 * c[1] for c in counters]
         achieved = float(np.mean(flops)) / (test_ms * 1e-3) / 1e12
         roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                        frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=None,
+                        frac=round(achieved / PEAK_FP32_TFLOPS, 5), traffic=traffic,
                         kernel=kernel_name, kernel_ms=round(test_ms, 3),
                         generation_ms=round(gen_ms, 3), render_ms=round(render_ms, 3),
                         flops_per_launch=float(np.mean(flops)),
@@ -213,8 +246,8 @@ def main():
         bytes_per_launch = d.out_w * d.out_h * (8 + d.n_samples / 4.0)
         achieved = bytes_per_launch / (test_ms * 1e-3) / 1e9
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
-                        frac=round(achieved / PEAK_HBM_GBS, 6), traffic=None,
-                        kernel="grainwise_tiles", kernel_ms=round(test_ms, 3),
+                        frac=round(achieved / PEAK_HBM_GBS, 6), traffic=traffic,
+                        kernel=kernel_name, kernel_ms=round(test_ms, 3),
                         bytes_per_launch=bytes_per_launch,
                         pairs_per_launch=float(np.mean([c[1] for c in counters])))
 

@@ -48,6 +48,7 @@ class LaunchConfig(C.Structure):
         ("pix_tile_w", C.c_int), ("pix_tile_h", C.c_int), ("pix_lds_bytes", C.c_int),
         ("grain_tile_w", C.c_int), ("grain_tile_h", C.c_int), ("grain_lds_bytes", C.c_int),
         ("grain_grains_per_thread", C.c_int), ("pix_variant", C.c_int), ("pix_block", C.c_int),
+        ("grain_variant", C.c_int), ("grain_block", C.c_int),
     ]
 
 

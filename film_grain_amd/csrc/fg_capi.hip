@@ -530,13 +530,57 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     return FG_OK;
 }
 
+// Grain-wise formulation: 1 the reference's loop order over all (grain,
+// sample) pairs, 2 runs of the sorted sample order (automatic; needs N <= 2048).
+int grain_variant(fg_ctx *ctx, const fg_plane_desc *d)
+{
+    if (ctx->cfg.grain_variant == 1 || d->n_samples > kHbmMaxSamples) return 1;
+    return 2;
+}
+
+int grain_block(const fg_ctx *ctx) { return ctx->cfg.grain_block == 256 ? 256 : 512; }
+
+// Sorted grain-wise kernel: the configured tile, else the largest tile of the
+// ladder whose bitset takes at most 32 KB; queue entries address at most
+// 128 x 256 tiles.  Returns false when no tile fits the LDS limit.
+bool sorted_grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
+{
+    static const int ladder[][2] = {{128, 64}, {64, 64}, {64, 32}, {32, 32}, {32, 16}, {16, 16},
+                                    {16, 8},   {8, 8},   {8, 4},   {4, 4},   {2, 2},   {1, 1}};
+    const int q = sample_buckets(d->n_samples);
+    const size_t words = (d->n_samples + 31) / 32;
+    const size_t limit = (size_t)ctx->max_lds - (size_t)grain_block(ctx) / 64 * fg::kQueueCap * 4;
+    auto fits = [&](int w, int h) { return fg::grainwise_sorted_lds(w, h, (int)d->n_samples, q) <= limit; };
+    if (ctx->cfg.grain_tile_w > 0 && ctx->cfg.grain_tile_h > 0 && ctx->cfg.grain_tile_w <= 128 &&
+        ctx->cfg.grain_tile_h <= 256 && fits(ctx->cfg.grain_tile_w, ctx->cfg.grain_tile_h)) {
+        *tw = ctx->cfg.grain_tile_w;
+        *th = ctx->cfg.grain_tile_h;
+    } else {
+        *tw = 0;
+        for (const auto &t : ladder) {
+            if ((size_t)t[0] * t[1] * words * 4 <= 32768 && fits(t[0], t[1])) {
+                *tw = t[0];
+                *th = t[1];
+                break;
+            }
+        }
+        if (*tw == 0) return false;
+    }
+    *lds = (int)fg::grainwise_sorted_lds(*tw, *th, (int)d->n_samples, q);
+    return true;
+}
+
 int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
                  float *d_out, hipStream_t s, bool stats)
 {
-    int tw, th, lds, gpt;
-    grain_config(ctx, d, &tw, &th, &lds, &gpt);
-    if (lds > ctx->max_lds)
-        return set_error(ctx, FG_EINVAL, "sample count too large for one grain-wise tile");
+    int variant = stats ? 1 : grain_variant(ctx, d);
+    int tw, th, lds, gpt = 0;
+    if (variant == 2 && !sorted_grain_config(ctx, d, &tw, &th, &lds)) variant = 1;
+    if (variant == 1) {
+        grain_config(ctx, d, &tw, &th, &lds, &gpt);
+        if (lds > ctx->max_lds)
+            return set_error(ctx, FG_EINVAL, "sample count too large for one grain-wise tile");
+    }
     fg::GrainArgs a{};
     a.lambda = d_lambda;
     a.offs = reinterpret_cast<const float2 *>(d_offs);
@@ -566,14 +610,32 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
     a.r_out_max = d->rm * d->zoom;
     a.off_range = ctx->d_range;
+    const char *what = "Grain renderer";
+    if (variant == 2) {
+        // the sample order lives in the context's scratch
+        int rc = scratch_acquire(ctx, what);
+        if (rc != FG_OK) return rc;
+        const int q = sample_buckets(d->n_samples);
+        if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes(a.n_samples, q), what)) != FG_OK)
+            return rc;
+        fg::SampleTable st{};
+        mark(ctx, 0, s);
+        hipError_t e = fg::launch_sample_table(a.offs, a.n_samples, 1.0f, q, ctx->samples.p, &st, s);
+        mark(ctx, 1, s);
+        mark(ctx, 2, s);
+        if (e == hipSuccess) e = fg::launch_grainwise_sorted(a, st, (int)d->dist_kind, grain_block(ctx), s);
+        mark(ctx, 3, s);
+        if (e != hipSuccess) return hip_error(ctx, e, what);
+        return scratch_release(ctx, s, what);
+    }
     hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
-    if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    if (e != hipSuccess) return hip_error(ctx, e, what);
     mark(ctx, 0, s);
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     e = fg::launch_grainwise(a, (int)d->dist_kind, stats, s);
     mark(ctx, 3, s);
-    if (e != hipSuccess) return hip_error(ctx, e, "Grain renderer");
+    if (e != hipSuccess) return hip_error(ctx, e, what);
     return FG_OK;
 }
 
@@ -788,7 +850,8 @@ int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg)
             cfg->grain_tile_h < 0 || cfg->grain_grains_per_thread < 0 ||
             cfg->grain_grains_per_thread > 64 || cfg->pix_lds_bytes < 0 || cfg->grain_lds_bytes < 0 ||
             cfg->pix_variant < 0 || cfg->pix_variant > 3 ||
-            (cfg->pix_block != 0 && cfg->pix_block != 256 && cfg->pix_block != 512))
+            (cfg->pix_block != 0 && cfg->pix_block != 256 && cfg->pix_block != 512) || cfg->grain_variant < 0 ||
+            cfg->grain_variant > 2 || (cfg->grain_block != 0 && cfg->grain_block != 256 && cfg->grain_block != 512))
             return set_error(ctx, FG_EINVAL, "invalid launch configuration");
         ctx->cfg = *cfg;
     } else {
@@ -805,8 +868,14 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
     std::lock_guard<std::mutex> lock(ctx->mu);
     *out = fg_launch_config{};
     if (algo == 1) {
-        grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes,
-                     &out->grain_grains_per_thread);
+        out->grain_variant = grain_variant(ctx, desc);
+        out->grain_block = grain_block(ctx);
+        if (out->grain_variant == 2 &&
+            !sorted_grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes))
+            out->grain_variant = 1;
+        if (out->grain_variant == 1)
+            grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes,
+                         &out->grain_grains_per_thread);
     } else {
         out->pix_variant = pixel_variant(ctx, desc);
         out->pix_block = splat_hbm_block(ctx);

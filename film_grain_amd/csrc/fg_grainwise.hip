@@ -19,6 +19,7 @@
 // grain against 64 different offsets and the grain is a uniform (broadcast)
 // LDS read.
 #include "fg_device.h"
+#include "fg_pixelwise_hbm.h"
 
 namespace fg {
 
@@ -168,6 +169,244 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
         atomicAdd(&a.stats[1], st_pairs);
         atomicAdd(&a.stats[2], st_hits);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Sorted-sample formulation (same result, far fewer (grain, sample) pairs)
+// ---------------------------------------------------------------------------
+// For grain centre t = cx*zoom and sample k, bounds() has a column iff some
+// integer n lies in [(t + X_k - R) - 0.5, (t + X_k + R) - 0.5], i.e. iff
+// frac(X_k) lies in the cyclic interval [A - 2R, A] with A = frac(R - (t -
+// 0.5)).  With the samples ordered by frac(X_k) (SampleTable, built from the
+// output-pixel offsets) those samples are one cyclic run, found through the
+// bucket index; the interval is widened by the table's eps, so the run holds
+// every such sample.  Each lane owns one input pixel of the tile's halo
+// region: it draws that pixel's grains in the reference's order and, per
+// grain, walks its run; a (grain, sample) pair whose bounds() box is
+// non-empty and meets the tile goes to the wave's queue, one entry per pixel
+// of the box inside the tile, and the queue is drained through the
+// reference's disk test into the tile's LDS bitset.
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTable st)
+{
+    constexpr int NW = BLOCK / 64;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t s_queue[NW][kQueueCap];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tile = blockIdx.x;
+    if (tile >= a.n_tiles) return;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
+    const int X1 = min(X0 + a.tile_w, a.out_w), Y1 = min(Y0 + a.tile_h, a.out_h);
+    const float fx0 = (float)X0, fx1 = (float)(X1 - 1), fy0 = (float)Y0, fy1 = (float)(Y1 - 1);
+    const int words = a.words, N = a.n_samples, Q = st.qbuckets;
+    const int tile_words = a.tile_w * a.tile_h * words;
+
+    // ---- LDS: bitset | xy[N] | k[N] | qidx[Q + 1] ----
+    size_t off = 0;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
+    off += ((size_t)tile_words * 4 + 15) & ~(size_t)15;
+    float2 *sxy = reinterpret_cast<float2 *>(lds + off);
+    off += ((size_t)N * 8 + 15) & ~(size_t)15;
+    uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
+    off += ((size_t)N * 2 + 15) & ~(size_t)15;
+    uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
+    off += ((size_t)(Q + 1) * 2 + 15) & ~(size_t)15;
+    if (off > (size_t)a.lds_bytes) {
+        if (tid == 0) atomicOr(a.status, kStatusRegion);
+        return;
+    }
+    for (int w = tid; w < tile_words; w += BLOCK) bits[w] = 0u;
+    for (int k = tid; k < N; k += BLOCK) {
+        sxy[k] = st.xy[k];
+        sk[k] = st.k[k];
+    }
+    for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
+    const float4 ext = *st.ext;
+
+    // Input pixels whose grains can reach the tile (generous margins; extra
+    // pixels only produce grains that miss).
+    const float z = a.zoom, Rmax = a.r_out_max;
+    const int xi0 = max(0, (int)floorf(((float)X0 + 0.5f - Rmax - ext.y) / z) - 2);
+    const int xi1 = min(a.in_w - 1, (int)ceilf(((float)X1 - 0.5f + Rmax - ext.x) / z) + 2);
+    const int yi0 = max(0, (int)floorf(((float)Y0 + 0.5f - Rmax - ext.w) / z) - 2);
+    const int yi1 = min(a.in_h - 1, (int)ceilf(((float)Y1 - 0.5f + Rmax - ext.z) / z) + 2);
+    const int rwi = xi1 - xi0 + 1;
+    const int npx = (xi1 >= xi0 && yi1 >= yi0) ? rwi * (yi1 - yi0 + 1) : 0;
+    __syncthreads();
+
+    // widening of the fraction interval: every coordinate of a box is below
+    // C in magnitude, so each of its f32 steps is off by at most C * 2^-24
+    const float C = (float)max(a.out_w, a.out_h) + fmaxf(fmaxf(fabsf(ext.x), fabsf(ext.y)),
+                                                         fmaxf(fabsf(ext.z), fabsf(ext.w))) + Rmax + 8.0f;
+    const float eps = C * 9.5367431640625e-07f + 1.52587890625e-05f;  // C * 2^-20 + 2^-16
+    const float Qf = (float)Q;
+    uint32_t *queue = s_queue[wid];
+    int qcount = 0;
+    // the lane's current grain: target origin and output radius
+    float gbx = 0.0f, gby = 0.0f, gR = 0.0f;
+
+    // Entries: lane (6 bits) | sorted position << 6 (11 bits) | tile column
+    // << 17 (7 bits) | tile row << 24 (8 bits).  The reference's disk test
+    // (src/grainwise.rs:82-100) for entries [0, n).
+    auto drain = [&](int n) {
+        uint32_t e = 0;
+        if (lane < n) e = queue[lane];
+        if (lane + 64 < qcount) queue[lane] = queue[lane + 64];
+        const int l = (int)(e & 63u);
+        const float bx = __shfl(gbx, l, 64), by = __shfl(gby, l, 64), R = __shfl(gR, l, 64);
+        if (lane >= n) return;
+        const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
+        const float2 X = sxy[p];
+        const int k = sk[p];
+        const float txp = bx + X.x, typ = by + X.y;
+        const float dy = ((float)(Y0 + ty) + 0.5f) - typ;
+        const float dy2 = dy * dy;
+        const float dx = ((float)(X0 + tx) + 0.5f) - txp;
+        if (dx * dx + dy2 <= R * R) atomicOr(&bits[(ty * a.tile_w + tx) * words + (k >> 5)], 1u << (k & 31));
+    };
+    auto push = [&](bool cand, uint32_t entry) {
+        const unsigned long long mask = __ballot(cand);
+        const int before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (cand) queue[qcount + before] = entry;
+        qcount += __popcll(mask);
+        if (qcount >= 64) {
+            drain(64);
+            qcount -= 64;
+        }
+    };
+
+    for (int pb = wid * 64; pb < npx; pb += BLOCK) {
+        const int pix = pb + lane;
+        Xoshiro rng;
+        rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
+        uint32_t q = 0, done = 0;
+        float fx = 0.0f, fy = 0.0f;
+        if (pix < npx) {
+            const int x = xi0 + pix % rwi, y = yi0 + pix / rwi;
+            const float lam = a.lambda[(size_t)y * a.in_w + x];
+            if (lam > 0.0f) {
+                rng.seed(mix3(a.seed, kPixelStream, x, y));
+                q = poisson_u32(rng, (double)lam);
+            }
+            fx = (float)x;
+            fy = (float)y;
+        }
+        while (__any(done < q)) {
+            // the lane's next grain (src/grainwise.rs:48-67)
+            int p0 = 0, len = 0;
+            float bx = 0.0f, by = 0.0f, R = 0.0f;
+            if (done < q) {
+                const float cx = fx + uniform_f32(rng, 1.0f);
+                const float cy = fy + uniform_f32(rng, 1.0f);
+                float rad = KIND == 0 ? a.r_const : lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
+                ++done;
+                if (rad > a.rm) rad = a.rm;
+                R = rad * z;
+                bx = cx * z;
+                by = cy * z;
+                // skip radii <= 0 and grains whose boxes miss the tile for
+                // every sample (bounds() is monotone in the target)
+                if (rad > 0.0f && R > 0.0f && floorf(((bx + ext.y) + R) - 0.5f) >= fx0 &&
+                    ceilf(((bx + ext.x) - R) - 0.5f) <= fx1 && floorf(((by + ext.w) + R) - 0.5f) >= fy0 &&
+                    ceilf(((by + ext.z) - R) - 0.5f) <= fy1) {
+                    const float nc = R - (bx - 0.5f);
+                    const float A = nc - floorf(nc);
+                    const float lo = (A - (R + R)) - eps, hi = A + eps;
+                    if (!(lo * Qf > -2.0f * Qf) || !(hi * Qf < 2.0f * Qf) ||
+                        (int)floorf(hi * Qf) + 1 - (int)floorf(lo * Qf) >= Q) {
+                        len = N;
+                    } else {
+                        int lq = (int)floorf(lo * Qf), hq = (int)floorf(hi * Qf) + 1;
+                        if (lq < 0) {
+                            lq += Q;
+                            hq += Q;
+                        }
+                        lq = min(max(lq, 0), Q - 1);
+                        hq = min(max(hq, lq + 1), 2 * Q);
+                        p0 = sq[lq];
+                        const int end = hq <= Q ? (int)sq[hq] : N + (int)sq[hq - Q];
+                        len = end - p0;
+                    }
+                }
+            }
+            gbx = bx;
+            gby = by;
+            gR = R;
+            for (int it = 0; __any(it < len); ++it) {
+                const int pr = p0 + it;
+                const int p = (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
+                const float2 X = sxy[p];
+                const float txp = bx + X.x, typ = by + X.y;
+                // bounds() of src/grainwise.rs:126-142 before its clamp; the
+                // tile lies inside the image, so meeting the tile is the clamp
+                const float xl = ceilf((txp - R) - 0.5f), xh = floorf((txp + R) - 0.5f);
+                const float yl = ceilf((typ - R) - 0.5f), yh = floorf((typ + R) - 0.5f);
+                const bool cand = (it < len) & (xl <= xh) & (yl <= yh) & (xl <= fx1) & (xh >= fx0) & (yl <= fy1) &
+                                  (yh >= fy0);
+                const uint32_t base = (uint32_t)lane | ((uint32_t)p << 6);
+                const float cxl = fmaxf(xl, fx0), cxh = fminf(xh, fx1);
+                const float cyl = fmaxf(yl, fy0), cyh = fminf(yh, fy1);
+                const int tx = (int)(cxl - fx0), ty = (int)(cyl - fy0);
+                const bool multi = cand & ((cxh > cxl) | (cyh > cyl));
+                if (!__any(multi)) {
+                    push(cand, base | ((uint32_t)tx << 17) | ((uint32_t)ty << 24));
+                } else {
+                    const int bw = cand ? (int)(cxh - cxl) + 1 : 0, bh = cand ? (int)(cyh - cyl) + 1 : 0;
+                    const int nb = bw * bh;
+                    for (int j = 0; __any(j < nb); ++j) {
+                        const bool c = j < nb;
+                        const int jy = c ? j / bw : 0, jx = c ? j - jy * bw : 0;
+                        push(c, base | ((uint32_t)(tx + jx) << 17) | ((uint32_t)(ty + jy) << 24));
+                    }
+                }
+            }
+            // the queue refers to the lanes' current grains
+            if (qcount > 0) {
+                drain(qcount);
+                qcount = 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    const int tw = X1 - X0, th = Y1 - Y0;
+    for (int p = tid; p < tw * th; p += BLOCK) {
+        const int ox = p % tw, oy = p / tw;
+        const uint32_t *pw = bits + ((size_t)oy * a.tile_w + ox) * words;
+        uint32_t count = 0;
+        for (int w = 0; w < words; ++w) count += __popc(pw[w]);
+        a.out[(size_t)(Y0 + oy) * a.out_w + (X0 + ox)] = (float)count * a.inv_samples;
+    }
+}
+
+size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
+{
+    const size_t words = ((size_t)n_samples + 31) / 32;
+    auto a16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    return a16((size_t)tile_w * tile_h * words * 4) + a16((size_t)n_samples * 8) + a16((size_t)n_samples * 2) +
+           a16((size_t)(qbuckets + 1) * 2);
+}
+
+hipError_t launch_grainwise_sorted(const GrainArgs &a, const SampleTable &st, int dist_kind, int block,
+                                   hipStream_t s)
+{
+    const dim3 grid(a.n_tiles);
+    const size_t lds = (size_t)a.lds_bytes;
+    if (dist_kind == 0) {
+        if (block == 256)
+            hipLaunchKernelGGL((grainwise_sorted<0, 256>), grid, dim3(256), lds, s, a, st);
+        else
+            hipLaunchKernelGGL((grainwise_sorted<0, 512>), grid, dim3(512), lds, s, a, st);
+    } else {
+        if (block == 256)
+            hipLaunchKernelGGL((grainwise_sorted<1, 256>), grid, dim3(256), lds, s, a, st);
+        else
+            hipLaunchKernelGGL((grainwise_sorted<1, 512>), grid, dim3(512), lds, s, a, st);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_grainwise(const GrainArgs &a, int dist_kind, bool with_stats, hipStream_t s)

@@ -48,6 +48,13 @@ hipError_t scan_bins(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_
 hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind, const double *T,
                              const uint32_t *bin_start, const unsigned long long *bin_mask, float4 *grains,
                              hipStream_t s);
+// Grain-wise integrator over the sorted sample order (fg_grainwise.hip); the
+// table is built from the output-pixel offsets with zoom 1.  Needs N <= 2048
+// and tiles of at most 128 x 256.
+size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
+hipError_t launch_grainwise_sorted(const GrainArgs &a, const SampleTable &st, int dist_kind, int block,
+                                   hipStream_t s);
+
 // LDS bytes the splat kernel needs beyond its static queues for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
 hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &t, const uint32_t *bin_start,

@@ -130,6 +130,10 @@ typedef struct {
                         loop order, 2 splat with per-tile grain generation, 3 splat from grains
                         generated once per plane in HBM; all give identical results */
     int pix_block;   /* threads per workgroup of the HBM splat kernel: 512 (default) or 256 */
+    int grain_variant; /* grain-wise formulation: 0 automatic (= 2), 1 the reference's loop order
+                          over every (grain, sample) pair, 2 runs of the sample order sorted by
+                          offset fraction; identical results */
+    int grain_block; /* threads per workgroup of the sorted grain-wise kernel: 512 (default) or 256 */
 } fg_launch_config;
 int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg);
 int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out);

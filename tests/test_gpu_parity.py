@@ -135,3 +135,46 @@ def test_grainwise_edge_cases(hip_ctx, oracle):
         want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
         got = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
         _compare(case.name, got, want, case.n_samples)
+
+
+def _grain_cases(O):
+    u = synth.sinusoid_field(23, 17, 8, n_waves=3)
+    yield fgcases.c3_case(O, size=48, n_samples=128)
+    yield fgcases.build_case(O, "g_lognorm_z2", synth.sinusoid_field(30, 20, 3, n_waves=4), 0.2, 0.1, "lognorm",
+                             zoom=2.0, n_samples=96, seed=3)
+    yield fgcases.build_case(O, "g_n1", u, 0.3, n_samples=1, seed=1)
+    yield fgcases.build_case(O, "g_n300_zoom1.5", u[:10, :10], 0.1, 0.05, "lognorm", n_samples=300, zoom=1.5,
+                             seed=2)
+    yield fgcases.build_case(O, "g_big_radius_zoom3", np.full((4, 5), 0.7, np.float32), 0.6, n_samples=40, zoom=3.0,
+                             seed=3)
+    yield fgcases.build_case(O, "g_zoom0.5_sigma2", u, 0.5, n_samples=33, zoom=0.5, sigma=2.0, seed=4)
+
+
+@pytest.mark.parametrize("shape", [dict(grain_variant=1), dict(grain_variant=2),
+                                   dict(grain_variant=2, grain_block=256, grain_tile_w=24, grain_tile_h=8),
+                                   dict(grain_variant=2, grain_tile_w=128, grain_tile_h=4),
+                                   dict(grain_variant=2, grain_tile_w=1, grain_tile_h=3)])
+def test_grainwise_formulations_and_shapes(hip_ctx, oracle, shape):
+    """Both grain-wise formulations, over several launch shapes, give the
+    oracle's planes (large radii exercise boxes of several pixels)."""
+    for case in _grain_cases(oracle):
+        want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
+        hip_ctx.set_launch_config(**shape)
+        try:
+            got = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
+        finally:
+            hip_ctx.reset_launch_config()
+        _compare(f"grain{shape}:{case.name}", got, want, case.n_samples)
+
+
+def test_grainwise_c3_1024_sorted_equals_reference_order(hip_ctx, oracle):
+    """A 1024x1024 crop of the grain-wise workload: the sorted formulation
+    equals the reference-order kernel on every pixel."""
+    case = fgcases.c3_case(oracle, size=1024, n_samples=128)
+    hip_ctx.set_launch_config(grain_variant=1)
+    try:
+        ref = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
+    finally:
+        hip_ctx.reset_launch_config()
+    got = hip_ctx.render_host("grain", case.desc, case.lam, case.offsets)
+    _compare("c3_1024:sorted_vs_reference_order", got, ref, case.n_samples)
