@@ -43,14 +43,15 @@ def pixel_space(desc):
 
 
 def grain_space(desc):
+    """Candidate grain-wise launch shapes for the sorted formulation: tile
+    shape and workgroup size."""
     words = (desc.n_samples + 31) // 32
     out = []
-    for tw, th in [(32, 32), (64, 32), (64, 64), (128, 64)]:
-        for gpt in (2, 4, 8):
-            lds = tw * th * words * 4 + 256 * gpt * 16
-            if lds <= 150 * 1024:
-                out.append(dict(grain_tile_w=tw, grain_tile_h=th, grain_lds_bytes=lds,
-                                grain_grains_per_thread=gpt))
+    for tw, th in [(16, 16), (32, 16), (32, 32), (64, 16), (64, 32), (64, 64), (128, 32), (128, 64)]:
+        if tw * th * words * 4 > 96 * 1024:
+            continue
+        for block in (256, 512):
+            out.append(dict(grain_variant=2, grain_tile_w=tw, grain_tile_h=th, grain_block=block))
     return out
 
 

@@ -49,6 +49,8 @@ struct fg_ctx {
     DevBuf thresholds, bin_count, bin_start, scan_temp, grains;
     DevBuf samples;   // sample order of the splat kernel
     DevBuf bin_mask;  // nonempty cells per bin (256 bits)
+    DevBuf pix_state; // grain-wise: per input pixel grain count and stream state (36 B)
+    DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
 
     hipEvent_t scratch_done = nullptr;  // recorded after the last kernel that reads the scratch
     bool scratch_pending = false;
@@ -99,6 +101,9 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->grains);
     free_buf(ctx->samples);
     free_buf(ctx->bin_mask);
+    free_buf(ctx->pix_state);
+    free_buf(ctx->tiles);
+    free_buf(ctx->tile_temp);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -409,6 +414,37 @@ void splat_bounds(const fg::PixelArgs &a, const fg::PixGeom &g, const float4 &r,
     *safe_r = sr > 0.0f ? sr : -1.0f;
 }
 
+// Heaviest-first launch order of n tiles: the caller's weights kernel fills
+// TileOrder::w / ids, then sort() leaves the sorted ids in TileOrder::order.
+// Workgroups start in launch order, so the heavy tiles start first and the
+// light ones fill the tail.
+struct TileOrder {
+    uint32_t *w = nullptr, *w_sorted = nullptr;
+    int *ids = nullptr, *order = nullptr;
+};
+
+int tile_order_prepare(fg_ctx *ctx, int n, TileOrder *t, hipStream_t s, const char *what)
+{
+    int rc = ensure_bytes(ctx, ctx->tiles, (size_t)n * 16, what);
+    if (rc != FG_OK) return rc;
+    size_t temp = 0;
+    hipError_t e = fg::sort_tiles(nullptr, &temp, nullptr, nullptr, nullptr, nullptr, n, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = ensure_bytes(ctx, ctx->tile_temp, temp, what)) != FG_OK) return rc;
+    unsigned char *p = static_cast<unsigned char *>(ctx->tiles.p);
+    t->w = reinterpret_cast<uint32_t *>(p);
+    t->w_sorted = reinterpret_cast<uint32_t *>(p + (size_t)n * 4);
+    t->ids = reinterpret_cast<int *>(p + (size_t)n * 8);
+    t->order = reinterpret_cast<int *>(p + (size_t)n * 12);
+    return FG_OK;
+}
+
+hipError_t tile_order_sort(fg_ctx *ctx, int n, const TileOrder &t, hipStream_t s)
+{
+    size_t temp = ctx->tile_temp.cap;
+    return fg::sort_tiles(ctx->tile_temp.p, &temp, t.w, t.w_sorted, t.ids, t.order, n, s);
+}
+
 // Per-plane generation into HBM, then the splat-from-HBM kernel.
 int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
                   hipStream_t s)
@@ -447,8 +483,13 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = ensure_bytes(ctx, ctx->grains, std::max<size_t>(total, 1) * sizeof(float4), what)) != FG_OK)
         return rc;
     float4 *grains = static_cast<float4 *>(ctx->grains.p);
+    TileOrder order;
+    if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
+    if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a, g, start, order.w, order.ids, s);
+    if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);
+    a.tile_list = order.order;
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     const int block = splat_hbm_block(ctx);
@@ -618,9 +659,21 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
         const int q = sample_buckets(d->n_samples);
         if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes(a.n_samples, q), what)) != FG_OK)
             return rc;
+        const size_t npx = (size_t)a.in_w * a.in_h;
+        if ((rc = ensure_bytes(ctx, ctx->pix_state, npx * 36 + 32, what)) != FG_OK) return rc;
+        unsigned char *ps = static_cast<unsigned char *>(ctx->pix_state.p);
+        a.pix_s01 = reinterpret_cast<ulonglong2 *>(ps);
+        a.pix_s23 = reinterpret_cast<ulonglong2 *>(ps + npx * 16);
+        a.pix_q = reinterpret_cast<uint32_t *>(ps + npx * 32);
+        TileOrder order;
+        if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
         fg::SampleTable st{};
         mark(ctx, 0, s);
         hipError_t e = fg::launch_sample_table(a.offs, a.n_samples, 1.0f, q, ctx->samples.p, &st, s);
+        if (e == hipSuccess) e = fg::launch_pixel_state(a, s);
+        if (e == hipSuccess) e = fg::launch_grain_tile_weights(a, st, order.w, order.ids, s);
+        if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);
+        a.tile_list = order.order;
         mark(ctx, 1, s);
         mark(ctx, 2, s);
         if (e == hipSuccess) e = fg::launch_grainwise_sorted(a, st, (int)d->dist_kind, grain_block(ctx), s);

@@ -18,6 +18,8 @@
 // slotted grains with lanes over samples k, so the 64 lanes of a wave test one
 // grain against 64 different offsets and the grain is a uniform (broadcast)
 // LDS read.
+#include <algorithm>
+
 #include "fg_device.h"
 #include "fg_pixelwise_hbm.h"
 
@@ -186,6 +188,95 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
 // non-empty and meets the tile goes to the wave's queue, one entry per pixel
 // of the box inside the tile, and the queue is drained through the
 // reference's disk test into the tile's LDS bitset.
+// Poisson pre-pass (src/grainwise.rs:37-45): per input pixel with lambda > 0,
+// pixel_rng and the grain count; the count and the stream's state after the
+// count draws are stored so that the splat kernel resumes each pixel's stream
+// where the reference does (keeps the large Poisson sampler out of the splat
+// kernel's registers).  Pixels with lambda <= 0 get q = 0.
+__global__ __launch_bounds__(kBlock) void pixel_state_kernel(GrainArgs a)
+{
+    const long long n = (long long)a.in_w * a.in_h;
+    for (long long p = (long long)blockIdx.x * kBlock + threadIdx.x; p < n; p += (long long)gridDim.x * kBlock) {
+        const int y = (int)(p / a.in_w), x = (int)(p - (long long)y * a.in_w);
+        const float lam = a.lambda[p];
+        Xoshiro rng;
+        rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
+        uint32_t q = 0;
+        if (lam > 0.0f) {
+            rng.seed(mix3(a.seed, kPixelStream, x, y));
+            q = poisson_u32(rng, (double)lam);
+        }
+        a.pix_q[p] = q;
+        a.pix_s01[p] = make_ulonglong2(rng.s0, rng.s1);
+        a.pix_s23[p] = make_ulonglong2(rng.s2, rng.s3);
+    }
+}
+
+hipError_t launch_pixel_state(const GrainArgs &a, hipStream_t s)
+{
+    const long long n = (long long)a.in_w * a.in_h;
+    const long long blocks = std::min<long long>((n + kBlock - 1) / kBlock, 1LL << 20);
+    hipLaunchKernelGGL(pixel_state_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// Input pixels whose grains can reach output rect [X0, X1) x [Y0, Y1): centre
+// in [x, x+1], target cx*zoom + offset, covered outputs within target +- R -
+// 0.5 (generous margins; extra pixels only produce grains that miss).  ext
+// holds the offsets' (min x, max x, min y, max y).
+struct PixRegion {
+    int x0, y0, w, n;
+};
+__device__ __forceinline__ PixRegion grain_tile_region(const GrainArgs &a, const float4 &ext, int X0, int Y0, int X1,
+                                                       int Y1)
+{
+    const float z = a.zoom, Rmax = a.r_out_max;
+    const int xi0 = max(0, (int)floorf(((float)X0 + 0.5f - Rmax - ext.y) / z) - 2);
+    const int xi1 = min(a.in_w - 1, (int)ceilf(((float)X1 - 0.5f + Rmax - ext.x) / z) + 2);
+    const int yi0 = max(0, (int)floorf(((float)Y0 + 0.5f - Rmax - ext.w) / z) - 2);
+    const int yi1 = min(a.in_h - 1, (int)ceilf(((float)Y1 - 0.5f + Rmax - ext.z) / z) + 2);
+    PixRegion r;
+    r.x0 = xi0;
+    r.y0 = yi0;
+    r.w = xi1 - xi0 + 1;
+    r.n = (xi1 >= xi0 && yi1 >= yi0) ? r.w * (yi1 - yi0 + 1) : 0;
+    return r;
+}
+
+// Work of each grain-wise tile (the grains of its input region, from the
+// Poisson pre-pass), for a heaviest-first launch order; one workgroup per tile.
+__global__ __launch_bounds__(kBlock) void grain_tile_weights_kernel(GrainArgs a, SampleTable st, uint32_t *w,
+                                                                    int *ids)
+{
+    __shared__ uint32_t s_sum[kBlock / 64];
+    const int tile = blockIdx.x;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
+    const PixRegion r = grain_tile_region(a, *st.ext, X0, Y0, min(X0 + a.tile_w, a.out_w),
+                                          min(Y0 + a.tile_h, a.out_h));
+    uint32_t sum = 0;
+    for (int p = threadIdx.x; p < r.n; p += kBlock) {
+        const int y = r.y0 + p / r.w, x = r.x0 + p % r.w;
+        sum += a.pix_q[(size_t)y * a.in_w + x];
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kBlock / 64; ++i) t += s_sum[i];
+        w[tile] = t;
+        ids[tile] = tile;
+    }
+}
+
+hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, uint32_t *w, int *ids,
+                                     hipStream_t s)
+{
+    hipLaunchKernelGGL(grain_tile_weights_kernel, dim3((unsigned)a.n_tiles), dim3(kBlock), 0, s, a, st, w, ids);
+    return hipGetLastError();
+}
+
 template <int KIND, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTable st)
 {
@@ -194,7 +285,8 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
     __shared__ uint32_t s_queue[NW][kQueueCap];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int tile = blockIdx.x;
+    int tile = blockIdx.x;
+    if (a.tile_list) tile = a.tile_list[tile];
     if (tile >= a.n_tiles) return;
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
@@ -225,15 +317,9 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     const float4 ext = *st.ext;
 
-    // Input pixels whose grains can reach the tile (generous margins; extra
-    // pixels only produce grains that miss).
     const float z = a.zoom, Rmax = a.r_out_max;
-    const int xi0 = max(0, (int)floorf(((float)X0 + 0.5f - Rmax - ext.y) / z) - 2);
-    const int xi1 = min(a.in_w - 1, (int)ceilf(((float)X1 - 0.5f + Rmax - ext.x) / z) + 2);
-    const int yi0 = max(0, (int)floorf(((float)Y0 + 0.5f - Rmax - ext.w) / z) - 2);
-    const int yi1 = min(a.in_h - 1, (int)ceilf(((float)Y1 - 0.5f + Rmax - ext.z) / z) + 2);
-    const int rwi = xi1 - xi0 + 1;
-    const int npx = (xi1 >= xi0 && yi1 >= yi0) ? rwi * (yi1 - yi0 + 1) : 0;
+    const PixRegion reg = grain_tile_region(a, ext, X0, Y0, X1, Y1);
+    const int xi0 = reg.x0, yi0 = reg.y0, rwi = reg.w, npx = reg.n;
     __syncthreads();
 
     // widening of the fraction interval: every coordinate of a box is below
@@ -286,11 +372,13 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
         float fx = 0.0f, fy = 0.0f;
         if (pix < npx) {
             const int x = xi0 + pix % rwi, y = yi0 + pix / rwi;
-            const float lam = a.lambda[(size_t)y * a.in_w + x];
-            if (lam > 0.0f) {
-                rng.seed(mix3(a.seed, kPixelStream, x, y));
-                q = poisson_u32(rng, (double)lam);
-            }
+            const size_t pi = (size_t)y * a.in_w + x;
+            q = a.pix_q[pi];
+            const ulonglong2 s01 = a.pix_s01[pi], s23 = a.pix_s23[pi];
+            rng.s0 = s01.x;
+            rng.s1 = s01.y;
+            rng.s2 = s23.x;
+            rng.s3 = s23.y;
             fx = (float)x;
             fy = (float)y;
         }

@@ -52,6 +52,11 @@ struct GrainArgs {
     int grains_per_thread;
     int lds_bytes;
     float zoom, rm, r_const, inv_samples, r_out_max;
+    // per input pixel, written by the Poisson pre-pass of the sorted kernel:
+    // the grain count and the Xoshiro state after the count draws
+    uint32_t *pix_q;
+    ulonglong2 *pix_s01, *pix_s23;
+    const int *tile_list;  // optional launch order of the tiles (sorted kernel)
 };
 
 // Writes (min x, max x, min y, max y) of n (x, y) pairs to *range (one workgroup).

@@ -52,8 +52,22 @@ hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind
 // table is built from the output-pixel offsets with zoom 1.  Needs N <= 2048
 // and tiles of at most 128 x 256.
 size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
+// Per input pixel: grain count and stream state after the count draws
+// (a.pix_q, a.pix_s01, a.pix_s23), read by the sorted grain-wise kernel.
+hipError_t launch_pixel_state(const GrainArgs &a, hipStream_t s);
 hipError_t launch_grainwise_sorted(const GrainArgs &a, const SampleTable &st, int dist_kind, int block,
                                    hipStream_t s);
+
+// Heaviest-first tile order: w[t] = the work of tile t (grains of the bins
+// covering its region), ids[t] = t; sort_tiles orders ids by w, descending
+// (hipCUB radix sort; with temp == nullptr only sets *temp_bytes).
+hipError_t launch_pixel_tile_weights(const PixelArgs &a, const PixGeom &g, const uint32_t *bin_start, uint32_t *w,
+                                     int *ids, hipStream_t s);
+hipError_t sort_tiles(void *temp, size_t *temp_bytes, const uint32_t *w_in, uint32_t *w_out, const int *ids_in,
+                      int *ids_out, int n, hipStream_t s);
+// Grain-wise: w[t] = grains of the input pixels whose grains can reach tile t.
+hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, uint32_t *w, int *ids,
+                                     hipStream_t s);
 
 // LDS bytes the splat kernel needs beyond its static queues for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);

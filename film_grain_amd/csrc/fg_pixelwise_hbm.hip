@@ -355,6 +355,70 @@ size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
            align16((size_t)n_samples * 2) + align16((size_t)(qbuckets + 1) * 2);
 }
 
+// Cell region of output rect [X0, X1) x [Y0, Y1) (the union of its points'
+// windows, exact: every step from a pixel index to a window bound is
+// monotone), relative to the extended grid, and the bins covering it.
+struct TileRegion {
+    int ci0, ci1, cj0, cj1;
+    int bx0, bx1, by0, by1;
+};
+__device__ __forceinline__ TileRegion tile_region(const PixelArgs &a, const PixGeom &geo, int X0, int Y0, int X1,
+                                                  int Y1)
+{
+    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
+    const int fast = a.div_fast;
+    const float4 orange = *a.off_range;
+    const float px_lo = ((float)X0 + 0.5f) * iz, px_hi = ((float)(X1 - 1) + 0.5f) * iz;
+    const float py_lo = ((float)Y0 + 0.5f) * iz, py_hi = ((float)(Y1 - 1) + 0.5f) * iz;
+    const int I0 = floor_div((px_lo - orange.y) - rm, delta, rd, fast);
+    const int I1 = floor_div((px_hi - orange.x) + rm, delta, rd, fast);
+    const int J0 = floor_div((py_lo - orange.w) - rm, delta, rd, fast);
+    const int J1 = floor_div((py_hi - orange.z) + rm, delta, rd, fast);
+    TileRegion r;
+    r.bx0 = max(I0 - geo.Ix0, 0) / kBinSide;
+    r.bx1 = min(I1 - geo.Ix0, geo.W - 1) / kBinSide;
+    r.by0 = max(J0 - geo.Jy0, 0) / kBinSide;
+    r.by1 = min(J1 - geo.Jy0, geo.H - 1) / kBinSide;
+    r.ci0 = I0 - geo.Ix0;
+    r.ci1 = I1 - geo.Ix0;
+    r.cj0 = J0 - geo.Jy0;
+    r.cj1 = J1 - geo.Jy0;
+    return r;
+}
+
+// Work of each splat tile, the grains of the bins covering its region, for
+// a heaviest-first launch order.
+__global__ __launch_bounds__(kBlock) void pixel_tile_weights_kernel(PixelArgs a, PixGeom geo,
+                                                                    const uint32_t *bin_start, uint32_t *w, int *ids)
+{
+    const int tile = blockIdx.x * kBlock + threadIdx.x;
+    if (tile >= a.n_tiles) return;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
+    const TileRegion r = tile_region(a, geo, X0, Y0, min(X0 + a.tile_w, a.out_w), min(Y0 + a.tile_h, a.out_h));
+    uint32_t sum = 0;
+    if (r.bx0 <= r.bx1)
+        for (int by = r.by0; by <= r.by1; ++by)
+            sum += bin_start[by * geo.nbx + r.bx1 + 1] - bin_start[by * geo.nbx + r.bx0];
+    w[tile] = sum;
+    ids[tile] = tile;
+}
+
+hipError_t launch_pixel_tile_weights(const PixelArgs &a, const PixGeom &g, const uint32_t *bin_start, uint32_t *w,
+                                     int *ids, hipStream_t s)
+{
+    hipLaunchKernelGGL(pixel_tile_weights_kernel, dim3((unsigned)((a.n_tiles + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, a, g, bin_start, w, ids);
+    return hipGetLastError();
+}
+
+hipError_t sort_tiles(void *temp, size_t *temp_bytes, const uint32_t *w_in, uint32_t *w_out, const int *ids_in,
+                      int *ids_out, int n, hipStream_t s)
+{
+    return hipcub::DeviceRadixSort::SortPairsDescending(temp, *temp_bytes, w_in, w_out, ids_in, ids_out, n, 0, 32,
+                                                        s);
+}
+
 __device__ __forceinline__ float lane_f(float v, int l)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -389,19 +453,9 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
     const int fast = a.div_fast;
     const int N = a.n_samples, words = a.words, Q = st.qbuckets;
-    const float4 orange = *a.off_range;
-
-    // cell region of the tile (exact), then the bins covering it
-    const float px_lo = (t.fx0 + 0.5f) * iz, px_hi = (t.fx1 + 0.5f) * iz;
-    const float py_lo = (t.fy0 + 0.5f) * iz, py_hi = (t.fy1 + 0.5f) * iz;
-    const int I0 = floor_div((px_lo - orange.y) - rm, delta, rd, fast);
-    const int I1 = floor_div((px_hi - orange.x) + rm, delta, rd, fast);
-    const int J0 = floor_div((py_lo - orange.w) - rm, delta, rd, fast);
-    const int J1 = floor_div((py_hi - orange.z) + rm, delta, rd, fast);
-    const int bx0 = max(I0 - geo.Ix0, 0) / kBinSide, bx1 = min(I1 - geo.Ix0, geo.W - 1) / kBinSide;
-    const int by0 = max(J0 - geo.Jy0, 0) / kBinSide, by1 = min(J1 - geo.Jy0, geo.H - 1) / kBinSide;
-    // region in packed-cell coordinates, for culling grains of partly covered bins
-    const int rci0 = I0 - geo.Ix0, rci1 = I1 - geo.Ix0, rcj0 = J0 - geo.Jy0, rcj1 = J1 - geo.Jy0;
+    const TileRegion reg = tile_region(a, geo, t.X0, t.Y0, t.X1, t.Y1);
+    const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
+    const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
 
     // ---- LDS: bitset | xy[N] | o[N] | k[N] | qidx[Q + 1] ----
     const int tile_words = a.tile_w * a.tile_h * words;
