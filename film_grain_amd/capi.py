@@ -30,8 +30,11 @@ EXPORTED = [
     "fg_ctx_create", "fg_ctx_destroy", "fg_last_error", "fg_render_pixelwise",
     "fg_render_grainwise", "fg_render_pixelwise_device", "fg_render_grainwise_device",
     "fg_stream_status", "fg_pixelwise_stats_device", "fg_grainwise_stats_device",
-    "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings", "fg_version",
+    "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings",
+    "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy", "fg_version",
 ]
+FG_ECAPACITY = 7
+ERROR_NAMES[FG_ECAPACITY] = "FG_ECAPACITY"
 
 
 class PlaneDesc(C.Structure):
@@ -73,6 +76,14 @@ def lib() -> C.CDLL:
         raise FileNotFoundError(
             f"{LIB_PATH} is missing: build it with `python -m film_grain_amd.build` "
             "(the HIP backend has no CPU fallback)")
+    # PyTorch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Loaded first, it satisfies this library's
+    # dependency, so both share one runtime and device pointers pass freely;
+    # loaded second, it would bring a second runtime into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     vp = C.c_void_p
     fp = C.POINTER(C.c_float)
@@ -104,6 +115,17 @@ def lib() -> C.CDLL:
     L.fg_set_timing.restype = C.c_int
     L.fg_get_timings.argtypes = [vp, C.POINTER(C.c_float)]
     L.fg_get_timings.restype = C.c_int
+    L.fg_batch_create.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                  C.c_int, vp, C.POINTER(vp)]
+    L.fg_batch_create.restype = C.c_int
+    for name in ("fg_batch_run", "fg_batch_status"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp]
+        f.restype = C.c_int
+    L.fg_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fg_batch_info.restype = C.c_int
+    L.fg_batch_destroy.argtypes = [vp]
+    L.fg_batch_destroy.restype = None
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
     _lib = L
@@ -206,6 +228,50 @@ class Context:
         self._check(self._L.fg_get_launch_config(self.handle, C.byref(desc), 1 if algo == "grain" else 2,
                                                  C.byref(cfg)))
         return {name: getattr(cfg, name) for name, _ in LaunchConfig._fields_}
+
+
+class Batch:
+    """A planned batch of pixel-wise planes on one context (fg_batch_*).
+    planes: list of (PlaneDesc, lambda tensor, offsets_input tensor, out tensor)
+    on the context's device; the tensors must outlive the batch."""
+
+    def __init__(self, ctx: Context, planes, capture_graph: bool = True, stream=None):
+        self._L = ctx._L
+        self.ctx = ctx
+        n = len(planes)
+        descs = (PlaneDesc * n)(*[p[0] for p in planes])
+        vp = C.c_void_p
+        lams = (vp * n)(*[p[1].data_ptr() for p in planes])
+        offs = (vp * n)(*[p[2].data_ptr() for p in planes])
+        outs = (vp * n)(*[p[3].data_ptr() for p in planes])
+        h = C.c_void_p()
+        rc = self._L.fg_batch_create(ctx.handle, descs, n, lams, offs, outs, 1 if capture_graph else 0,
+                                     C.c_void_p(_stream_handle(stream)), C.byref(h))
+        ctx._check(rc)
+        self.handle = h
+        self._keep = planes
+
+    def run(self, stream=None):
+        self.ctx._check(self._L.fg_batch_run(self.handle, C.c_void_p(_stream_handle(stream))))
+
+    def status(self, stream=None):
+        self.ctx._check(self._L.fg_batch_status(self.handle, C.c_void_p(_stream_handle(stream))))
+
+    def info(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._L.fg_batch_info(self.handle, C.byref(a), C.byref(b), C.byref(c))
+        return dict(items=a.value, grid=b.value, graph=bool(c.value))
+
+    def close(self):
+        if self.handle:
+            self._L.fg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _stream_handle(stream) -> int:

@@ -499,6 +499,53 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     return scratch_release(ctx, s, what);
 }
 
+// Launch arguments of one pixel-wise plane (all but the launch order and the
+// tile splat kernel's offsets staging flag).
+int pixel_args(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs, float *d_out, int tw,
+               int th, int lds, const float4 *d_range, int *d_status, fg::PixelArgs *out)
+{
+    fg::PixelArgs a{};
+    a.lambda = d_lambda;
+    a.offs = reinterpret_cast<const float2 *>(d_offs);
+    a.out = d_out;
+    a.status = d_status;
+    a.stats = ctx->d_stats;
+    a.tile_list = nullptr;
+    a.off_range = d_range;
+    a.seed = d->seed;
+    a.log_mu = (double)d->log_mu;
+    a.log_sigma = (double)d->log_sigma;
+    a.in_w = (int)d->in_w;
+    a.in_h = (int)d->in_h;
+    a.out_w = (int)d->out_w;
+    a.out_h = (int)d->out_h;
+    a.n_samples = (int)d->n_samples;
+    a.tile_w = tw;
+    a.tile_h = th;
+    a.tiles_x = (a.out_w + tw - 1) / tw;
+    const long long ntiles = (long long)a.tiles_x * ((a.out_h + th - 1) / th);
+    if (ntiles > 0x7fffffffLL) return set_error(ctx, FG_EINVAL, "too many tiles");
+    a.n_tiles = (int)ntiles;
+    a.lds_bytes = lds;
+    a.words = (int)((d->n_samples + 31) / 32);
+    a.inv_zoom = 1.0f / d->zoom;
+    a.zoom = d->zoom;
+    a.rm = d->rm;
+    a.delta = d->delta;
+    a.rdelta = 1.0f / d->delta;
+    a.div_fast = division_fast_ok(ctx, a.delta, a.rdelta) ? 1 : 0;
+    a.uscale = fg::uniform_f32_scale(0.0f, d->delta);
+    float rc = d->radius_mean;  // RadiusProfile::sample for Const, then the rm clamp
+    if (rc > d->rm) rc = d->rm;
+    a.r_const = rc;
+    a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
+    a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
+    a.grain_total = nullptr;
+    a.grain_cap = 0;
+    *out = a;
+    return FG_OK;
+}
+
 int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
                  float *d_out, hipStream_t s, bool stats)
 {
@@ -518,29 +565,8 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     if (variant == 3 && !hbm_config(ctx, d, &tw, &th, &lds)) variant = 2;
     if (variant != 3) pixel_config(ctx, d, variant, &tw, &th, &lds);
     fg::PixelArgs a{};
-    a.lambda = d_lambda;
-    a.offs = reinterpret_cast<const float2 *>(d_offs);
-    a.out = d_out;
-    a.status = ctx->d_status;
-    a.stats = ctx->d_stats;
-    a.tile_list = nullptr;
-    a.off_range = ctx->d_range;
-    a.seed = d->seed;
-    a.log_mu = (double)d->log_mu;
-    a.log_sigma = (double)d->log_sigma;
-    a.in_w = (int)d->in_w;
-    a.in_h = (int)d->in_h;
-    a.out_w = (int)d->out_w;
-    a.out_h = (int)d->out_h;
-    a.n_samples = (int)d->n_samples;
-    a.tile_w = tw;
-    a.tile_h = th;
-    a.tiles_x = (a.out_w + tw - 1) / tw;
-    const long long ntiles = (long long)a.tiles_x * ((a.out_h + th - 1) / th);
-    if (ntiles > 0x7fffffffLL) return set_error(ctx, FG_EINVAL, "too many tiles");
-    a.n_tiles = (int)ntiles;
-    a.lds_bytes = lds;
-    a.words = (int)((d->n_samples + 31) / 32);
+    int rc0 = pixel_args(ctx, d, d_lambda, d_offs, d_out, tw, th, lds, ctx->d_range, ctx->d_status, &a);
+    if (rc0 != FG_OK) return rc0;
     const size_t bits = (((size_t)tw * th * a.words * 4) + 15) & ~(size_t)15;
     // the tile splat kernel needs room for one grain record (32 B) per thread
     const bool splat = variant == 2;
@@ -549,18 +575,6 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
                      bits + (size_t)d->n_samples * 8 + min_grains * 32 <= (size_t)lds) ? 1 : 0;
     if (splat && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + min_grains * 32 > (size_t)lds)
         return set_error(ctx, FG_EINVAL, "sample count too large for one pixel-wise tile");
-    a.inv_zoom = 1.0f / d->zoom;
-    a.zoom = d->zoom;
-    a.rm = d->rm;
-    a.delta = d->delta;
-    a.rdelta = 1.0f / d->delta;
-    a.div_fast = division_fast_ok(ctx, a.delta, a.rdelta) ? 1 : 0;
-    a.uscale = fg::uniform_f32_scale(0.0f, d->delta);
-    float rc = d->radius_mean;  // RadiusProfile::sample for Const, then the rm clamp
-    if (rc > d->rm) rc = d->rm;
-    a.r_const = rc;
-    a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
-    a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
     if (variant == 3) return run_pixel_hbm(ctx, a, geom, range, (int)d->dist_kind, s);
     mark(ctx, 0, s);
     mark(ctx, 1, s);
@@ -773,6 +787,208 @@ int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct fg_batch {
+    fg_ctx *ctx = nullptr;
+    int n = 0;
+    std::vector<void *> allocs;
+    std::vector<fg::BatchPlane> planes;
+    std::vector<int> kind;             // radius law per plane
+    std::vector<double *> T;           // thresholds per plane
+    std::vector<long long> bin_off;    // first bin of each plane in the batch's bins
+    std::vector<void *> samples;       // sample table buffer per plane
+    std::vector<int> qb;               // sample buckets per plane
+    fg::BatchPlane *d_planes = nullptr;
+    float4 *ranges = nullptr;
+    int *status = nullptr;
+    uint32_t *cnt = nullptr, *start = nullptr, *partials = nullptr, *weights = nullptr, *counter = nullptr;
+    unsigned long long *mask = nullptr;
+    int2 *items = nullptr;
+    float4 *grains = nullptr;
+    long long total_bins = 0;
+    int n_items = 0, grid = 0, block = 512, lds = 0;
+    uint32_t cap = 0;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_alloc(fg_batch *b, void **p, size_t bytes, const char *what)
+{
+    const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return hip_error(b->ctx, e, what);
+    }
+    b->allocs.push_back(*p);
+    return FG_OK;
+}
+
+void batch_free(fg_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    for (void *p : b->allocs) (void)hipFree(p);
+    b->allocs.clear();
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->capture_stream = nullptr;
+}
+
+// Generation of every plane, then the batch scan (the counting half).
+hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
+{
+    hipError_t e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->status), 1, 0u, s);
+    for (int p = 0; p < b->n && e == hipSuccess; ++p) {
+        const fg::BatchPlane &bp = b->planes[p];
+        e = fg::launch_threshold(bp.a, b->T[p], s);
+        if (e == hipSuccess)
+            e = fg::launch_bin_count(bp.a, bp.geo, b->T[p], b->cnt + b->bin_off[p], b->mask + 4 * b->bin_off[p], s);
+    }
+    if (e == hipSuccess) e = fg::launch_scan_u32(b->cnt, b->start, b->total_bins, b->partials, s);
+    return e;
+}
+
+// The whole pipeline of one run.
+hipError_t batch_enqueue(fg_batch *b, hipStream_t s)
+{
+    hipError_t e = batch_enqueue_counts(b, s);
+    if (e == hipSuccess) e = fg::launch_capacity_check(b->start + b->total_bins, b->cap, b->status, s);
+    for (int p = 0; p < b->n && e == hipSuccess; ++p) {
+        const fg::BatchPlane &bp = b->planes[p];
+        e = fg::launch_bin_grains(bp.a, bp.geo, b->kind[p], b->T[p], bp.bin_start, b->mask + 4 * b->bin_off[p],
+                                  b->grains, s);
+        fg::SampleTable st{};
+        if (e == hipSuccess)
+            e = fg::launch_sample_table(bp.a.offs, bp.a.n_samples, bp.a.zoom, b->qb[p], b->samples[p], &st, s);
+    }
+    if (e == hipSuccess) e = fg::launch_batch_weights(b->d_planes, b->n, b->weights, b->n_items, s);
+    if (e == hipSuccess) e = fg::launch_batch_order(b->d_planes, b->n, b->weights, b->n_items, b->items, nullptr, s);
+    if (e == hipSuccess) e = fg::launch_fill_u32(b->counter, 1, 0u, s);
+    if (e == hipSuccess)
+        e = fg::launch_splat_batch(b->d_planes, b->items, b->n_items, b->counter, b->grid, b->block, b->lds, s);
+    return e;
+}
+
+int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *const *lams,
+                 const float *const *offs, float *const *outs, int capture, hipStream_t s, fg_batch *b)
+{
+    const char *what = "Batch renderer";
+    b->ctx = ctx;
+    b->n = n;
+    int rc;
+    for (int p = 0; p < n; ++p) {
+        if ((rc = validate(ctx, &descs[p])) != FG_OK) return rc;
+        if (!lams[p] || !offs[p] || !outs[p]) return set_error(ctx, FG_EINVAL, "null buffer in batch");
+        if (descs[p].n_samples > kHbmMaxSamples)
+            return set_error(ctx, FG_EINVAL, "batch planes need n_samples <= 2048");
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = batch_alloc(b, (void **)&b->ranges, sizeof(float4) * n, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->status, sizeof(int), what)) != FG_OK) return rc;
+    for (int p = 0; p < n && e == hipSuccess; ++p)
+        e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(offs[p]), (int)descs[p].n_samples,
+                                     b->ranges + p, s);
+    std::vector<float4> r(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(r.data(), b->ranges, sizeof(float4) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+
+    // per plane: geometry, launch shape, arguments, buffers
+    b->planes.resize(n);
+    b->kind.resize(n);
+    b->T.resize(n);
+    b->bin_off.resize(n);
+    b->samples.resize(n);
+    b->qb.resize(n);
+    std::vector<int> lds_p(n);
+    long long items = 0;
+    for (int p = 0; p < n; ++p) {
+        const fg_plane_desc *d = &descs[p];
+        fg::BatchPlane &bp = b->planes[p];
+        if (!plane_geometry(d, r[p], &bp.geo))
+            return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": cell grid exceeds 65535 cells per side");
+        int tw, th;
+        if (!hbm_config(ctx, d, &tw, &th, &lds_p[p]))
+            return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": no tile fits the LDS");
+        if ((rc = pixel_args(ctx, d, lams[p], offs[p], outs[p], tw, th, lds_p[p], b->ranges + p, b->status,
+                             &bp.a)) != FG_OK)
+            return rc;
+        splat_bounds(bp.a, bp.geo, r[p], &bp.st.eps, &bp.st.safe_r);
+        b->kind[p] = (int)d->dist_kind;
+        b->bin_off[p] = b->total_bins;
+        b->total_bins += (long long)bp.geo.nbx * bp.geo.nby;
+        bp.item_base = items;
+        items += bp.a.n_tiles;
+        b->qb[p] = sample_buckets(d->n_samples);
+        if ((rc = batch_alloc(b, (void **)&b->T[p], (size_t)d->in_w * d->in_h * sizeof(double), what)) != FG_OK)
+            return rc;
+        if ((rc = batch_alloc(b, &b->samples[p], fg::sample_table_bytes((int)d->n_samples, b->qb[p]), what)) != FG_OK)
+            return rc;
+        fg::sample_table_layout(b->samples[p], (int)d->n_samples, b->qb[p], &bp.st);
+    }
+    if (items > 0x7fffffffLL || b->total_bins >= 0xffffffffLL)
+        return set_error(ctx, FG_EINVAL, "batch too large");
+    b->n_items = (int)items;
+    b->lds = *std::max_element(lds_p.begin(), lds_p.end());
+    b->block = splat_hbm_block(ctx);
+    if ((size_t)b->lds + (size_t)b->block / 64 * fg::kQueueCap * 4 > (size_t)ctx->max_lds)
+        return set_error(ctx, FG_EINVAL, "batch planes need more LDS than a workgroup has");
+    const long long nb = b->total_bins;
+    if ((rc = batch_alloc(b, (void **)&b->cnt, (size_t)nb * 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->start, (size_t)(nb + 1) * 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->partials, fg::scan_partials(nb) * 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->mask, (size_t)nb * 32, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->weights, (size_t)b->n_items * 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->items, (size_t)b->n_items * sizeof(int2), what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->counter, 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->d_planes, sizeof(fg::BatchPlane) * n, what)) != FG_OK) return rc;
+
+    // counting pass over the current inputs sizes the grain array
+    e = batch_enqueue_counts(b, s);
+    uint32_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, b->start + nb, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    const double want = (double)total * 1.25 + 65536.0;
+    b->cap = (uint32_t)std::min(want, 4294967295.0);
+    if ((rc = batch_alloc(b, (void **)&b->grains, (size_t)b->cap * sizeof(float4), what)) != FG_OK) return rc;
+    for (int p = 0; p < n; ++p) {
+        fg::BatchPlane &bp = b->planes[p];
+        bp.a.lds_bytes = b->lds;
+        bp.a.grain_total = b->start + nb;
+        bp.a.grain_cap = b->cap;
+        bp.bin_start = b->start + b->bin_off[p];
+        bp.grains = b->grains;
+    }
+    e = hipMemcpyAsync(b->d_planes, b->planes.data(), sizeof(fg::BatchPlane) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    b->grid = (int)std::min<long long>(std::max(b->n_items, 1),
+                                       (long long)ctx->num_cus * fg::splat_batch_blocks_per_cu(b->block, b->lds));
+    if (capture) {
+        e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            const hipError_t e2 = batch_enqueue(b, b->capture_stream);
+            e = hipStreamEndCapture(b->capture_stream, &b->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return hip_error(ctx, e, what);
+    }
+    return FG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fg_ctx_create(int device_ordinal, fg_ctx **out)
@@ -962,6 +1178,74 @@ int fg_get_timings(fg_ctx *ctx, float ms[3])
     if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], ctx->tev[2], ctx->tev[3]);
     if (e != hipSuccess) return hip_error(ctx, e, "timing");
     return FG_OK;
+}
+
+int fg_batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n_planes, const float *const *d_lambdas,
+                    const float *const *d_offsets_input_xy, float *const *d_outs, int capture_graph,
+                    void *hip_stream, fg_batch **out)
+{
+    if (!ctx || !out) return FG_EINVAL;
+    *out = nullptr;
+    if (n_planes <= 0 || !descs || !d_lambdas || !d_offsets_input_xy || !d_outs)
+        return set_error(ctx, FG_EINVAL, "empty batch");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = hip_stream ? reinterpret_cast<hipStream_t>(hip_stream) : ctx->stream;
+    fg_batch *b = new fg_batch();
+    const int rc = batch_create(ctx, descs, n_planes, d_lambdas, d_offsets_input_xy, d_outs, capture_graph, s, b);
+    if (rc != FG_OK) {
+        (void)hipStreamSynchronize(s);
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return FG_OK;
+}
+
+int fg_batch_run(fg_batch *b, void *hip_stream)
+{
+    if (!b) return FG_EINVAL;
+    fg_ctx *ctx = b->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = hip_stream ? reinterpret_cast<hipStream_t>(hip_stream) : ctx->stream;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = b->exec ? hipGraphLaunch(b->exec, s) : batch_enqueue(b, s);
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Batch renderer");
+}
+
+int fg_batch_status(fg_batch *b, void *hip_stream)
+{
+    if (!b) return FG_EINVAL;
+    fg_ctx *ctx = b->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = hip_stream ? reinterpret_cast<hipStream_t>(hip_stream) : ctx->stream;
+    int status = 0;
+    hipError_t e = hipMemcpyAsync(&status, b->status, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Batch renderer");
+    if (status & fg::kStatusCapacity)
+        return set_error(ctx, FG_ECAPACITY, "batch grains exceed the planned capacity; create the batch again");
+    if (status != 0) return set_error(ctx, FG_EHIP, "batch renderer: device status " + std::to_string(status));
+    return FG_OK;
+}
+
+int fg_batch_info(const fg_batch *b, int *n_items, int *grid_blocks, int *graph_captured)
+{
+    if (!b) return FG_EINVAL;
+    if (n_items) *n_items = b->n_items;
+    if (grid_blocks) *grid_blocks = b->grid;
+    if (graph_captured) *graph_captured = b->exec ? 1 : 0;
+    return FG_OK;
+}
+
+void fg_batch_destroy(fg_batch *b)
+{
+    if (!b) return;
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+    delete b;
 }
 
 const char *fg_version(void) { return "film_grain_amd 0.1.0 (gfx950)"; }

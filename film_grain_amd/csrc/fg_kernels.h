@@ -16,6 +16,7 @@ constexpr int kQueueCap = 128;        // per-wave candidate queue of the splat k
 constexpr int kStatusNeighbourhood = 1;  // one output pixel's cell window exceeds LDS
 constexpr int kStatusCancelled = 2;
 constexpr int kStatusRegion = 4;         // a tile's cell region exceeds 65535 cells per side
+constexpr int kStatusCapacity = 8;       // a batch's grains exceed the planned capacity
 
 // Pixel-wise integrator launch (src/pixelwise.rs:11-106).
 struct PixelArgs {
@@ -35,6 +36,11 @@ struct PixelArgs {
     int offs_in_lds;            // splat: stage the offsets table in LDS
     int div_fast;               // 1: floor((a)/delta) via the checked reciprocal sequence
     float inv_zoom, zoom, rm, delta, rdelta, uscale, r_const, r2_const, inv_samples;
+    // batches: the grain total (device) and the grain array's capacity; when
+    // the total exceeds it the grain and splat kernels do nothing (the
+    // capacity check reports it).  grain_total == nullptr: unchecked.
+    const uint32_t *grain_total;
+    uint32_t grain_cap;
 };
 
 // Grain-wise integrator launch (src/grainwise.rs:12-124).

@@ -69,6 +69,41 @@ hipError_t sort_tiles(void *temp, size_t *temp_bytes, const uint32_t *w_in, uint
 hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, uint32_t *w, int *ids,
                                      hipStream_t s);
 
+// One plane of a batch: everything its splat tiles read.
+struct BatchPlane {
+    PixelArgs a;
+    PixGeom geo;
+    SampleTable st;
+    const uint32_t *bin_start;  // this plane's bins inside the batch's bin_start
+    const float4 *grains;       // the batch's grain array (bin_start holds batch offsets)
+    long long item_base;        // first (plane, tile) item of this plane
+};
+
+// Pointers of a sample table laid out in buf (as launch_sample_table does).
+void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t);
+// Resident workgroups of the persistent splat kernel per CU.
+int splat_batch_blocks_per_cu(int block, int lds_bytes);
+// Persistent splat over (plane, tile) items in launch order (items[i] =
+// (plane, tile)); the counter must be 0 at launch.
+hipError_t launch_splat_batch(const BatchPlane *planes, const int2 *items, int n_items, unsigned int *counter,
+                              int grid, int block, int lds_bytes, hipStream_t s);
+
+// Graph-safe batch helpers (kernels only, no library calls; fg_batch.hip).
+// Exclusive scan of n u32 values (n + 1 outputs, out[n] = total) with a
+// caller-provided partials buffer of scan_partials(n) entries.
+size_t scan_partials(long long n);
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, long long n, uint32_t *partials, hipStream_t s);
+// Fill n u32 words with v.
+hipError_t launch_fill_u32(uint32_t *p, long long n, uint32_t v, hipStream_t s);
+// Per item: the work of (plane, tile) as in launch_pixel_tile_weights.
+hipError_t launch_batch_weights(const BatchPlane *planes, int n_planes, uint32_t *w, int n_items, hipStream_t s);
+// items in heaviest-first order (buckets of floor(log2(w + 1)), descending);
+// scratch holds 2 * 64 u32.
+hipError_t launch_batch_order(const BatchPlane *planes, int n_planes, const uint32_t *w, int n_items, int2 *items,
+                              uint32_t *scratch, hipStream_t s);
+// Sets kStatusCapacity in *status when the batch's grain total exceeds cap.
+hipError_t launch_capacity_check(const uint32_t *total, uint32_t cap, int *status, hipStream_t s);
+
 // LDS bytes the splat kernel needs beyond its static queues for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
 hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &t, const uint32_t *bin_start,

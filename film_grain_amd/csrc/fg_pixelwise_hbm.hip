@@ -26,6 +26,7 @@
 
 #include "fg_device.h"
 #include "fg_pixelwise_hbm.h"
+#include "fg_region.h"
 
 namespace fg {
 
@@ -187,6 +188,7 @@ __global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int bin = blockIdx.x * (kBlock / 64) + wid;
     if (bin >= g.nbx * g.nby) return;
+    if (a.grain_total && *a.grain_total > a.grain_cap) return;
     unsigned long long m[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[q] = bin_mask[(size_t)bin * 4 + q];
@@ -324,6 +326,17 @@ size_t sample_table_bytes(int n, int qbuckets)
 hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
                                hipStream_t s)
 {
+    sample_table_layout(buf, n, qbuckets, t);
+    hipLaunchKernelGGL(sample_sort_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, offs,
+                       n, zoom, *t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sample_index_kernel, dim3(1), dim3(kBlock), 0, s, n, *t);
+    return hipGetLastError();
+}
+
+void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
+{
     unsigned char *p = static_cast<unsigned char *>(buf);
     t->ext = reinterpret_cast<float4 *>(p);
     p += 16;
@@ -337,12 +350,6 @@ hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbucke
     p += align16((size_t)n * 2);
     t->qidx = reinterpret_cast<uint16_t *>(p);
     t->qbuckets = qbuckets;
-    hipLaunchKernelGGL(sample_sort_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, offs,
-                       n, zoom, *t);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sample_index_kernel, dim3(1), dim3(kBlock), 0, s, n, *t);
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -355,36 +362,6 @@ size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
            align16((size_t)n_samples * 2) + align16((size_t)(qbuckets + 1) * 2);
 }
 
-// Cell region of output rect [X0, X1) x [Y0, Y1) (the union of its points'
-// windows, exact: every step from a pixel index to a window bound is
-// monotone), relative to the extended grid, and the bins covering it.
-struct TileRegion {
-    int ci0, ci1, cj0, cj1;
-    int bx0, bx1, by0, by1;
-};
-__device__ __forceinline__ TileRegion tile_region(const PixelArgs &a, const PixGeom &geo, int X0, int Y0, int X1,
-                                                  int Y1)
-{
-    const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom;
-    const int fast = a.div_fast;
-    const float4 orange = *a.off_range;
-    const float px_lo = ((float)X0 + 0.5f) * iz, px_hi = ((float)(X1 - 1) + 0.5f) * iz;
-    const float py_lo = ((float)Y0 + 0.5f) * iz, py_hi = ((float)(Y1 - 1) + 0.5f) * iz;
-    const int I0 = floor_div((px_lo - orange.y) - rm, delta, rd, fast);
-    const int I1 = floor_div((px_hi - orange.x) + rm, delta, rd, fast);
-    const int J0 = floor_div((py_lo - orange.w) - rm, delta, rd, fast);
-    const int J1 = floor_div((py_hi - orange.z) + rm, delta, rd, fast);
-    TileRegion r;
-    r.bx0 = max(I0 - geo.Ix0, 0) / kBinSide;
-    r.bx1 = min(I1 - geo.Ix0, geo.W - 1) / kBinSide;
-    r.by0 = max(J0 - geo.Jy0, 0) / kBinSide;
-    r.by1 = min(J1 - geo.Jy0, geo.H - 1) / kBinSide;
-    r.ci0 = I0 - geo.Ix0;
-    r.ci1 = I1 - geo.Ix0;
-    r.cj0 = J0 - geo.Jy0;
-    r.cj1 = J1 - geo.Jy0;
-    return r;
-}
 
 // Work of each splat tile, the grains of the bins covering its region, for
 // a heaviest-first launch order.
@@ -425,18 +402,20 @@ __device__ __forceinline__ float lane_f(float v, int l)
 }
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// One output tile of one plane: the body of the single-plane kernel and of the
+// persistent batch kernel (which therefore give identical planes).  lds is the
+// workgroup's dynamic LDS (a.lds_bytes), s_queue its per-wave queues.  Starts
+// with a barrier, so a workgroup may call it for tile after tile.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom geo, SampleTable st,
-                                                          const uint32_t *bin_start, const float4 *grains)
+__device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st,
+                                           const uint32_t *bin_start, const float4 *grains, int tile,
+                                           unsigned char *lds, uint32_t (*s_queue)[kQueueCap])
 {
     constexpr int NW = BLOCK / 64;
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t s_queue[NW][kQueueCap];
-
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    int tile = blockIdx.x;
-    if (a.tile_list) tile = a.tile_list[tile];
+    __syncthreads();
     if (tile >= a.n_tiles) return;
+    if (a.grain_total && *a.grain_total > a.grain_cap) return;
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     SplatTile t;
     t.X0 = txi * a.tile_w;
@@ -656,6 +635,62 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
         for (int w = 0; w < words; ++w) count += __popc(pw[w]);
         a.out[(size_t)(t.Y0 + oy) * a.out_w + (t.X0 + ox)] = (float)count * a.inv_samples;
     }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom geo, SampleTable st,
+                                                          const uint32_t *bin_start, const float4 *grains)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
+    const int tile = a.tile_list ? a.tile_list[blockIdx.x] : (int)blockIdx.x;
+    splat_tile<BLOCK>(a, geo, st, bin_start, grains, tile, lds, s_queue);
+}
+
+// Persistent batch kernel: every workgroup takes (plane, tile) items from the
+// batch's launch order through an atomic counter until the items run out
+// (each workgroup exits after reading an index past the end).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *planes, const int2 *items, int n_items,
+                                                            unsigned int *counter)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= n_items) break;
+        const int2 pt = items[it];
+        const BatchPlane &p = planes[pt.x];
+        splat_tile<BLOCK>(p.a, p.geo, p.st, p.bin_start, p.grains, pt.y, lds, s_queue);
+    }
+}
+
+int splat_batch_blocks_per_cu(int block, int lds_bytes)
+{
+    int nb = 0;
+    const void *f = block == 512 ? reinterpret_cast<const void *>(&splat_batch_kernel<512>)
+                                 : reinterpret_cast<const void *>(&splat_batch_kernel<256>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, (size_t)lds_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_splat_batch(const BatchPlane *planes, const int2 *items, int n_items, unsigned int *counter,
+                              int grid, int block, int lds_bytes, hipStream_t s)
+{
+    if (block == 512)
+        hipLaunchKernelGGL((splat_batch_kernel<512>), dim3(grid), dim3(512), (size_t)lds_bytes, s, planes, items,
+                           n_items, counter);
+    else
+        hipLaunchKernelGGL((splat_batch_kernel<256>), dim3(grid), dim3(256), (size_t)lds_bytes, s, planes, items,
+                           n_items, counter);
+    return hipGetLastError();
 }
 
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s)

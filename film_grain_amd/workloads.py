@@ -47,6 +47,55 @@ class Prepared:
     desc: object         # capi.PlaneDesc
 
 
+@dataclass
+class BatchImage:
+    w: int
+    h: int
+    rgb: bool
+    radius: float
+    sd: float
+    n_samples: int
+    seed: int
+
+
+def batch_images(master_seed: int = 2024, n_images: int = 64, min_side: int = 128,
+                 max_side: int = 2048) -> list[BatchImage]:
+    """The uneven batch: sides uniform in [min_side, max_side], radius
+    U(0.05, 0.3) px, sigma_r / radius U(0, 0.5), N from {32, 64, 128, 256, 512},
+    luma or RGB with equal odds, filter sigma 0.8, zoom 1."""
+    rng = np.random.default_rng(master_seed)
+    out = []
+    for i in range(n_images):
+        w = int(rng.integers(min_side, max_side + 1))
+        h = int(rng.integers(min_side, max_side + 1))
+        r = float(rng.uniform(0.05, 0.3))
+        ratio = float(rng.uniform(0.0, 0.5))
+        n = int(rng.choice([32, 64, 128, 256, 512]))
+        rgb = bool(rng.integers(0, 2))
+        out.append(BatchImage(w, h, rgb, r, ratio * r, n, master_seed * 1000 + i))
+    return out
+
+
+def prepare_batch(images: list[BatchImage]):
+    """Per plane of every image: (plane description, lambda plane,
+    offsets_input) for the pixel-wise path, with the image's derived
+    parameters (sinusoid fields: 4 sinusoids + 2% noise per channel)."""
+    from film_grain_amd import capi, host, synth
+    planes = []
+    for im in images:
+        cli = host.CliArgs(radius=im.radius, radius_stddev=im.sd, radius_dist="lognorm" if im.sd > 0 else "const",
+                           zoom=1.0, iters=im.n_samples, seed=im.seed, algo="pixel", sigma=0.8,
+                           color_mode="rgb" if im.rgb else "luma")
+        d, _offs, offs_in = host.derive(cli, im.w, im.h)
+        desc = capi.make_desc(d.in_w, d.in_h, d.out_w, d.out_h, d.n_samples, d.dist_kind, d.seed, d.zoom,
+                              d.delta, d.rm, d.radius_mean, d.log_mu, d.log_sigma)
+        for c in range(3 if im.rgb else 1):
+            u = synth.sinusoid_field(im.w, im.h, im.seed * 10 + c, n_waves=4, noise=0.02)
+            nrm, _ = host.normalize_plane(u)
+            planes.append((desc, host.lambda_plane(nrm, d.inv_e_pi_r2), np.ascontiguousarray(offs_in)))
+    return planes
+
+
 def prepare(name: str) -> Prepared:
     from film_grain_amd import capi, host
     wl = WORKLOADS[name]

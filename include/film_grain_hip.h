@@ -147,6 +147,34 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
 int fg_set_timing(fg_ctx *ctx, int enable);
 int fg_get_timings(fg_ctx *ctx, float ms[3]);
 
+/* ---- Batched rendering -------------------------------------------------
+ * No counterpart in the reference, which renders one plane per call
+ * (src/color.rs:47-64 loops over planes; src/lib.rs:151-165 dispatches each).
+ * A batch is a fixed set of pixel-wise planes of any sizes and parameters
+ * (N <= 2048 each), with their device buffers.  fg_batch_create plans it:
+ * per-plane geometry and launch shapes, buffers sized from one counting pass
+ * over the current inputs (grain capacity = 1.25 x that count + 65536), and,
+ * with capture_graph set, one HIP graph of the whole pipeline: per-plane grain
+ * generation, one scan over all planes' bins, per-plane sample tables, a
+ * heaviest-first order of all (plane, tile) items, and one persistent splat
+ * launch whose workgroups take items from an atomic counter.  fg_batch_run
+ * enqueues the pipeline (or replays the graph) on `hip_stream`; the inputs may
+ * change between runs, and a run whose grains exceed the capacity renders
+ * nothing and makes fg_batch_status return FG_ECAPACITY (create the batch
+ * again).  Each plane's result equals fg_render_pixelwise_device's bit for
+ * bit.  A batch must not run concurrently with itself. */
+#define FG_ECAPACITY 7
+typedef struct fg_batch fg_batch;
+int fg_batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n_planes, const float *const *d_lambdas,
+                    const float *const *d_offsets_input_xy, float *const *d_outs, int capture_graph,
+                    void *hip_stream, fg_batch **out);
+int fg_batch_run(fg_batch *batch, void *hip_stream);
+/* Waits for `hip_stream` and returns the status of the batch's last run. */
+int fg_batch_status(fg_batch *batch, void *hip_stream);
+/* Number of (plane, tile) work items and the persistent grid size. */
+int fg_batch_info(const fg_batch *batch, int *n_items, int *grid_blocks, int *graph_captured);
+void fg_batch_destroy(fg_batch *batch);
+
 /* Library version string. */
 const char *fg_version(void);
 
