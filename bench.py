@@ -118,12 +118,75 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
                 sample=sample), dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
 
 
+def batch_bench(args):
+    """The uneven batch: 64 images (sides 128-2048, mixed radii, N and
+    channels), every plane resident in HBM; one graph-captured batch run per
+    step, against the same planes rendered one call at a time."""
+    import torch
+    from film_grain_amd import capi, workloads
+    images = workloads.batch_images(master_seed=2024, n_images=args.batch_images)
+    planes = workloads.prepare_batch(images)
+    dev = torch.device("cuda:0")
+    ins = [(d, torch.from_numpy(l).to(dev), torch.from_numpy(o).to(dev)) for d, l, o in planes]
+    outs = [torch.empty((d.out_h, d.out_w), dtype=torch.float32, device=dev) for d, _, _ in ins]
+    px_samples = sum(d.out_w * d.out_h * d.n_samples for d, _, _ in ins)
+    log(f"[bench] batch: {len(images)} images, {len(ins)} planes, {px_samples:.4g} pixel-samples")
+    ctx = capi.Context(0)
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    batch = capi.Batch(ctx, [(d, l, o, out) for (d, l, o), out in zip(ins, outs)], capture_graph=True)
+    log(f"[bench] batch planned in {time.perf_counter() - t0:.2f} s: {batch.info()}")
+    for _ in range(args.warmup):
+        batch.run(stream)
+    batch.status(stream)
+    ref = [o.clone() for o in outs]
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / steps
+
+    batch_s = timed(lambda: batch.run(stream), args.steps)
+    batch.status(stream)
+
+    def per_plane():
+        for (d, l, o), out in zip(ins, outs):
+            ctx.render_device("pixel", d, l, o, out, stream)
+
+    per_plane()
+    ctx.stream_status(stream)
+    single_s = timed(per_plane, max(1, args.steps // 2))
+    ctx.stream_status(stream)
+    same = all(torch.equal(a, b) for a, b in zip(outs, ref))
+    log(f"[bench] batch {batch_s * 1e3:.2f} ms, one call per plane {single_s * 1e3:.2f} ms, identical planes: {same}")
+    batch.close()
+    line = {
+        "metric": "Mpix·samples/s (uneven batch, one graph-captured persistent launch)",
+        "value": round(px_samples / batch_s / 1e6, 2), "unit": "Mpix·samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(images)} images, sides 128-2048, radius U(0.05,0.3), sigma_r/r U(0,0.5), "
+                               f"N in {{32..512}}, luma/RGB, seed 2024", "planes": len(ins),
+                   "pixel_samples": px_samples},
+        "per_plane_calls_ms": round(single_s * 1e3, 3),
+        "speedup_vs_per_plane_calls": round(single_s / batch_s, 3),
+        "identical_to_per_plane_calls": bool(same),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4"])
+    ap.add_argument("--batch-images", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
@@ -139,6 +202,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "batch-c4":
+        batch_bench(args)
+        return
 
     import torch
     from film_grain_amd import autotune, capi

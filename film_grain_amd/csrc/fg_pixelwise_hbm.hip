@@ -661,11 +661,13 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
         __syncthreads();
         if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
         __syncthreads();
-        const int it = s_item;
+        // the item is uniform: say so, so that the plane's fields are scalar loads
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= n_items) break;
         const int2 pt = items[it];
-        const BatchPlane &p = planes[pt.x];
-        splat_tile<BLOCK>(p.a, p.geo, p.st, p.bin_start, p.grains, pt.y, lds, s_queue);
+        const int pl = __builtin_amdgcn_readfirstlane(pt.x), tile = __builtin_amdgcn_readfirstlane(pt.y);
+        const BatchPlane &p = planes[pl];
+        splat_tile<BLOCK>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
     }
 }
 
