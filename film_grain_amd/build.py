@@ -34,7 +34,8 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["fg_pixelwise.hip", "fg_pixelwise_hbm.hip", "fg_grainwise.hip", "fg_batch.hip", "fg_capi.hip"]
+HIP_SOURCES = ["fg_pixelwise.hip", "fg_pixelwise_hbm.hip", "fg_grainwise.hip", "fg_batch.hip", "fg_mlp.hip",
+               "fg_capi.hip"]
 HOST_SOURCES = sorted(p.name for p in (CSRC / "host").glob("*.cpp")) if (CSRC / "host").exists() else []
 
 

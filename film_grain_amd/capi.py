@@ -31,8 +31,11 @@ EXPORTED = [
     "fg_render_grainwise", "fg_render_pixelwise_device", "fg_render_grainwise_device",
     "fg_stream_status", "fg_pixelwise_stats_device", "fg_grainwise_stats_device",
     "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings",
-    "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy", "fg_version",
+    "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy",
+    "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device", "fg_version",
 ]
+MLP_PARAMS = 5313
+MLP_FEATURES = 16
 FG_ECAPACITY = 7
 ERROR_NAMES[FG_ECAPACITY] = "FG_ECAPACITY"
 
@@ -126,6 +129,14 @@ def lib() -> C.CDLL:
     L.fg_batch_info.restype = C.c_int
     L.fg_batch_destroy.argtypes = [vp]
     L.fg_batch_destroy.restype = None
+    u32 = C.c_uint32
+    L.fg_mlp_features_device.argtypes = [vp, vp, u32, u32, u32, u32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.c_int, vp, vp]
+    L.fg_mlp_forward_device.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+    L.fg_mlp_grad_device.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, vp, vp, vp]
+    L.fg_mlp_adam_device.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
+    for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device"):
+        getattr(L, name).restype = C.c_int
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
     _lib = L
@@ -203,6 +214,41 @@ class Context:
                 C.c_void_p(out.data_ptr()), cnt, C.c_void_p(_stream_handle(stream)))
         self._check(rc)
         return [int(v) for v in cnt]
+
+    # ---- per-pixel error model (torch tensors on the device) ----
+    def mlp_features(self, u, out_w, out_h, zoom, radius, radius_sd, sigma, channel, out=None, stream=None):
+        """BF16 features (as int16 tensor [out_h*out_w, 16]) of a normalised
+        intensity plane u (f32 [in_h, in_w])."""
+        import torch
+        in_h, in_w = u.shape
+        if out is None:
+            out = torch.empty((out_h * out_w, MLP_FEATURES), dtype=torch.int16, device=u.device)
+        self._check(self._L.fg_mlp_features_device(
+            self.handle, C.c_void_p(u.data_ptr()), in_w, in_h, out_w, out_h, zoom, radius, radius_sd, sigma,
+            channel, C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
+        return out
+
+    def mlp_forward(self, params, feats, out=None, stream=None):
+        import torch
+        n = feats.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=feats.device)
+        self._check(self._L.fg_mlp_forward_device(self.handle, C.c_void_p(params.data_ptr()),
+                                                  C.c_void_p(feats.data_ptr()), n, C.c_void_p(out.data_ptr()),
+                                                  C.c_void_p(_stream_handle(stream))))
+        return out
+
+    def mlp_grad(self, params, feats, target, grad, loss, n_total=None, stream=None):
+        n = feats.shape[0]
+        self._check(self._L.fg_mlp_grad_device(self.handle, C.c_void_p(params.data_ptr()),
+                                               C.c_void_p(feats.data_ptr()), C.c_void_p(target.data_ptr()), n,
+                                               n_total or n, C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()),
+                                               C.c_void_p(_stream_handle(stream))))
+
+    def mlp_adam(self, params, grad, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        self._check(self._L.fg_mlp_adam_device(self.handle, C.c_void_p(params.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                               C.c_void_p(m.data_ptr()), C.c_void_p(v.data_ptr()), lr, beta1, beta2,
+                                               eps, step, C.c_void_p(_stream_handle(stream))))
 
     def set_timing(self, enable: bool):
         self._check(self._L.fg_set_timing(self.handle, 1 if enable else 0))

@@ -17,6 +17,7 @@
 
 #include "../../include/film_grain_hip.h"
 #include "fg_kernels.h"
+#include "fg_mlp.h"
 #include "fg_pixelwise_hbm.h"
 #include "fg_rng.h"
 
@@ -1246,6 +1247,53 @@ void fg_batch_destroy(fg_batch *b)
     (void)hipDeviceSynchronize();
     batch_free(b);
     delete b;
+}
+
+int fg_mlp_features_device(fg_ctx *ctx, const float *d_u, uint32_t in_w, uint32_t in_h, uint32_t out_w,
+                           uint32_t out_h, float zoom, float radius_mean, float radius_stddev, float sigma,
+                           int channel, uint16_t *d_features, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_u || !d_features) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (in_w == 0 || in_h == 0 || out_w == 0 || out_h == 0 || !(zoom > 0.0f) || channel < 0 || channel > 2)
+        return set_error(ctx, FG_EINVAL, "invalid feature request");
+    const hipError_t e = fg::launch_features(d_u, (int)in_w, (int)in_h, (int)out_w, (int)out_h, zoom, radius_mean,
+                                             radius_stddev, sigma, channel, d_features,
+                                             reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "features");
+}
+
+int fg_mlp_forward_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, int64_t n, float *d_out,
+                          void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_params || !d_features || !d_out || n < 0) return set_error(ctx, FG_EINVAL, "invalid MLP arguments");
+    if (n == 0) return FG_OK;
+    const hipError_t e = fg::launch_mlp_forward(d_params, d_features, n, d_out, reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "MLP forward");
+}
+
+int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, const float *d_target,
+                       int64_t n, int64_t n_total, float *d_grad, float *d_loss, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_params || !d_features || !d_target || !d_grad || !d_loss || n < 0 || n_total <= 0)
+        return set_error(ctx, FG_EINVAL, "invalid MLP arguments");
+    if (n == 0) return FG_OK;
+    const hipError_t e = fg::launch_mlp_grad(d_params, d_features, d_target, n, 1.0f / (float)n_total, d_grad, d_loss,
+                                             reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "MLP gradient");
+}
+
+int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
+                       float beta1, float beta2, float eps, int step, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_params || !d_grad || !d_m || !d_v || step < 1) return set_error(ctx, FG_EINVAL, "invalid Adam arguments");
+    const float bc1 = 1.0f - std::pow(beta1, (float)step), bc2 = 1.0f - std::pow(beta2, (float)step);
+    const hipError_t e = fg::launch_mlp_adam(d_params, d_grad, d_m, d_v, lr, beta1, beta2, eps, bc1, bc2,
+                                             reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Adam step");
 }
 
 const char *fg_version(void) { return "film_grain_amd 0.1.0 (gfx950)"; }

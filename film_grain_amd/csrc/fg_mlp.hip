@@ -1,0 +1,354 @@
+// fg_mlp.hip -- per-pixel Monte-Carlo error model: features and a small MLP
+// (16 inputs, 12 used -> 64 -> 64 -> 1, ReLU) with BF16 weights and FP32
+// accumulation on the gfx950 matrix cores, forward and backward.
+//
+// No counterpart in the reference.  The model predicts log(s), s the
+// per-sample standard deviation sqrt(p (1 - p)) of a pixel's coverage
+// indicator, from local statistics of the intensity plane; the sample budget
+// of the adaptive renderer is proportional to s.
+//
+// Layout.  Parameters are one f32 array (fg_mlp.h offsets): W1 [16][64]
+// (rows 12..15 zero), b1 [64], W2 [64][64] (input-major), b2 [64], w3 [64],
+// b3.  Each workgroup converts them to BF16 in LDS, in the orders the matrix
+// instructions read: W1T [64][16], W2T [64][64] (output-major) and W2 [64][64].
+// Features are BF16 [n][16].  A wave works on 16 rows at a time:
+//   layer 1: Z1 = X W1 + b1      4 x v_mfma_f32_16x16x16_bf16 (one per 16 outputs)
+//   layer 2: Z2 = H1 W2 + b2     8 x v_mfma_f32_16x16x32_bf16, H1 = relu(Z1) in BF16
+//                                 through LDS (accumulator rows -> operand rows)
+//   layer 3: y = H2 . w3 + b3     FP32 dot products (a 64-vector), H2 = relu(Z2)
+// Backward of the mean squared error to a target t (fused, per tile):
+//   g = 2 (y - t) / n, dZ2 = g w3 * [Z2 > 0], dZ1 = (dZ2 W2^T) * [Z1 > 0]
+//   dW3 = H2^T g, dW2 = H1^T dZ2, dW1 = X^T dZ1 (MFMA over the 16 rows), biases
+//   by sums.  Products feeding a matrix instruction use BF16 operands (dZ2,
+//   dZ1, H1, X); their accumulations are FP32.  The CPU reference in the
+//   tests rounds the same operands to BF16 at the same points.
+// Matrix operand layouts (wave64): 16x16x16: lane l holds A[l%16][4(l/16)+0..3],
+// B[4(l/16)+0..3][l%16]; 16x16x32: A[l%16][8(l/16)+0..7], B[8(l/16)+0..7][l%16];
+// both: D[4(l/16)+0..3][l%16].
+#include <hip/hip_runtime.h>
+
+#include "fg_kernels.h"
+#include "fg_mlp.h"
+
+namespace fg {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// f32 -> bf16 bits, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t bf16_bits(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+struct MlpLds {
+    uint16_t w1t[64 * 16];  // [j][k]
+    uint16_t w2t[64 * 64];  // [j_out][k_in]
+    uint16_t w2[64 * 64];   // [k_in][j_out]
+    float b1[64], b2[64], w3[64], b3;
+    uint16_t h[4][16 * 64];  // per wave: a 16 x 64 BF16 tile (H1, then dZ2)
+    uint16_t x[4][16 * 16];  // per wave: the feature tile
+};
+
+__device__ void mlp_load(const float *P, MlpLds &s)
+{
+    for (int i = threadIdx.x; i < 64 * 16; i += blockDim.x) {
+        const int j = i / 16, k = i % 16;
+        s.w1t[i] = bf16_bits(P[kMlpW1 + k * 64 + j]);
+    }
+    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+        const int j = i / 64, k = i % 64;
+        s.w2t[i] = bf16_bits(P[kMlpW2 + k * 64 + j]);
+        s.w2[i] = bf16_bits(P[kMlpW2 + i]);
+    }
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        s.b1[i] = P[kMlpB1 + i];
+        s.b2[i] = P[kMlpB2 + i];
+        s.w3[i] = bf16_float(bf16_bits(P[kMlpW3 + i]));
+    }
+    if (threadIdx.x == 0) s.b3 = P[kMlpB3];
+    __syncthreads();
+}
+
+// Forward of one 16-row tile for the calling wave.  z1/z2 receive the
+// pre-activations in accumulator layout (4 column blocks x 4 rows), y the
+// row outputs (lanes of each 16-lane group hold the same 4 rows).
+__device__ __forceinline__ void mlp_tile_forward(const MlpLds &s, uint16_t *hbuf, const uint16_t *xt, v4f z1[4],
+                                                 v4f z2[4], float y[4])
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    // layer 1: A = X[r][4g..4g+3], B = W1[4g..][cb*16 + r]
+    v4s a1;
+    for (int t = 0; t < 4; ++t) a1[t] = (short)xt[r * 16 + 4 * g + t];
+    for (int cb = 0; cb < 4; ++cb) {
+        v4s b;
+        for (int t = 0; t < 4; ++t) b[t] = (short)s.w1t[(cb * 16 + r) * 16 + 4 * g + t];
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, c, 0, 0, 0);
+        const float bias = s.b1[cb * 16 + r];
+        for (int i = 0; i < 4; ++i) {
+            c[i] += bias;
+            hbuf[(4 * g + i) * 64 + cb * 16 + r] = bf16_bits(fmaxf(c[i], 0.0f));
+        }
+        z1[cb] = c;
+    }
+    // layer 2: A = H1[r][kc*32 + 8g..+7], B = W2[kc*32 + 8g..][cb*16 + r]
+    v8bf a2[2];
+    for (int kc = 0; kc < 2; ++kc)
+        for (int t = 0; t < 8; ++t) a2[kc][t] = __builtin_bit_cast(__bf16, hbuf[r * 64 + kc * 32 + 8 * g + t]);
+    for (int cb = 0; cb < 4; ++cb) {
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kc = 0; kc < 2; ++kc) {
+            v8bf b;
+            for (int t = 0; t < 8; ++t)
+                b[t] = __builtin_bit_cast(__bf16, s.w2t[(cb * 16 + r) * 64 + kc * 32 + 8 * g + t]);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[kc], b, c, 0, 0, 0);
+        }
+        const float bias = s.b2[cb * 16 + r];
+        for (int i = 0; i < 4; ++i) c[i] += bias;
+        z2[cb] = c;
+    }
+    // layer 3: y[row] = sum_j relu(Z2[row][j]) w3[j] + b3; lane r of group g
+    // holds columns r, 16 + r, 32 + r, 48 + r of rows 4g..4g+3, so the sum
+    // runs over its four columns, then over the 16 lanes of the group
+    for (int i = 0; i < 4; ++i) {
+        float acc = 0.0f;
+        for (int cb = 0; cb < 4; ++cb) acc += fmaxf(z2[cb][i], 0.0f) * s.w3[cb * 16 + r];
+        for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o, 64);
+        y[i] = acc + s.b3;
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const float *P, const uint16_t *X, long long n, float *out)
+{
+    __shared__ MlpLds s;
+    mlp_load(P, s);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const long long tiles = (n + 15) / 16;
+    for (long long t = (long long)blockIdx.x * 4 + wid; t < tiles; t += (long long)gridDim.x * 4) {
+        uint16_t *xt = s.x[wid];
+        for (int i = lane; i < 256; i += 64) {
+            const long long row = t * 16 + i / 16;
+            xt[i] = row < n ? X[row * 16 + (i % 16)] : (uint16_t)0;
+        }
+        v4f z1[4], z2[4];
+        float y[4];
+        mlp_tile_forward(s, s.h[wid], xt, z1, z2, y);
+        if (r == 0)
+            for (int i = 0; i < 4; ++i) {
+                const long long row = t * 16 + 4 * g + i;
+                if (row < n) out[row] = y[i];
+            }
+    }
+}
+
+// Fused forward + backward of the mean squared error over rows [0, n) with
+// loss scale 1 / n_total; gradients are added to grad (kMlpParams f32) and
+// the loss sum to *loss.
+__global__ __launch_bounds__(256) void mlp_grad_kernel(const float *P, const uint16_t *X, const float *target,
+                                                       long long n, float inv_total, float *grad, float *loss)
+{
+    __shared__ MlpLds s;
+    __shared__ float s_grad[kMlpParams];
+    __shared__ float s_loss;
+    mlp_load(P, s);
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) s_grad[i] = 0.0f;
+    if (threadIdx.x == 0) s_loss = 0.0f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    v4f dw2[4][4], dw1[4];
+    for (int a = 0; a < 4; ++a) {
+        dw1[a] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < 4; ++b) dw2[a][b] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float db1[4] = {0, 0, 0, 0}, db2[4] = {0, 0, 0, 0}, dw3[4] = {0, 0, 0, 0}, db3 = 0.0f, lsum = 0.0f;
+    const long long tiles = (n + 15) / 16;
+    for (long long t = (long long)blockIdx.x * 4 + wid; t < tiles; t += (long long)gridDim.x * 4) {
+        uint16_t *xt = s.x[wid];
+        uint16_t *hb = s.h[wid];
+        for (int i = lane; i < 256; i += 64) {
+            const long long row = t * 16 + i / 16;
+            xt[i] = row < n ? X[row * 16 + (i % 16)] : (uint16_t)0;
+        }
+        v4f z1[4], z2[4];
+        float y[4];
+        mlp_tile_forward(s, hb, xt, z1, z2, y);
+        // output gradient (rows past n contribute nothing)
+        float gr[4];
+        for (int i = 0; i < 4; ++i) {
+            const long long row = t * 16 + 4 * g + i;
+            const float d = row < n ? y[i] - target[row] : 0.0f;
+            gr[i] = 2.0f * d * inv_total;
+            if (r == 0) lsum += d * d;
+        }
+        // layer 3 and dZ2 (accumulator layout), dW2 = H1^T dZ2 on MFMA
+        v4f dz2[4];
+        for (int cb = 0; cb < 4; ++cb) {
+            const float w = s.w3[cb * 16 + r];
+            for (int i = 0; i < 4; ++i) {
+                const float h2 = fmaxf(z2[cb][i], 0.0f);
+                dw3[cb] += gr[i] * h2;
+                dz2[cb][i] = z2[cb][i] > 0.0f ? gr[i] * w : 0.0f;
+                db2[cb] += dz2[cb][i];
+            }
+        }
+        if (r == 0)
+            for (int i = 0; i < 4; ++i) db3 += gr[i];
+        // dZ2 as BF16 B operand (rows 4g.., column r) and A^T operand via LDS
+        v4s bz[4];
+        for (int cb = 0; cb < 4; ++cb)
+            for (int i = 0; i < 4; ++i) bz[cb][i] = (short)bf16_bits(dz2[cb][i]);
+        // A = H1^T[kb*16 + r][rows 4g..4g+3] from the H1 tile (row-major)
+        for (int kb = 0; kb < 4; ++kb) {
+            v4s a;
+            for (int i = 0; i < 4; ++i) a[i] = (short)hb[(4 * g + i) * 64 + kb * 16 + r];
+            for (int cb = 0; cb < 4; ++cb)
+                dw2[kb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bz[cb], dw2[kb][cb], 0, 0, 0);
+        }
+        // dH1 = dZ2 W2^T: A = dZ2[r][kc*32 + 8g..] through LDS, B[b][a] = W2[a][b]
+        for (int cb = 0; cb < 4; ++cb)
+            for (int i = 0; i < 4; ++i) hb[(4 * g + i) * 64 + cb * 16 + r] = (uint16_t)bz[cb][i];
+        v8bf az[2];
+        for (int kc = 0; kc < 2; ++kc)
+            for (int q = 0; q < 8; ++q) az[kc][q] = __builtin_bit_cast(__bf16, hb[r * 64 + kc * 32 + 8 * g + q]);
+        v4f dz1[4];
+        for (int cb = 0; cb < 4; ++cb) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int kc = 0; kc < 2; ++kc) {
+                v8bf b;
+                for (int q = 0; q < 8; ++q)
+                    b[q] = __builtin_bit_cast(__bf16, s.w2[(cb * 16 + r) * 64 + kc * 32 + 8 * g + q]);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(az[kc], b, c, 0, 0, 0);
+            }
+            for (int i = 0; i < 4; ++i) {
+                c[i] = z1[cb][i] > 0.0f ? c[i] : 0.0f;
+                db1[cb] += c[i];
+            }
+            dz1[cb] = c;
+        }
+        // dW1 = X^T dZ1: A = X^T[r][rows 4g..] (feature r), B = dZ1 (BF16)
+        v4s ax;
+        for (int i = 0; i < 4; ++i) ax[i] = (short)xt[(4 * g + i) * 16 + r];
+        for (int cb = 0; cb < 4; ++cb) {
+            v4s b;
+            for (int i = 0; i < 4; ++i) b[i] = (short)bf16_bits(dz1[cb][i]);
+            dw1[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ax, b, dw1[cb], 0, 0, 0);
+        }
+    }
+    // reduce: accumulator blocks hold D[4g+i][r]
+    for (int kb = 0; kb < 4; ++kb)
+        for (int cb = 0; cb < 4; ++cb)
+            for (int i = 0; i < 4; ++i)
+                atomicAdd(&s_grad[kMlpW2 + (kb * 16 + 4 * g + i) * 64 + cb * 16 + r], dw2[kb][cb][i]);
+    for (int cb = 0; cb < 4; ++cb)
+        for (int i = 0; i < 4; ++i) atomicAdd(&s_grad[kMlpW1 + (4 * g + i) * 64 + cb * 16 + r], dw1[cb][i]);
+    for (int cb = 0; cb < 4; ++cb) {
+        atomicAdd(&s_grad[kMlpB1 + cb * 16 + r], db1[cb]);
+        atomicAdd(&s_grad[kMlpB2 + cb * 16 + r], db2[cb]);
+        atomicAdd(&s_grad[kMlpW3 + cb * 16 + r], dw3[cb]);
+    }
+    if (r == 0) {
+        atomicAdd(&s_grad[kMlpB3], db3);
+        atomicAdd(&s_loss, lsum);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x)
+        if (s_grad[i] != 0.0f) atomicAdd(&grad[i], s_grad[i]);
+    if (threadIdx.x == 0) atomicAdd(loss, s_loss);
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; rows 12..15 of W1 stay 0.
+__global__ void mlp_adam_kernel(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2,
+                                float eps, float bc1, float bc2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    if (i >= kMlpW1 + 12 * 64 && i < kMlpW1 + 16 * 64) {
+        P[i] = 0.0f;
+        return;
+    }
+    const float gi = grad[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// Features of output pixel (x, y) of a plane: the normalised intensity u of
+// the input pixel it maps to (floor((x + 0.5) / zoom), clamped), the mean and
+// variance of u over its 3x3 neighbourhood (edges clamped), the gradient
+// magnitude (central differences), u (1 - u), the radius law, filter sigma and
+// zoom (scaled to order one), and the channel one-hot.
+__global__ __launch_bounds__(256) void feature_kernel(const float *u, int in_w, int in_h, int out_w, int out_h,
+                                                      float inv_zoom, float f_radius, float f_sd, float f_sigma,
+                                                      float f_zoom, int channel, uint16_t *X)
+{
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (long long)out_w * out_h) return;
+    const int y = (int)(p / out_w), x = (int)(p - (long long)y * out_w);
+    const int ix = min(max((int)floorf(((float)x + 0.5f) * inv_zoom), 0), in_w - 1);
+    const int iy = min(max((int)floorf(((float)y + 0.5f) * inv_zoom), 0), in_h - 1);
+    auto at = [&](int dx, int dy) {
+        const int xx = min(max(ix + dx, 0), in_w - 1), yy = min(max(iy + dy, 0), in_h - 1);
+        return u[(size_t)yy * in_w + xx];
+    };
+    float s = 0.0f, s2 = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const float v = at(dx, dy);
+            s += v;
+            s2 += v * v;
+        }
+    const float mean = s * (1.0f / 9.0f);
+    const float var = fmaxf(s2 * (1.0f / 9.0f) - mean * mean, 0.0f);
+    const float gx = 0.5f * (at(1, 0) - at(-1, 0)), gy = 0.5f * (at(0, 1) - at(0, -1));
+    const float c = at(0, 0);
+    float f[16] = {c,       mean,       sqrtf(var), sqrtf(gx * gx + gy * gy), c * (1.0f - c), f_radius, f_sd,
+                   f_sigma, f_zoom,     channel == 0 ? 1.0f : 0.0f, channel == 1 ? 1.0f : 0.0f,
+                   channel == 2 ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint16_t *dst = X + p * 16;
+    for (int i = 0; i < 16; ++i) dst[i] = bf16_bits(f[i]);
+}
+
+hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s)
+{
+    const long long tiles = (n + 15) / 16;
+    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 8192);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const float *P, const uint16_t *X, const float *target, long long n, float inv_total,
+                           float *grad, float *loss, hipStream_t s)
+{
+    const long long tiles = (n + 15) / 16;
+    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 1024);
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, target, n, inv_total, grad,
+                       loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2, float eps,
+                           float bc1, float bc2, hipStream_t s)
+{
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, grad, m, v, lr, b1, b2,
+                       eps, bc1, bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const float *u, int in_w, int in_h, int out_w, int out_h, float zoom, float radius,
+                           float sd, float sigma, int channel, uint16_t *X, hipStream_t s)
+{
+    const long long n = (long long)out_w * out_h;
+    hipLaunchKernelGGL(feature_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, u, in_w, in_h, out_w, out_h,
+                       1.0f / zoom, radius / 0.3f, sd / 0.15f, sigma / 2.0f, zoom / 4.0f, channel, X);
+    return hipGetLastError();
+}
+
+}  // namespace fg

@@ -175,6 +175,32 @@ int fg_batch_status(fg_batch *batch, void *hip_stream);
 int fg_batch_info(const fg_batch *batch, int *n_items, int *grid_blocks, int *graph_captured);
 void fg_batch_destroy(fg_batch *batch);
 
+/* ---- Per-pixel Monte-Carlo error model -----------------------------------
+ * No counterpart in the reference.  A small MLP (16 inputs, 12 used -> 64 ->
+ * 64 -> 1, ReLU) predicts log s, s = sqrt(p (1 - p)) the per-sample standard
+ * deviation of an output pixel's coverage indicator, from local statistics
+ * of the normalised intensity plane.  BF16 weights and operands, FP32
+ * accumulation on the matrix cores; forward, fused backward of the mean
+ * squared error, and an Adam step, all on device memory and asynchronous on
+ * `hip_stream`.  Parameters: FG_MLP_PARAMS floats (layout in fg_mlp.h).
+ * Features: FG_MLP_FEATURES BF16 values per output pixel:
+ *   u, 3x3 mean, 3x3 std, gradient magnitude, u(1-u), radius/0.3,
+ *   radius_stddev/0.15, sigma/2, zoom/4, channel one-hot (3), 4 zeros. */
+#define FG_MLP_PARAMS 5313
+#define FG_MLP_FEATURES 16
+int fg_mlp_features_device(fg_ctx *ctx, const float *d_u, uint32_t in_w, uint32_t in_h, uint32_t out_w,
+                           uint32_t out_h, float zoom, float radius_mean, float radius_stddev, float sigma,
+                           int channel, uint16_t *d_features, void *hip_stream);
+int fg_mlp_forward_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, int64_t n, float *d_out,
+                          void *hip_stream);
+/* Adds the gradient of (1/n_total) sum (y - target)^2 over these n rows to
+ * d_grad and the sum of squared errors to *d_loss. */
+int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, const float *d_target,
+                       int64_t n, int64_t n_total, float *d_grad, float *d_loss, void *hip_stream);
+/* One Adam step (step counts from 1) on d_params with moments d_m, d_v. */
+int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
+                       float beta1, float beta2, float eps, int step, void *hip_stream);
+
 /* Library version string. */
 const char *fg_version(void);
 
