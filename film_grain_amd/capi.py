@@ -32,7 +32,8 @@ EXPORTED = [
     "fg_stream_status", "fg_pixelwise_stats_device", "fg_grainwise_stats_device",
     "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings",
     "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy",
-    "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device", "fg_version",
+    "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
+    "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_version",
 ]
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
@@ -135,7 +136,10 @@ def lib() -> C.CDLL:
     L.fg_mlp_forward_device.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
     L.fg_mlp_grad_device.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, vp, vp, vp]
     L.fg_mlp_adam_device.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
-    for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device"):
+    L.fg_mlp_budget_device.argtypes = [vp, vp, vp, C.c_int64, C.c_float, C.c_int, C.c_int, vp, vp]
+    L.fg_render_pixelwise_adaptive_device.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, vp, vp]
+    for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
+                 "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device"):
         getattr(L, name).restype = C.c_int
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
@@ -204,6 +208,14 @@ class Context:
                 C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
         self._check(rc)
 
+    def render_adaptive_device(self, desc: PlaneDesc, lam, offsets, budget, out, stream=None):
+        """Pixel-wise render where output pixel i uses the first budget[i]
+        (u16) of the desc.n_samples offsets."""
+        rc = self._L.fg_render_pixelwise_adaptive_device(
+            self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
+            C.c_void_p(budget.data_ptr()), C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
+        self._check(rc)
+
     def stream_status(self, stream=None):
         self._check(self._L.fg_stream_status(self.handle, C.c_void_p(_stream_handle(stream))))
 
@@ -249,6 +261,19 @@ class Context:
         self._check(self._L.fg_mlp_adam_device(self.handle, C.c_void_p(params.data_ptr()), C.c_void_p(grad.data_ptr()),
                                                C.c_void_p(m.data_ptr()), C.c_void_p(v.data_ptr()), lr, beta1, beta2,
                                                eps, step, C.c_void_p(_stream_handle(stream))))
+
+    def mlp_budget(self, pred, feats, n, c, n_min, n_max, out=None, stream=None):
+        """u16 sample budgets (int16 tensor) clamp(ceil(c * s), n_min, n_max):
+        s = exp(pred) or, with pred None, sqrt(u (1 - u)) from the features."""
+        import torch
+        dev = (pred if pred is not None else feats).device
+        if out is None:
+            out = torch.empty(n, dtype=torch.int16, device=dev)
+        self._check(self._L.fg_mlp_budget_device(
+            self.handle, C.c_void_p(pred.data_ptr() if pred is not None else 0),
+            C.c_void_p(feats.data_ptr() if feats is not None else 0), n, c, n_min, n_max,
+            C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
+        return out
 
     def set_timing(self, enable: bool):
         self._check(self._L.fg_set_timing(self.handle, 1 if enable else 0))

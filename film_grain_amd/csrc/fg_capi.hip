@@ -52,6 +52,9 @@ struct fg_ctx {
     DevBuf bin_mask;  // nonempty cells per bin (256 bits)
     DevBuf pix_state; // grain-wise: per input pixel grain count and stream state (36 B)
     DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
+    DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
+    std::vector<int> level_n;
+    std::vector<fg::SampleTable> level_tables;
 
     hipEvent_t scratch_done = nullptr;  // recorded after the last kernel that reads the scratch
     bool scratch_pending = false;
@@ -105,6 +108,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->pix_state);
     free_buf(ctx->tiles);
     free_buf(ctx->tile_temp);
+    free_buf(ctx->levels);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -488,6 +492,41 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
+    if (e == hipSuccess && a.budget) {
+        // adaptive sampling: one sample table per level (the first 16, 32,
+        // ... samples, then all), and each tile's level
+        std::vector<int> &ln = ctx->level_n;
+        ln.clear();
+        for (int l = 16; l < a.n_samples; l *= 2) ln.push_back(l);
+        ln.push_back(a.n_samples);
+        const int nl = (int)ln.size();
+        size_t bytes = 0;
+        for (int l : ln) bytes += (fg::sample_table_bytes(l, sample_buckets((uint32_t)l)) + 255) & ~(size_t)255;
+        const size_t head = 256 + (((size_t)nl * sizeof(fg::SampleTable) + 255) & ~(size_t)255);
+        if ((rc = ensure_bytes(ctx, ctx->levels, head + bytes + (size_t)a.n_tiles, what)) != FG_OK) return rc;
+        unsigned char *base = static_cast<unsigned char *>(ctx->levels.p);
+        int *d_level_n = reinterpret_cast<int *>(base);
+        fg::SampleTable *d_tables = reinterpret_cast<fg::SampleTable *>(base + 256);
+        unsigned char *p = base + head;
+        ctx->level_tables.assign(nl, fg::SampleTable{});
+        for (int i = 0; i < nl && e == hipSuccess; ++i) {
+            fg::SampleTable &lt = ctx->level_tables[i];
+            e = fg::launch_sample_table(a.offs, ln[i], a.zoom, sample_buckets((uint32_t)ln[i]), p, &lt, s);
+            lt.eps = st.eps;  // bounds of the whole offset set hold for every prefix
+            lt.safe_r = st.safe_r;
+            p += (fg::sample_table_bytes(ln[i], sample_buckets((uint32_t)ln[i])) + 255) & ~(size_t)255;
+        }
+        uint8_t *d_tile_level = p;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tables, ctx->level_tables.data(), sizeof(fg::SampleTable) * nl,
+                               hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_level_n, ln.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s);
+        a.level_tables = d_tables;
+        a.tile_level = d_tile_level;
+        if (e == hipSuccess) e = fg::launch_tile_levels(a, d_level_n, nl, d_tile_level, s);
+        // the host arrays above must outlive the copies
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a, g, start, order.w, order.ids, s);
     if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);
     a.tile_list = order.order;
@@ -547,10 +586,14 @@ int pixel_args(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const
     return FG_OK;
 }
 
-int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
-                 float *d_out, hipStream_t s, bool stats)
+// Pixel-wise render; with budget set (adaptive sampling), only the HBM splat
+// formulation applies.
+int do_pixelwise_budget(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
+                        float *d_out, hipStream_t s, bool stats, const uint16_t *budget)
 {
     int variant = stats ? 1 : pixel_variant(ctx, d);
+    if (budget && variant != 3)
+        return set_error(ctx, FG_EINVAL, "adaptive sampling needs the HBM splat formulation (n_samples <= 2048)");
     fg::PixGeom geom{};
     float4 range{};
     hipError_t e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offs), (int)d->n_samples,
@@ -564,10 +607,13 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     }
     int tw, th, lds;
     if (variant == 3 && !hbm_config(ctx, d, &tw, &th, &lds)) variant = 2;
+    if (budget && variant != 3)
+        return set_error(ctx, FG_EINVAL, "adaptive sampling: the plane's cell grid does not fit the HBM splat path");
     if (variant != 3) pixel_config(ctx, d, variant, &tw, &th, &lds);
     fg::PixelArgs a{};
     int rc0 = pixel_args(ctx, d, d_lambda, d_offs, d_out, tw, th, lds, ctx->d_range, ctx->d_status, &a);
     if (rc0 != FG_OK) return rc0;
+    a.budget = budget;
     const size_t bits = (((size_t)tw * th * a.words * 4) + 15) & ~(size_t)15;
     // the tile splat kernel needs room for one grain record (32 B) per thread
     const bool splat = variant == 2;
@@ -584,6 +630,12 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     return FG_OK;
+}
+
+int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
+                 float *d_out, hipStream_t s, bool stats)
+{
+    return do_pixelwise_budget(ctx, d, d_lambda, d_offs, d_out, s, stats, nullptr);
 }
 
 // Grain-wise formulation: 1 the reference's loop order over all (grain,
@@ -1072,6 +1124,20 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
                         reinterpret_cast<hipStream_t>(hip_stream), false);
 }
 
+int fg_render_pixelwise_adaptive_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                                        const float *d_offsets_input_xy, const uint16_t *d_budget, float *d_out,
+                                        void *hip_stream)
+{
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offsets_input_xy || !d_budget || !d_out) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    return do_pixelwise_budget(ctx, desc, d_lambda, d_offsets_input_xy, d_out,
+                               reinterpret_cast<hipStream_t>(hip_stream), false, d_budget);
+}
+
 int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
                                const float *d_offsets_xy, float *d_out, void *hip_stream)
 {
@@ -1283,6 +1349,19 @@ int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_fea
     const hipError_t e = fg::launch_mlp_grad(d_params, d_features, d_target, n, 1.0f / (float)n_total, d_grad, d_loss,
                                              reinterpret_cast<hipStream_t>(hip_stream));
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "MLP gradient");
+}
+
+int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n, float c,
+                         int n_min, int n_max, uint16_t *d_budget, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if ((!d_pred_log_s && !d_features) || !d_budget || n < 0 || n_min < 1 || n_max < n_min || n_max > 65535 ||
+        !(c >= 0.0f))
+        return set_error(ctx, FG_EINVAL, "invalid budget arguments");
+    if (n == 0) return FG_OK;
+    const hipError_t e = fg::launch_budget(d_pred_log_s, d_features, n, c, n_min, n_max, d_budget,
+                                           reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "budget");
 }
 
 int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,

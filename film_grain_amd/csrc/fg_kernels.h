@@ -41,6 +41,14 @@ struct PixelArgs {
     // capacity check reports it).  grain_total == nullptr: unchecked.
     const uint32_t *grain_total;
     uint32_t grain_cap;
+    // adaptive sampling (HBM splat only): per output pixel its sample count
+    // n in [1, n_samples] (the first n offsets, as a render with N = n uses),
+    // per tile the index of the sample table covering its largest count, and
+    // those tables (struct SampleTable, one per level).  budget == nullptr:
+    // every pixel uses all n_samples.
+    const uint16_t *budget;
+    const uint8_t *tile_level;
+    const void *level_tables;
 };
 
 // Grain-wise integrator launch (src/grainwise.rs:12-124).

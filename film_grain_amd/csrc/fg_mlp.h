@@ -23,6 +23,8 @@ hipError_t launch_features(const float *u, int in_w, int in_h, int out_w, int ou
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s);
 hipError_t launch_mlp_grad(const float *P, const uint16_t *X, const float *target, long long n, float inv_total,
                            float *grad, float *loss, hipStream_t s);
+hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, int nmin, int nmax,
+                         uint16_t *budget, hipStream_t s);
 hipError_t launch_mlp_adam(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2, float eps,
                            float bc1, float bc2, hipStream_t s);
 

@@ -316,6 +316,28 @@ __global__ __launch_bounds__(256) void feature_kernel(const float *u, int in_w, 
     for (int i = 0; i < 16; ++i) dst[i] = bf16_bits(f[i]);
 }
 
+// Sample budget n = clamp(ceil(c * s), nmin, nmax) per pixel, from s =
+// exp(predicted log s) (pred != nullptr) or, for the allocator without the
+// model, s = sqrt(u (1 - u)) from feature 4 of X.
+__global__ __launch_bounds__(256) void budget_kernel(const float *pred, const uint16_t *X, long long n, float c,
+                                                     int nmin, int nmax, uint16_t *budget)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float s = pred ? expf(pred[i]) : sqrtf(fmaxf(bf16_float(X[i * 16 + 4]), 0.0f));
+    const float v = ceilf(c * s);
+    const int b = v >= (float)nmax ? nmax : (v <= (float)nmin ? nmin : (int)v);
+    budget[i] = (uint16_t)b;
+}
+
+hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, int nmin, int nmax,
+                         uint16_t *budget, hipStream_t s)
+{
+    hipLaunchKernelGGL(budget_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pred, X, n, c, nmin, nmax,
+                       budget);
+    return hipGetLastError();
+}
+
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s)
 {
     const long long tiles = (n + 15) / 16;

@@ -27,6 +27,7 @@ struct SampleTable {
     const uint16_t *qidx;  // qbuckets + 1 entries: first position whose fraction >= q / qbuckets
     const float4 *ext;     // (min X, max X, min Y, max Y) of the scaled offsets
     const float *key;      // the fractions, same order
+    int n;                 // samples in the table
     int qbuckets;          // power of two
     float eps;             // widening of the fraction interval (covers f32 rounding of the box)
     float safe_r;          // a grain of radius <= safe_r that passes the disk test of a point
@@ -79,6 +80,10 @@ struct BatchPlane {
     long long item_base;        // first (plane, tile) item of this plane
 };
 
+// Adaptive sampling: tile_level[t] = first level (level_n ascending) covering
+// the largest budget in tile t (a.budget).
+hipError_t launch_tile_levels(const PixelArgs &a, const int *level_n, int n_levels, uint8_t *tile_level,
+                              hipStream_t s);
 // Pointers of a sample table laid out in buf (as launch_sample_table does).
 void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t);
 // Resident workgroups of the persistent splat kernel per CU.

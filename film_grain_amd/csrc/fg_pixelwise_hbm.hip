@@ -349,6 +349,7 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
     t->k = reinterpret_cast<uint16_t *>(p);
     p += align16((size_t)n * 2);
     t->qidx = reinterpret_cast<uint16_t *>(p);
+    t->n = n;
     t->qbuckets = qbuckets;
 }
 
@@ -377,8 +378,38 @@ __global__ __launch_bounds__(kBlock) void pixel_tile_weights_kernel(PixelArgs a,
     if (r.bx0 <= r.bx1)
         for (int by = r.by0; by <= r.by1; ++by)
             sum += bin_start[by * geo.nbx + r.bx1 + 1] - bin_start[by * geo.nbx + r.bx0];
+    if (a.budget) {  // adaptive: the work also scales with the tile's samples
+        const int n = reinterpret_cast<const SampleTable *>(a.level_tables)[a.tile_level[tile]].n;
+        sum = (uint32_t)min((unsigned long long)sum * (unsigned long long)n / 16ull, 0xFFFFFFFFull);
+    }
     w[tile] = sum;
     ids[tile] = tile;
+}
+
+// Adaptive sampling: per tile, the first level whose sample count covers the
+// largest budget among its pixels (levels ascending, the last = n_samples).
+__global__ __launch_bounds__(kBlock) void tile_levels_kernel(PixelArgs a, const int *level_n, int n_levels,
+                                                             uint8_t *tile_level)
+{
+    const int tile = blockIdx.x * kBlock + threadIdx.x;
+    if (tile >= a.n_tiles) return;
+    const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
+    const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
+    const int X1 = min(X0 + a.tile_w, a.out_w), Y1 = min(Y0 + a.tile_h, a.out_h);
+    int m = 1;
+    for (int y = Y0; y < Y1; ++y)
+        for (int x = X0; x < X1; ++x) m = max(m, (int)a.budget[(size_t)y * a.out_w + x]);
+    int l = 0;
+    while (l + 1 < n_levels && level_n[l] < m) ++l;
+    tile_level[tile] = (uint8_t)l;
+}
+
+hipError_t launch_tile_levels(const PixelArgs &a, const int *level_n, int n_levels, uint8_t *tile_level,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(tile_levels_kernel, dim3((unsigned)((a.n_tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       a, level_n, n_levels, tile_level);
+    return hipGetLastError();
 }
 
 hipError_t launch_pixel_tile_weights(const PixelArgs &a, const PixGeom &g, const uint32_t *bin_start, uint32_t *w,
@@ -407,7 +438,7 @@ __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_re
 // workgroup's dynamic LDS (a.lds_bytes), s_queue its per-wave queues.  Starts
 // with a barrier, so a workgroup may call it for tile after tile.
 template <int BLOCK>
-__device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st,
+__device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st_all,
                                            const uint32_t *bin_start, const float4 *grains, int tile,
                                            unsigned char *lds, uint32_t (*s_queue)[kQueueCap])
 {
@@ -431,7 +462,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     t.J0 = geo.Jy0;
     const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
     const int fast = a.div_fast;
-    const int N = a.n_samples, words = a.words, Q = st.qbuckets;
+    // adaptive sampling: the tile's level table holds its first N samples
+    const SampleTable st =
+        a.budget ? reinterpret_cast<const SampleTable *>(a.level_tables)[a.tile_level[tile]] : st_all;
+    const int N = st.n, words = a.words, Q = st.qbuckets;
     const TileRegion reg = tile_region(a, geo, t.X0, t.Y0, t.X1, t.Y1);
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
     const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
@@ -632,8 +666,17 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const int ox = p % tw, oy = p / tw;
         const uint32_t *pw = bits + ((size_t)oy * a.tile_w + ox) * words;
         uint32_t count = 0;
-        for (int w = 0; w < words; ++w) count += __popc(pw[w]);
-        a.out[(size_t)(t.Y0 + oy) * a.out_w + (t.X0 + ox)] = (float)count * a.inv_samples;
+        const size_t po = (size_t)(t.Y0 + oy) * a.out_w + (t.X0 + ox);
+        if (a.budget) {
+            // the pixel's first n samples, as a render with N = n counts them
+            const int n = min(max((int)a.budget[po], 1), N);
+            for (int w = 0; w < (n >> 5); ++w) count += __popc(pw[w]);
+            if (n & 31) count += __popc(pw[n >> 5] & ((1u << (n & 31)) - 1u));
+            a.out[po] = (float)count * (1.0f / (float)n);
+        } else {
+            for (int w = 0; w < words; ++w) count += __popc(pw[w]);
+            a.out[po] = (float)count * a.inv_samples;
+        }
     }
 }
 

@@ -103,6 +103,19 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
 int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
                                const float *d_offsets_xy, float *d_out, void *hip_stream);
 
+/* Adaptive sampling: pixel-wise render where output pixel i uses only the
+ * first d_budget[i] of the n_samples offsets (u16 per output pixel, clamped to
+ * [1, n_samples]) and is (covered samples) * (1 / d_budget[i]): for every
+ * pixel exactly what fg_render_pixelwise_device gives with n_samples =
+ * d_budget[i] and the first d_budget[i] offsets (Derived::offsets_input of a
+ * smaller N is a prefix of a larger N's, src/rng.rs:9-24).  Each tile does
+ * the work of the smallest level (16, 32, 64, ... samples, then n_samples)
+ * that covers its largest budget.  No reference counterpart; needs the HBM
+ * splat formulation (n_samples <= 2048). */
+int fg_render_pixelwise_adaptive_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                                        const float *d_offsets_input_xy, const uint16_t *d_budget, float *d_out,
+                                        void *hip_stream);
+
 /* Waits for `hip_stream` and returns the device-side status of the work
  * enqueued by the device-resident entry points since the previous call. */
 int fg_stream_status(fg_ctx *ctx, void *hip_stream);
@@ -197,6 +210,12 @@ int fg_mlp_forward_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_
  * d_grad and the sum of squared errors to *d_loss. */
 int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, const float *d_target,
                        int64_t n, int64_t n_total, float *d_grad, float *d_loss, void *hip_stream);
+/* Per-pixel sample budgets for fg_render_pixelwise_adaptive_device:
+ * budget = clamp(ceil(c * s), n_min, n_max) with s = exp(d_pred_log_s[i]), or,
+ * with d_pred_log_s NULL, s = sqrt(u (1 - u)) from the features (the
+ * allocator without the model). */
+int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n, float c,
+                         int n_min, int n_max, uint16_t *d_budget, void *hip_stream);
 /* One Adam step (step counts from 1) on d_params with moments d_m, d_v. */
 int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
                        float beta1, float beta2, float eps, int step, void *hip_stream);
