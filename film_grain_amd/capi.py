@@ -37,6 +37,8 @@ EXPORTED = [
 ]
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
+LOSS_SQUARED = 0
+LOSS_NLL = 1
 FG_ECAPACITY = 7
 ERROR_NAMES[FG_ECAPACITY] = "FG_ECAPACITY"
 
@@ -134,7 +136,7 @@ def lib() -> C.CDLL:
     L.fg_mlp_features_device.argtypes = [vp, vp, u32, u32, u32, u32, C.c_float, C.c_float, C.c_float, C.c_float,
                                          C.c_int, vp, vp]
     L.fg_mlp_forward_device.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
-    L.fg_mlp_grad_device.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, vp, vp, vp]
+    L.fg_mlp_grad_device.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp]
     L.fg_mlp_adam_device.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
     L.fg_mlp_budget_device.argtypes = [vp, vp, vp, C.c_int64, C.c_float, C.c_int, C.c_int, vp, vp]
     L.fg_render_pixelwise_adaptive_device.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, vp, vp]
@@ -250,12 +252,16 @@ class Context:
                                                   C.c_void_p(_stream_handle(stream))))
         return out
 
-    def mlp_grad(self, params, feats, target, grad, loss, n_total=None, stream=None):
+    def mlp_grad(self, params, feats, target, grad, loss, n_total=None, loss_kind="squared", stream=None):
+        """Adds the gradient of the mean loss ("squared": (y - target)^2,
+        "nll": Gaussian negative log-likelihood of the residual target under
+        the predicted log std y) to grad and the loss sum to loss."""
         n = feats.shape[0]
+        kind = {"squared": LOSS_SQUARED, "nll": LOSS_NLL}[loss_kind]
         self._check(self._L.fg_mlp_grad_device(self.handle, C.c_void_p(params.data_ptr()),
                                                C.c_void_p(feats.data_ptr()), C.c_void_p(target.data_ptr()), n,
-                                               n_total or n, C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()),
-                                               C.c_void_p(_stream_handle(stream))))
+                                               n_total or n, kind, C.c_void_p(grad.data_ptr()),
+                                               C.c_void_p(loss.data_ptr()), C.c_void_p(_stream_handle(stream))))
 
     def mlp_adam(self, params, grad, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
         self._check(self._L.fg_mlp_adam_device(self.handle, C.c_void_p(params.data_ptr()), C.c_void_p(grad.data_ptr()),

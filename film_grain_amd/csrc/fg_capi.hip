@@ -261,13 +261,13 @@ int sample_buckets(uint32_t n)
     return q;
 }
 
-// HBM splat kernel: the configured tile, else the largest tile of the ladder
-// whose bitset takes at most 16 KB (measured on MI355X, 4K RGB, N = 256:
-// occupancy decides); the LDS is exactly what the tile and the sample table
-// need.  Returns false when no tile fits the workgroup LDS limit.  Queue
-// entries hold a tile column in 7 bits and a row in 8 bits, so tiles are at
-// most 128 x 256.
-bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
+// HBM splat kernel: the configured tile, else a tile of the ladder chosen
+// below (with min_tiles > 0, stepped down until the plane has that many
+// tiles); the LDS is exactly what the tile and the sample table need.
+// Returns false when no tile fits the workgroup LDS limit.  Queue entries
+// hold a tile column in 7 bits and a row in 8 bits, so tiles are at most
+// 128 x 256.
+bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, long long min_tiles)
 {
     static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
                                     {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
@@ -280,15 +280,44 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds)
         *tw = ctx->cfg.pix_tile_w;
         *th = ctx->cfg.pix_tile_h;
     } else {
+        // The largest ladder tile of at most max(128, 512 zoom^2) output
+        // pixels (about 512 input pixels: larger tiles leave too few
+        // workgroups to balance) whose bitset takes at most 16 KB; else a
+        // tile of at least 128 pixels with up to 32 KB (N = 2048); else any
+        // that fits (measured on MI355X, 1080p zoom-1 planes, N = 16..2048).
+        const double cap_px = std::max(128.0, 512.0 * (double)d->zoom * (double)d->zoom);
         *tw = 0;
-        for (const auto &t : ladder) {
-            if ((size_t)t[0] * t[1] * words * 4 <= 16384 && fits(t[0], t[1])) {
-                *tw = t[0];
-                *th = t[1];
-                break;
+        for (int pass = 0; pass < 3 && *tw == 0; ++pass) {
+            for (const auto &t : ladder) {
+                const int px = t[0] * t[1];
+                const size_t bits = (size_t)px * words * 4;
+                const bool ok = pass == 0   ? (double)px <= cap_px && bits <= 16384
+                                : pass == 1 ? (double)px <= cap_px && px >= 128 && bits <= 32768
+                                            : true;
+                if (ok && fits(t[0], t[1])) {
+                    *tw = t[0];
+                    *th = t[1];
+                    break;
+                }
             }
         }
         if (*tw == 0) return false;
+        // A plane alone on the device: smaller tiles until it has at least
+        // four per CU, so the heaviest tile does not set the frame time
+        // (480x272, N = 64: 255 tiles of 32x16 take 2.6 ms).
+        if (min_tiles > 0) {
+            size_t i = 0;
+            while (i < sizeof ladder / sizeof ladder[0] && (ladder[i][0] != *tw || ladder[i][1] != *th)) ++i;
+            auto tiles = [&](int w, int h) {
+                return (long long)((d->out_w + w - 1) / w) * (long long)((d->out_h + h - 1) / h);
+            };
+            while (i + 1 < sizeof ladder / sizeof ladder[0] && tiles(*tw, *th) < min_tiles &&
+                   ladder[i + 1][0] * ladder[i + 1][1] >= 32 && fits(ladder[i + 1][0], ladder[i + 1][1])) {
+                ++i;
+                *tw = ladder[i][0];
+                *th = ladder[i][1];
+            }
+        }
     }
     *lds = (int)fg::splat_hbm_lds(*tw, *th, (int)d->n_samples, q);
     return true;
@@ -606,7 +635,7 @@ int do_pixelwise_budget(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lamb
         if (!plane_geometry(d, range, &geom)) variant = 2;
     }
     int tw, th, lds;
-    if (variant == 3 && !hbm_config(ctx, d, &tw, &th, &lds)) variant = 2;
+    if (variant == 3 && !hbm_config(ctx, d, &tw, &th, &lds, 4LL * ctx->num_cus)) variant = 2;
     if (budget && variant != 3)
         return set_error(ctx, FG_EINVAL, "adaptive sampling: the plane's cell grid does not fit the HBM splat path");
     if (variant != 3) pixel_config(ctx, d, variant, &tw, &th, &lds);
@@ -969,7 +998,7 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
         if (!plane_geometry(d, r[p], &bp.geo))
             return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": cell grid exceeds 65535 cells per side");
         int tw, th;
-        if (!hbm_config(ctx, d, &tw, &th, &lds_p[p]))
+        if (!hbm_config(ctx, d, &tw, &th, &lds_p[p], 0))
             return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": no tile fits the LDS");
         if ((rc = pixel_args(ctx, d, lams[p], offs[p], outs[p], tw, th, lds_p[p], b->ranges + p, b->status,
                              &bp.a)) != FG_OK)
@@ -1216,7 +1245,7 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
         out->pix_variant = pixel_variant(ctx, desc);
         out->pix_block = splat_hbm_block(ctx);
         if (out->pix_variant == 3 &&
-            !hbm_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes))
+            !hbm_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes, 4LL * ctx->num_cus))
             out->pix_variant = 2;
         if (out->pix_variant != 3)
             pixel_config(ctx, desc, out->pix_variant, &out->pix_tile_w, &out->pix_tile_h,
@@ -1340,14 +1369,15 @@ int fg_mlp_forward_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_
 }
 
 int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, const float *d_target,
-                       int64_t n, int64_t n_total, float *d_grad, float *d_loss, void *hip_stream)
+                       int64_t n, int64_t n_total, int loss_kind, float *d_grad, float *d_loss, void *hip_stream)
 {
     if (!ctx) return FG_EINVAL;
-    if (!d_params || !d_features || !d_target || !d_grad || !d_loss || n < 0 || n_total <= 0)
+    if (!d_params || !d_features || !d_target || !d_grad || !d_loss || n < 0 || n_total <= 0 ||
+        (loss_kind != FG_LOSS_SQUARED && loss_kind != FG_LOSS_NLL))
         return set_error(ctx, FG_EINVAL, "invalid MLP arguments");
     if (n == 0) return FG_OK;
-    const hipError_t e = fg::launch_mlp_grad(d_params, d_features, d_target, n, 1.0f / (float)n_total, d_grad, d_loss,
-                                             reinterpret_cast<hipStream_t>(hip_stream));
+    const hipError_t e = fg::launch_mlp_grad(d_params, d_features, d_target, n, 1.0f / (float)n_total, loss_kind,
+                                             d_grad, d_loss, reinterpret_cast<hipStream_t>(hip_stream));
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "MLP gradient");
 }
 

@@ -21,8 +21,10 @@ constexpr int kMlpParams = kMlpB3 + 1;
 hipError_t launch_features(const float *u, int in_w, int in_h, int out_w, int out_h, float zoom, float radius,
                            float sd, float sigma, int channel, uint16_t *X, hipStream_t s);
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s);
+// loss_kind 0: squared error to target; 1: Gaussian negative log-likelihood
+// of the residual target under the predicted log std.
 hipError_t launch_mlp_grad(const float *P, const uint16_t *X, const float *target, long long n, float inv_total,
-                           float *grad, float *loss, hipStream_t s);
+                           int loss_kind, float *grad, float *loss, hipStream_t s);
 hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, int nmin, int nmax,
                          uint16_t *budget, hipStream_t s);
 hipError_t launch_mlp_adam(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2, float eps,

@@ -16,8 +16,12 @@
 //   layer 2: Z2 = H1 W2 + b2     8 x v_mfma_f32_16x16x32_bf16, H1 = relu(Z1) in BF16
 //                                 through LDS (accumulator rows -> operand rows)
 //   layer 3: y = H2 . w3 + b3     FP32 dot products (a 64-vector), H2 = relu(Z2)
-// Backward of the mean squared error to a target t (fused, per tile):
-//   g = 2 (y - t) / n, dZ2 = g w3 * [Z2 > 0], dZ1 = (dZ2 W2^T) * [Z1 > 0]
+// Backward of the loss per row (fused, per tile), either the squared error to
+// a target t, L = (y - t)^2, g = dL/dy = 2 (y - t), or the Gaussian negative
+// log-likelihood of a residual t under the predicted log std y,
+// L = y + t^2 e^(-2y) / 2, g = 1 - t^2 e^(-2y) (the exponent clamped to
+// [-80, 80]); then with g scaled by 1 / n:
+//   dZ2 = g w3 * [Z2 > 0], dZ1 = (dZ2 W2^T) * [Z1 > 0]
 //   dW3 = H2^T g, dW2 = H1^T dZ2, dW1 = X^T dZ1 (MFMA over the 16 rows), biases
 //   by sums.  Products feeding a matrix instruction use BF16 operands (dZ2,
 //   dZ1, H1, X); their accumulations are FP32.  The CPU reference in the
@@ -147,11 +151,13 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *P, const 
     }
 }
 
-// Fused forward + backward of the mean squared error over rows [0, n) with
+// Fused forward + backward of the loss (loss_kind 0: squared error, 1:
+// Gaussian negative log-likelihood of the residual) over rows [0, n) with
 // loss scale 1 / n_total; gradients are added to grad (kMlpParams f32) and
-// the loss sum to *loss.
+// the unscaled loss sum to *loss.
 __global__ __launch_bounds__(256) void mlp_grad_kernel(const float *P, const uint16_t *X, const float *target,
-                                                       long long n, float inv_total, float *grad, float *loss)
+                                                       long long n, float inv_total, int loss_kind, float *grad,
+                                                       float *loss)
 {
     __shared__ MlpLds s;
     __shared__ float s_grad[kMlpParams];
@@ -182,9 +188,20 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const float *P, const uin
         float gr[4];
         for (int i = 0; i < 4; ++i) {
             const long long row = t * 16 + 4 * g + i;
-            const float d = row < n ? y[i] - target[row] : 0.0f;
-            gr[i] = 2.0f * d * inv_total;
-            if (r == 0) lsum += d * d;
+            if (row >= n) {
+                gr[i] = 0.0f;
+                continue;
+            }
+            const float t = target[row];
+            if (loss_kind == 1) {
+                const float e = expf(fminf(fmaxf(-2.0f * y[i], -80.0f), 80.0f)) * t * t;
+                gr[i] = (1.0f - e) * inv_total;
+                if (r == 0) lsum += y[i] + 0.5f * e;
+            } else {
+                const float d = y[i] - t;
+                gr[i] = 2.0f * d * inv_total;
+                if (r == 0) lsum += d * d;
+            }
         }
         // layer 3 and dZ2 (accumulator layout), dW2 = H1^T dZ2 on MFMA
         v4f dz2[4];
@@ -347,12 +364,12 @@ hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, fl
 }
 
 hipError_t launch_mlp_grad(const float *P, const uint16_t *X, const float *target, long long n, float inv_total,
-                           float *grad, float *loss, hipStream_t s)
+                           int loss_kind, float *grad, float *loss, hipStream_t s)
 {
     const long long tiles = (n + 15) / 16;
     const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 1024);
-    hipLaunchKernelGGL(mlp_grad_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, target, n, inv_total, grad,
-                       loss);
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, target, n, inv_total,
+                       loss_kind, grad, loss);
     return hipGetLastError();
 }
 

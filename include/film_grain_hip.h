@@ -206,10 +206,14 @@ int fg_mlp_features_device(fg_ctx *ctx, const float *d_u, uint32_t in_w, uint32_
                            int channel, uint16_t *d_features, void *hip_stream);
 int fg_mlp_forward_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, int64_t n, float *d_out,
                           void *hip_stream);
-/* Adds the gradient of (1/n_total) sum (y - target)^2 over these n rows to
- * d_grad and the sum of squared errors to *d_loss. */
+/* Adds the gradient of (1/n_total) sum L over these n rows to d_grad and the
+ * sum of L to *d_loss.  FG_LOSS_SQUARED: L = (y - target)^2.  FG_LOSS_NLL:
+ * the Gaussian negative log-likelihood of the residual target under the
+ * predicted log std y, L = y + target^2 exp(-2y) / 2. */
+#define FG_LOSS_SQUARED 0
+#define FG_LOSS_NLL 1
 int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_features, const float *d_target,
-                       int64_t n, int64_t n_total, float *d_grad, float *d_loss, void *hip_stream);
+                       int64_t n, int64_t n_total, int loss_kind, float *d_grad, float *d_loss, void *hip_stream);
 /* Per-pixel sample budgets for fg_render_pixelwise_adaptive_device:
  * budget = clamp(ceil(c * s), n_min, n_max) with s = exp(d_pred_log_s[i]), or,
  * with d_pred_log_s NULL, s = sqrt(u (1 - u)) from the features (the

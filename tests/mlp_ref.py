@@ -46,12 +46,24 @@ def forward(P, X):
     return z1, h1, z2, h2, y
 
 
-def grad(P, X, t, n_total=None):
-    """Gradient of (1/n_total) sum (y - t)^2, packed like the parameters."""
+def loss_terms(y, t, loss="squared"):
+    """Per-row loss and its derivative in y: squared error (y - t)^2, or the
+    Gaussian negative log-likelihood y + t^2 exp(-2y) / 2 of a residual t
+    under the predicted log std y (exponent clamped to [-80, 80])."""
+    if loss == "nll":
+        e = np.exp(np.clip(-2.0 * y, -80.0, 80.0)) * t * t
+        return y + 0.5 * e, 1.0 - e
+    return (y - t) ** 2, 2.0 * (y - t)
+
+
+def grad(P, X, t, n_total=None, loss="squared"):
+    """Gradient of (1/n_total) sum L(y, t), packed like the parameters, and
+    the loss sum."""
     W1, b1, W2, b2, w3, b3 = unpack(P)
     n_total = n_total or X.shape[0]
     z1, h1, z2, h2, y = forward(P, X)
-    g = 2.0 * (y - t) / n_total
+    lv, dl = loss_terms(y, np.asarray(t, np.float64), loss)
+    g = dl / n_total
     dw3 = h2.T @ g
     db3 = g.sum()
     dz2 = np.where(z2 > 0, g[:, None] * bf16(w3).astype(np.float64)[None, :], 0.0)
@@ -69,7 +81,7 @@ def grad(P, X, t, n_total=None):
     out[P_B2:P_W3] = db2
     out[P_W3:P_B3] = dw3
     out[P_B3] = db3
-    return out, float(((y - t) ** 2).sum())
+    return out, float(lv.sum())
 
 
 def features(u, out_w, out_h, zoom, radius, sd, sigma, channel):

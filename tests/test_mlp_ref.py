@@ -4,6 +4,7 @@ differences of its loss, and its forward pass equals a plain restatement of
 the network.  (A dropped term, wrong sign or transposed operand in the
 reference's backward pass fails the first check.)"""
 import numpy as np
+import pytest
 
 from tests import mlp_ref
 
@@ -12,20 +13,27 @@ def _no_rounding(monkeypatch):
     monkeypatch.setattr(mlp_ref, "bf16", lambda x: np.asarray(x, dtype=np.float64))
 
 
-def test_gradient_equals_finite_differences(monkeypatch):
+@pytest.mark.parametrize("kind", ["squared", "nll"])
+def test_gradient_equals_finite_differences(monkeypatch, kind):
     _no_rounding(monkeypatch)
     rng = np.random.default_rng(0)
     n = 40
-    P = rng.normal(0, 0.4, mlp_ref.N_PARAMS)
+    # smaller weights for the likelihood keep y = O(1), where exp(-2y) and
+    # its finite differences stay well conditioned
+    P = rng.normal(0, 0.4 if kind == "squared" else 0.15, mlp_ref.N_PARAMS)
     P[mlp_ref.P_W1 + 12 * 64:mlp_ref.P_B1] = 0.0
     X = np.zeros((n, 16))
     X[:, :12] = rng.uniform(-1, 1, (n, 12))
     t = rng.normal(0, 1, n)
 
     def loss(p):
-        return ((mlp_ref.forward(p, X)[-1] - t) ** 2).sum() / n
+        y = mlp_ref.forward(p, X)[-1]
+        if kind == "nll":  # written out here, not through the reference
+            return (y + 0.5 * t * t * np.exp(-2.0 * y)).sum() / n
+        return ((y - t) ** 2).sum() / n
 
-    g, _ = mlp_ref.grad(P, X, t)
+    g, lsum = mlp_ref.grad(P, X, t, loss=kind)
+    assert abs(lsum / n - loss(P)) <= 1e-9 * abs(loss(P)) + 1e-12
     idx = np.concatenate([rng.choice(np.arange(mlp_ref.P_W1, mlp_ref.P_W1 + 12 * 64), 40, replace=False),
                           rng.choice(np.arange(mlp_ref.P_B1, mlp_ref.N_PARAMS), 80, replace=False),
                           [mlp_ref.P_B3]])
