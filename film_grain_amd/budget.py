@@ -326,11 +326,37 @@ def frame_buffers(planes: list[DevicePlane]) -> FrameBuffers:
     return FrameBuffers(X, pred, budget, out)
 
 
+def scale_for_mean_budget(s, mean_budget: float, n_min: int, n_max: int, probes: int = 64, rounds: int = 3,
+                          max_rows: int = 1 << 18) -> float:
+    """The largest c whose budgets clamp(ceil(c s), n_min, n_max) average at
+    most mean_budget (strided subsample of at most max_rows rows): the
+    allocation of a fixed total of samples."""
+    import torch
+    s = s.reshape(-1).float()
+    ss = s[::max(1, s.numel() // max_rows)]
+    pos = ss[ss > 0]
+    lo = 1e-3
+    hi = 2.0 * n_max / max(float(pos.min()), 1e-6) if pos.numel() else 1.0
+    for _ in range(rounds):
+        cs = torch.exp(torch.linspace(math.log(lo), math.log(hi), probes, device=ss.device))
+        m = torch.clamp(torch.ceil(cs[:, None] * ss[None, :]), n_min, n_max).mean(dim=1)
+        ok = (m <= mean_budget).nonzero()
+        if ok.numel() == 0:
+            return float(cs[0])
+        k = int(ok[-1])
+        if k == probes - 1:
+            return float(cs[-1])
+        lo, hi = float(cs[k]), float(cs[k + 1])
+    return lo
+
+
 def render_adaptive_frame(ctx, model: ErrorModel | None, planes: list[DevicePlane], bufs: FrameBuffers,
-                          target_mse: float, kind: str = "learned", n_min: int = C5["n_min"],
-                          n_max: int = C5["n_max"], refs=None):
-    """One frame with per-pixel budgets: features, predictions, one scale c
-    for the whole frame, budgets, adaptive renders.  Returns (outputs, c)."""
+                          target_mse: float | None = None, kind: str = "learned", n_min: int = C5["n_min"],
+                          n_max: int = C5["n_max"], refs=None, mean_budget: float | None = None):
+    """Planes rendered with per-pixel budgets under one scale c for all of
+    them: features, predictions, c (meeting target_mse, or averaging
+    mean_budget samples per pixel), budgets, adaptive renders, each launched
+    for the largest budget of its plane.  Returns (outputs, c)."""
     import torch
     s_all = []
     for i, dp in enumerate(planes):
@@ -340,7 +366,11 @@ def render_adaptive_frame(ctx, model: ErrorModel | None, planes: list[DevicePlan
             s_all.append(torch.exp(bufs.pred[i]) * SIGMA64_TO_S)
         else:
             s_all.append(allocator_s(kind, model, ctx, bufs.X[i], None if refs is None else refs[i]))
-    c = choose_scale(torch.cat(s_all), target_mse, n_min, n_max)
+    s_cat = torch.cat(s_all)
+    if mean_budget is not None:
+        c = scale_for_mean_budget(s_cat, mean_budget, n_min, n_max)
+    else:
+        c = choose_scale(s_cat, target_mse, n_min, n_max)
     for i, dp in enumerate(planes):
         n = dp.out_w * dp.out_h
         if kind == "learned":
@@ -350,7 +380,11 @@ def render_adaptive_frame(ctx, model: ErrorModel | None, planes: list[DevicePlan
         else:
             ctx.mlp_budget(torch.log(torch.clamp(s_all[i], min=1e-30)).contiguous(), None, n, c, n_min, n_max,
                            out=bufs.budget[i])
-        ctx.render_adaptive_device(_desc_n(dp.desc, n_max), dp.lam, dp.offs[:n_max], bufs.budget[i], bufs.out[i])
+    # the largest budget of a plane sizes its launch (tile shape, sample levels)
+    his = torch.stack([(b.to(torch.int32) & 0xFFFF).max() for b in bufs.budget]).tolist()
+    for i, dp in enumerate(planes):
+        n_hi = max(1, min(n_max, int(his[i])))
+        ctx.render_adaptive_device(_desc_n(dp.desc, n_hi), dp.lam, dp.offs[:n_hi], bufs.budget[i], bufs.out[i])
     return bufs.out, c
 
 
@@ -439,6 +473,37 @@ def evaluate(ctx, model: ErrorModel | None, specs_per_frame, targets_n=(256, 102
     return results
 
 
+def compare_at_equal_budget(ctx, model: ErrorModel | None, specs, mean_budgets=(64, 256),
+                            kinds=("learned", "binomial", "oracle"), n_min: int = C5["n_min"],
+                            n_max: int = C5["n_max"], log=None):
+    """All planes of `specs` rendered together under one sample total (the
+    mean budget per pixel): per allocator, the RMSE against N_REF and the
+    mean budget reached.  A better error model puts the same samples where
+    they reduce the error most."""
+    planes = [upload(s) for s in specs]
+    refs = [render_reference(ctx, dp) for dp in planes]
+    bufs = frame_buffers(planes)
+    import torch
+    rows = []
+    for mb in mean_budgets:
+        for kind in kinds:
+            if kind == "learned" and model is None:
+                continue
+            outs, c = render_adaptive_frame(ctx, model, planes, bufs, kind=kind, n_min=n_min, n_max=n_max,
+                                            refs=refs, mean_budget=mb)
+            torch.cuda.synchronize()
+            mean_n = float(np.mean([float((b.to(torch.int32) & 0xFFFF).float().mean()) for b in bufs.budget]))
+            row = dict(mean_budget=mb, kind=kind, rmse=_frame_rmse(outs, refs), mean_n=mean_n)
+            rows.append(row)
+            if log:
+                log(f"[budget] equal budget {mb}: {kind} rmse {row['rmse'] * 255:.3f}/255 mean n {mean_n:.1f}")
+    fixed = []
+    for mb in mean_budgets:
+        outs = [render_fixed(ctx, dp, int(mb)) for dp in planes]
+        fixed.append(dict(mean_budget=mb, kind="fixed", rmse=_frame_rmse(outs, refs), mean_n=float(mb)))
+    return rows + fixed
+
+
 def s_ratio(refs) -> float:
     """E[s^2] / E[s]^2 of the per-sample stds of N_REF renders: the largest
     factor by which any per-pixel allocation can cut the mean sample count
@@ -472,6 +537,8 @@ def main(argv=None):
     ap.add_argument("--model", default=str(MODEL_PATH))
     ap.add_argument("--save", action="store_true")
     ap.add_argument("--json", default=None, help="write the evaluation here")
+    ap.add_argument("--equal-budget", action="store_true",
+                    help="compare allocators at equal total samples over all frames together")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     ctx = capi.Context(0)
@@ -501,7 +568,10 @@ def main(argv=None):
     else:
         model = ErrorModel.load(Path(args.model)) if Path(args.model).exists() else None
         frames = [make(50_000 + i) for i in range(args.frames)]
-        res = evaluate(ctx, model, frames, log=log)
+        if args.equal_budget:
+            res = compare_at_equal_budget(ctx, model, [p for f in frames for p in f], log=log)
+        else:
+            res = evaluate(ctx, model, frames, log=log)
         out = json.dumps(res, indent=1)
         if args.json:
             Path(args.json).write_text(out + "\n")

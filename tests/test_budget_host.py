@@ -81,3 +81,14 @@ def test_init_params_layout():
     W1 = P[:1024].reshape(16, 64)
     assert np.all(W1[12:] == 0) and np.all(P[1024:1088] == 0)
     assert P[5312] == pytest.approx(math.log(0.5 / budget.SIGMA64_TO_S))
+
+
+def test_scale_for_mean_budget_spends_at_most_the_total():
+    rng = np.random.default_rng(3)
+    s = torch.from_numpy(np.sqrt(rng.uniform(0, 0.25, 20000)).astype(np.float32))
+    for mb in (20.0, 64.0, 300.0, 900.0):
+        c = budget.scale_for_mean_budget(s, mb, 16, 1024)
+        n = torch.clamp(torch.ceil(c * s.double()), 16, 1024)
+        assert float(n.mean()) <= mb * (1 + 1e-6)
+        n2 = torch.clamp(torch.ceil(c * 1.02 * s.double()), 16, 1024)
+        assert float(n2.mean()) > mb * 0.99  # close to the largest such scale
