@@ -17,7 +17,14 @@ Also reported on the same JSON line:
                 on a bounded random sample of 16x16 output tiles of the same
                 frame, on the host's cores; those tiles also spot-check parity.
 
-    python bench.py [--steps K] [--warmup W] [--workload pixel-c2|grain-c3|pixel-c1]
+  adaptive      (default workload) the learned per-pixel sample budget on a
+                1920x1080 RGB frame: adaptive ms/frame at the RMSE fixed
+                N = 1024 reaches against the N = 4096 render, fixed-N ms/frame
+                at the same RMSE, allocators at equal sample totals, and the
+                error model's BF16 MFMA throughput (after the timed region).
+
+    python bench.py [--steps K] [--warmup W]
+                    [--workload pixel-c2|grain-c3|pixel-c1|batch-c4|adaptive-c5]
 """
 from __future__ import annotations
 
@@ -180,12 +187,98 @@ def batch_bench(args):
     ctx.close()
 
 
+# Algorithmic FLOPs per row of the error model (12 inputs, 64, 64, 1): the
+# forward pass, and forward + backward (dW3, dZ2, dW2, dH1, dW1).
+MLP_FWD_FLOPS = 2 * (12 * 64 + 64 * 64 + 64)
+MLP_GRAD_FLOPS = MLP_FWD_FLOPS + 2 * 64 + 2 * 64 + 2 * (2 * 64 * 64) + 2 * 12 * 64
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def adaptive_summary(steps: int = 3, warmup: int = 1):
+    """The learned sample budget on one 1920x1080 RGB frame (lognormal
+    0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024): ms/frame of the adaptive
+    render (features, error model, budgets and renders, all timed) at the
+    RMSE that fixed N = 1024 reaches against the N = 4096 render, the fixed-N
+    ms/frame at the adaptive frame's RMSE, the allocators at equal sample
+    totals, and the error model's MFMA throughput."""
+    import torch
+    from film_grain_amd import budget, capi
+    ctx = capi.Context(0)
+    model = budget.ErrorModel.load()
+    planes = [budget.upload(sp) for sp in budget.image_planes(50_000)]
+    refs = [budget.render_reference(ctx, dp) for dp in planes]
+    curve = budget.fixed_curve(ctx, planes, refs, ns=(256, 512, 1024, 2048), reps=steps)
+    bufs = budget.frame_buffers(planes)
+    target = next(e for n, ms, e in curve if n == budget.C5["n_fixed"])
+    for _ in range(warmup):
+        budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2, "learned")
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        outs, c = budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2, "learned")
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(ts))
+    rmse = budget._frame_rmse(outs, refs)
+    mean_n = float(np.mean([float((b.to(torch.int32) & 0xFFFF).float().mean()) for b in bufs.budget]))
+    fixed_ms = budget.ms_at_rmse(curve, rmse)
+    fixed_1024 = next(ms_ for n, ms_, e in curve if n == budget.C5["n_fixed"])
+    ratio = budget.s_ratio(refs)
+    log(f"[bench] adaptive {ms:.2f} ms/frame rmse {rmse * 255:.3f}/255 mean n {mean_n:.1f}; fixed at that rmse "
+        f"{fixed_ms} ms; fixed N=1024 {fixed_1024:.2f} ms; E[s^2]/E[s]^2 {ratio:.4f}")
+    eq = budget.compare_at_equal_budget(ctx, model, budget.image_planes(50_000), mean_budgets=(256,))
+    # the error model's kernels on the frame's features
+    X = torch.cat(bufs.X)
+    rows = X.shape[0]
+    pred = torch.empty(rows, dtype=torch.float32, device=X.device)
+    grad = torch.zeros(capi.MLP_PARAMS, dtype=torch.float32, device=X.device)
+    loss = torch.zeros(1, dtype=torch.float32, device=X.device)
+    e = torch.zeros(rows, dtype=torch.float32, device=X.device)
+    ctx.mlp_forward(model.P, X, out=pred)
+    ctx.mlp_grad(model.P, X, e, grad, loss, loss_kind="nll")
+    st = torch.cuda.current_stream()
+    a, b, g0, g1 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+    a.record(st)
+    for _ in range(5):
+        ctx.mlp_forward(model.P, X, out=pred)
+    b.record(st)
+    g0.record(st)
+    for _ in range(5):
+        ctx.mlp_grad(model.P, X, e, grad, loss, loss_kind="nll")
+    g1.record(st)
+    torch.cuda.synchronize()
+    fwd_ms, grad_ms = a.elapsed_time(b) / 5, g0.elapsed_time(g1) / 5
+    fwd_tf = rows * MLP_FWD_FLOPS / (fwd_ms * 1e-3) / 1e12
+    grad_tf = rows * MLP_GRAD_FLOPS / (grad_ms * 1e-3) / 1e12
+    log(f"[bench] error model on {rows} rows: forward {fwd_ms:.3f} ms ({fwd_tf:.1f} TFLOP/s), "
+        f"forward+backward {grad_ms:.3f} ms ({grad_tf:.1f} TFLOP/s)")
+    ctx.close()
+    return {
+        "metric": "ms/frame adaptive vs fixed-N at equal RMSE",
+        "workload": "1920x1080 RGB, lognormal 0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024, seed 50000; "
+                    "RMSE against the N=4096 render; target = the RMSE of fixed N=1024",
+        "adaptive_ms_per_frame": round(ms, 3), "rmse_255": round(rmse * 255, 4), "mean_budget": round(mean_n, 1),
+        "fixed_ms_at_equal_rmse": None if fixed_ms is None else round(fixed_ms, 3),
+        "fixed_over_adaptive": None if fixed_ms is None else round(fixed_ms / ms, 4),
+        "fixed_n1024_ms_per_frame": round(fixed_1024, 3),
+        "fixed_curve": [dict(n=n, ms=round(t, 3), rmse_255=round(e_ * 255, 4)) for n, t, e_ in curve],
+        "allocation_bound_E_s2_over_E_s_sq": round(ratio, 4),
+        "equal_samples_256_rmse_255": {r["kind"]: round(r["rmse"] * 255, 4) for r in eq},
+        "mlp": {"rows": rows, "forward_ms": round(fwd_ms, 4), "forward_tflops": round(fwd_tf, 2),
+                "grad_ms": round(grad_ms, 4), "grad_tflops": round(grad_tf, 2), "peak_bf16_tflops": PEAK_BF16_TFLOPS,
+                "flops_per_row": {"forward": MLP_FWD_FLOPS, "forward_backward": MLP_GRAD_FLOPS}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4"])
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4", "adaptive-c5"])
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the learned-budget comparison appended to the default line")
     ap.add_argument("--batch-images", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -204,6 +297,15 @@ def main():
         sys.exit(2)
     if args.workload == "batch-c4":
         batch_bench(args)
+        return
+    if args.workload == "adaptive-c5":
+        ad = adaptive_summary(args.steps, args.warmup)
+        line = {"metric": ad["metric"], "value": ad["adaptive_ms_per_frame"], "unit": "ms/frame", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": ad["adaptive_ms_per_frame"],
+                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": ad["workload"]}, "roofline": None,
+                "cpu_baseline": None, "adaptive": ad}
+        print(json.dumps(line), flush=True)
         return
 
     import torch
@@ -345,8 +447,10 @@ def main():
     }
     if parity is not None:
         line["parity_spot_check"] = parity
-    print(json.dumps(line), flush=True)
     ctx.close()
+    if args.workload == "pixel-c2" and not args.no_adaptive:
+        line["adaptive"] = adaptive_summary()
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -280,19 +280,22 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
         *tw = ctx->cfg.pix_tile_w;
         *th = ctx->cfg.pix_tile_h;
     } else {
-        // The largest ladder tile of at most max(128, 512 zoom^2) output
+        // The largest ladder tile of 128 to max(128, 512 zoom^2) output
         // pixels (about 512 input pixels: larger tiles leave too few
-        // workgroups to balance) whose bitset takes at most 16 KB; else a
-        // tile of at least 128 pixels with up to 32 KB (N = 2048); else any
-        // that fits (measured on MI355X, 1080p zoom-1 planes, N = 16..2048).
+        // workgroups to balance) whose bitset takes at most 16 KB; else one
+        // with up to 32 KB (N = 2048: 16x8 beats 8x8 by 19%); else a smaller
+        // tile with at most 16 KB; else any that fits (measured on MI355X,
+        // 1080p zoom-1 planes, N = 16..2048).
         const double cap_px = std::max(128.0, 512.0 * (double)d->zoom * (double)d->zoom);
         *tw = 0;
-        for (int pass = 0; pass < 3 && *tw == 0; ++pass) {
+        for (int pass = 0; pass < 4 && *tw == 0; ++pass) {
             for (const auto &t : ladder) {
                 const int px = t[0] * t[1];
                 const size_t bits = (size_t)px * words * 4;
-                const bool ok = pass == 0   ? (double)px <= cap_px && bits <= 16384
-                                : pass == 1 ? (double)px <= cap_px && px >= 128 && bits <= 32768
+                const bool in_cap = (double)px <= cap_px;
+                const bool ok = pass == 0   ? in_cap && px >= 128 && bits <= 16384
+                                : pass == 1 ? in_cap && px >= 128 && bits <= 32768
+                                : pass == 2 ? in_cap && bits <= 16384
                                             : true;
                 if (ok && fits(t[0], t[1])) {
                     *tw = t[0];
