@@ -709,7 +709,12 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
         if (it >= n_items) break;
         const int2 pt = items[it];
         const int pl = __builtin_amdgcn_readfirstlane(pt.x), tile = __builtin_amdgcn_readfirstlane(pt.y);
-        const BatchPlane &p = planes[pl];
+        // No thread writes the plane table while the kernel runs: reached
+        // through the constant address space, its fields at this uniform
+        // address become scalar loads into SGPRs, as kernel arguments are
+        // (generic loads kept them in VGPRs: 91 VGPRs against 68).
+        const auto *cp = (const __attribute__((address_space(4))) BatchPlane *)planes + pl;
+        const BatchPlane &p = *(const BatchPlane *)cp;
         splat_tile<BLOCK>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
     }
 }
