@@ -190,12 +190,16 @@ void fg_batch_destroy(fg_batch *batch);
 
 /* ---- Per-pixel Monte-Carlo error model -----------------------------------
  * No counterpart in the reference.  A small MLP (16 inputs, 12 used -> 64 ->
- * 64 -> 1, ReLU) predicts log s, s = sqrt(p (1 - p)) the per-sample standard
- * deviation of an output pixel's coverage indicator, from local statistics
- * of the normalised intensity plane.  BF16 weights and operands, FP32
- * accumulation on the matrix cores; forward, fused backward of the mean
- * squared error, and an Adam step, all on device memory and asynchronous on
- * `hip_stream`.  Parameters: FG_MLP_PARAMS floats (layout in fg_mlp.h).
+ * 64 -> 1, ReLU) predicts a log standard deviation of an output pixel's
+ * Monte-Carlo error from local statistics of the normalised intensity plane
+ * (trained on the Gaussian likelihood of N = 64 minus N = 4096 residuals, it
+ * is the log std of the N = 64 error, sqrt(p (1 - p) (1/64 - 1/4096)) for a
+ * pixel of covered fraction p).  BF16 weights and operands, FP32
+ * accumulation on the matrix cores; forward, fused backward of a squared-
+ * error or likelihood loss, and an Adam step, all on device memory and
+ * asynchronous on `hip_stream`.  Parameters: FG_MLP_PARAMS floats: W1
+ * [16][64] input-major (rows 12..15 zero), b1 [64], W2 [64][64], b2 [64],
+ * w3 [64], b3.
  * Features: FG_MLP_FEATURES BF16 values per output pixel:
  *   u, 3x3 mean, 3x3 std, gradient magnitude, u(1-u), radius/0.3,
  *   radius_stddev/0.15, sigma/2, zoom/4, channel one-hot (3), 4 zeros. */
