@@ -35,7 +35,11 @@ DEVICE = [
 ]
 
 HIP_SOURCES = ["fg_pixelwise.hip", "fg_pixelwise_hbm.hip", "fg_grainwise.hip", "fg_batch.hip", "fg_mlp.hip",
-               "fg_capi.hip"]
+               "fg_frame.hip", "fg_capi.hip"]
+# The exact-logf correction table (fg_logf.h) is generated from this
+# machine's C library at every build and checked over its whole domain.
+LOGF_GEN = "gen_logf_table.hip"
+LOGF_TABLE = "fg_logf_table.h"
 HOST_SOURCES = sorted(p.name for p in (CSRC / "host").glob("*.cpp")) if (CSRC / "host").exists() else []
 
 
@@ -52,14 +56,27 @@ def _run(cmd: list[str]) -> None:
             sys.stderr.write("\n".join(lines) + "\n")
 
 
+def logf_table() -> Path:
+    """Builds and runs the logf table generator; returns the header."""
+    gen = BUILD / "gen_logf_table"
+    _run([HIPCC, *COMMON, *DEVICE, "-fno-builtin", str(CSRC / LOGF_GEN), "-o", str(gen), "-lm", "-lpthread"])
+    header = BUILD / LOGF_TABLE
+    r = subprocess.run([str(gen), str(header)], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("logf table generation failed")
+    return header
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
+    logf_table()
     jobs = []
     objs = []
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
         objs.append(obj)
-        jobs.append([HIPCC, *COMMON, *DEVICE, "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([HIPCC, *COMMON, *DEVICE, f"-I{BUILD}", "-c", str(CSRC / src), "-o", str(obj)])
     for src in HOST_SOURCES:
         obj = BUILD / ("host_" + src + ".o")
         objs.append(obj)

@@ -33,8 +33,11 @@ EXPORTED = [
     "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings",
     "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy",
     "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
-    "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_version",
+    "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
+    "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_version",
 ]
+COLOR_LUMA, COLOR_RGB = 0, 1
+ALGO_PIXEL, ALGO_GRAIN = 0, 1
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
 LOSS_SQUARED = 0
@@ -50,6 +53,11 @@ class PlaneDesc(C.Structure):
         ("zoom", C.c_float), ("delta", C.c_float), ("rm", C.c_float),
         ("radius_mean", C.c_float), ("log_mu", C.c_float), ("log_sigma", C.c_float),
     ]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("plane", PlaneDesc), ("color_mode", C.c_uint32), ("algo", C.c_uint32),
+                ("inv_e_pi_r2", C.c_float), ("reserved", C.c_uint32)]
 
 
 class LaunchConfig(C.Structure):
@@ -140,8 +148,16 @@ def lib() -> C.CDLL:
     L.fg_mlp_adam_device.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
     L.fg_mlp_budget_device.argtypes = [vp, vp, vp, C.c_int64, C.c_float, C.c_int, C.c_int, vp, vp]
     L.fg_render_pixelwise_adaptive_device.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, vp, vp]
+    L.fg_render_frame_device.argtypes = [vp, C.POINTER(FrameDesc), vp, vp, vp, vp, vp]
+    L.fg_render_frame.argtypes = [vp, C.POINTER(FrameDesc), fp, fp, C.POINTER(C.c_uint8), fp, CANCEL_FN, vp]
+    L.fg_lambda_device.argtypes = [vp, vp, u32, u32, C.c_float, vp, vp, vp]
+    L.fg_color_split_device.argtypes = [vp, C.c_int, vp, u32, u32, vp, vp, vp, vp]
+    L.fg_color_merge_device.argtypes = [vp, C.c_int, vp, vp, vp, u32, u32, u32, u32, vp, vp]
+    L.fg_log_f32_device.argtypes = [vp, vp, C.c_int64, vp, vp]
     for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
-                 "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device"):
+                 "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device",
+                 "fg_render_frame", "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device",
+                 "fg_log_f32_device"):
         getattr(L, name).restype = C.c_int
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
@@ -217,6 +233,56 @@ class Context:
             self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
             C.c_void_p(budget.data_ptr()), C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
         self._check(rc)
+
+    # ---- frames: colour split, lambda pre-pass, render, merge ----
+    def render_frame_device(self, fd: FrameDesc, rgb, offsets, rgb8, planes=None, stream=None):
+        """rgb: (in_h, in_w, 3) f32 tensor; rgb8: (out_h, out_w, 3) uint8
+        tensor; planes: optional (k, out_h, out_w) f32 tensor."""
+        self._check(self._L.fg_render_frame_device(
+            self.handle, C.byref(fd), C.c_void_p(rgb.data_ptr()), C.c_void_p(offsets.data_ptr()),
+            C.c_void_p(rgb8.data_ptr()), C.c_void_p(planes.data_ptr() if planes is not None else 0),
+            C.c_void_p(_stream_handle(stream))))
+
+    def render_frame(self, fd: FrameDesc, rgb: np.ndarray, offsets: np.ndarray, with_planes: bool = False,
+                     cancel=None):
+        """Host-memory frame: returns (rgb8 (out_h, out_w, 3), planes or None)."""
+        d = fd.plane
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        offs = np.ascontiguousarray(offsets, dtype=np.float32)
+        out8 = np.empty((d.out_h, d.out_w, 3), dtype=np.uint8)
+        nr = 3 if fd.color_mode == COLOR_RGB else 1
+        planes = np.empty((nr, d.out_h, d.out_w), dtype=np.float32) if with_planes else None
+        fp = C.POINTER(C.c_float)
+        cb = CANCEL_FN(lambda _u: 1 if cancel() else 0) if cancel else CANCEL_FN(0)
+        self._check(self._L.fg_render_frame(
+            self.handle, C.byref(fd), rgb.ctypes.data_as(fp), offs.ctypes.data_as(fp),
+            out8.ctypes.data_as(C.POINTER(C.c_uint8)), planes.ctypes.data_as(fp) if planes is not None else None,
+            cb, None))
+        return out8, planes
+
+    def lambda_device(self, plane, inv_e_pi_r2, lam, norm=None, stream=None):
+        h, w = plane.shape
+        self._check(self._L.fg_lambda_device(
+            self.handle, C.c_void_p(plane.data_ptr()), w, h, inv_e_pi_r2,
+            C.c_void_p(norm.data_ptr() if norm is not None else 0), C.c_void_p(lam.data_ptr()),
+            C.c_void_p(_stream_handle(stream))))
+
+    def color_split_device(self, mode: int, rgb, p0, p1, p2, stream=None):
+        h, w = rgb.shape[:2]
+        self._check(self._L.fg_color_split_device(
+            self.handle, mode, C.c_void_p(rgb.data_ptr()), w, h, C.c_void_p(p0.data_ptr()),
+            C.c_void_p(p1.data_ptr()), C.c_void_p(p2.data_ptr()), C.c_void_p(_stream_handle(stream))))
+
+    def color_merge_device(self, mode: int, p0, p1, p2, rgb8, stream=None):
+        h, w = p0.shape
+        ch, cw = p1.shape
+        self._check(self._L.fg_color_merge_device(
+            self.handle, mode, C.c_void_p(p0.data_ptr()), C.c_void_p(p1.data_ptr()), C.c_void_p(p2.data_ptr()),
+            w, h, cw, ch, C.c_void_p(rgb8.data_ptr()), C.c_void_p(_stream_handle(stream))))
+
+    def log_f32_device(self, x, out, stream=None):
+        self._check(self._L.fg_log_f32_device(self.handle, C.c_void_p(x.data_ptr()), x.numel(),
+                                              C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
 
     def stream_status(self, stream=None):
         self._check(self._L.fg_stream_status(self.handle, C.c_void_p(_stream_handle(stream))))

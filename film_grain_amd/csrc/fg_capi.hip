@@ -16,6 +16,7 @@
 #include <string>
 
 #include "../../include/film_grain_hip.h"
+#include "fg_frame.h"
 #include "fg_kernels.h"
 #include "fg_mlp.h"
 #include "fg_pixelwise_hbm.h"
@@ -53,6 +54,8 @@ struct fg_ctx {
     DevBuf pix_state; // grain-wise: per input pixel grain count and stream state (36 B)
     DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
     DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
+    DevBuf frame;             // frames: split planes, lambda, max word, rendered planes
+    DevBuf frame_io;          // frames from host memory: RGB in, offsets, 8-bit RGB out, planes out
     std::vector<int> level_n;
     std::vector<fg::SampleTable> level_tables;
 
@@ -109,6 +112,8 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->tiles);
     free_buf(ctx->tile_temp);
     free_buf(ctx->levels);
+    free_buf(ctx->frame);
+    free_buf(ctx->frame_io);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -870,6 +875,65 @@ int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     return rc;
 }
 
+int validate_frame(fg_ctx *ctx, const fg_frame_desc *fd)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!fd) return set_error(ctx, FG_EINVAL, "null frame description");
+    const int rc = validate(ctx, &fd->plane);
+    if (rc != FG_OK) return rc;
+    if (fd->color_mode > 1) return set_error(ctx, FG_EINVAL, "unknown colour mode");
+    if (fd->algo > 1) return set_error(ctx, FG_EINVAL, "unknown algorithm");
+    if (!std::isfinite(fd->inv_e_pi_r2) || !(fd->inv_e_pi_r2 >= 0.0f))
+        return set_error(ctx, FG_EINVAL, "inv_e_pi_r2 must be finite and >= 0");
+    return FG_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// One frame on the device (src/lib.rs:141-165 with src/color.rs:47-114):
+// split, then per rendered plane the lambda pre-pass and the integrator, then
+// the merge.  The frame scratch holds the three split planes, one lambda
+// plane (reused: the planes run in stream order), the max word and, without
+// d_planes, the rendered planes.
+int frame_device(fg_ctx *ctx, const fg_frame_desc *fd, const float *d_rgb, const float *d_offs, uint8_t *d_rgb8,
+                 float *d_planes, hipStream_t s, fg_cancel_fn cancel, void *user)
+{
+    const char *what = "Frame renderer";
+    const fg_plane_desc &d = fd->plane;
+    const size_t nin = (size_t)d.in_w * d.in_h, nout = (size_t)d.out_w * d.out_h;
+    const int nr = fd->color_mode == FG_COLOR_RGB ? 3 : 1;
+    const size_t in_b = align256(nin * 4), out_b = align256(nout * 4);
+    const size_t bytes = 4 * in_b + 256 + (d_planes ? 0 : (size_t)nr * out_b);
+    // an earlier call's kernels may still read the scratch (on another stream)
+    int rc = scratch_acquire(ctx, what);
+    if (rc != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->frame, bytes, what)) != FG_OK) return rc;
+    unsigned char *base = static_cast<unsigned char *>(ctx->frame.p);
+    float *sp[3] = {reinterpret_cast<float *>(base), reinterpret_cast<float *>(base + in_b),
+                    reinterpret_cast<float *>(base + 2 * in_b)};
+    float *lam = reinterpret_cast<float *>(base + 3 * in_b);
+    unsigned *word = reinterpret_cast<unsigned *>(base + 4 * in_b);
+    float *outp = d_planes ? d_planes : reinterpret_cast<float *>(base + 4 * in_b + 256);
+    hipError_t e = fg::launch_split((int)fd->color_mode, d_rgb, (long long)nin, sp[0], sp[1], sp[2], s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    for (int i = 0; i < nr; ++i) {
+        if (cancel && cancel(user)) return set_error(ctx, FG_ECANCELLED, "cancelled");
+        e = fg::launch_lambda(sp[i], (long long)nin, fd->inv_e_pi_r2, word, nullptr, lam, s);
+        if (e != hipSuccess) return hip_error(ctx, e, what);
+        rc = fd->algo == FG_ALGO_PIXEL ? do_pixelwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false)
+                                       : do_grainwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false);
+        if (rc != FG_OK) return rc;
+    }
+    if (fd->color_mode == FG_COLOR_RGB)
+        e = fg::launch_merge(FG_COLOR_RGB, outp, outp + nout, outp + 2 * nout, (int)d.out_w, (int)d.out_h,
+                             (int)d.out_w, (int)d.out_h, d_rgb8, s);
+    else
+        e = fg::launch_merge(FG_COLOR_LUMA, outp, sp[1], sp[2], (int)d.out_w, (int)d.out_h, (int)d.in_w,
+                             (int)d.in_h, d_rgb8, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    return scratch_release(ctx, s, what);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1382,6 +1446,120 @@ int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_fea
     const hipError_t e = fg::launch_mlp_grad(d_params, d_features, d_target, n, 1.0f / (float)n_total, loss_kind,
                                              d_grad, d_loss, reinterpret_cast<hipStream_t>(hip_stream));
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "MLP gradient");
+}
+
+int fg_render_frame_device(fg_ctx *ctx, const fg_frame_desc *fd, const float *d_rgb, const float *d_offsets_xy,
+                           uint8_t *d_rgb8, float *d_planes, void *hip_stream)
+{
+    int rc = validate_frame(ctx, fd);
+    if (rc != FG_OK) return rc;
+    if (!d_rgb || !d_offsets_xy || !d_rgb8) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Frame renderer");
+    return frame_device(ctx, fd, d_rgb, d_offsets_xy, d_rgb8, d_planes, reinterpret_cast<hipStream_t>(hip_stream),
+                        nullptr, nullptr);
+}
+
+int fg_render_frame(fg_ctx *ctx, const fg_frame_desc *fd, const float *rgb, const float *offsets_xy, uint8_t *rgb8,
+                    float *planes, fg_cancel_fn cancel, void *cancel_user)
+{
+    const char *what = "Frame renderer";
+    int rc = validate_frame(ctx, fd);
+    if (rc != FG_OK) return rc;
+    if (!rgb || !offsets_xy || !rgb8) return set_error(ctx, FG_EINVAL, "null buffer");
+    const fg_plane_desc &d = fd->plane;
+    if (!finite_pairs(offsets_xy, d.n_samples)) return set_error(ctx, FG_EINVAL, "offsets must be finite");
+    if (cancel && cancel(cancel_user)) return set_error(ctx, FG_ECANCELLED, "cancelled");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    const size_t nin = (size_t)d.in_w * d.in_h, nout = (size_t)d.out_w * d.out_h;
+    const int nr = fd->color_mode == FG_COLOR_RGB ? 3 : 1;
+    const size_t rgb_b = align256(nin * 12), offs_b = align256((size_t)d.n_samples * 8), out_b = align256(nout * 3);
+    const size_t planes_b = planes ? align256((size_t)nr * nout * 4) : 0;
+    if ((rc = ensure_bytes(ctx, ctx->frame_io, rgb_b + offs_b + out_b + planes_b, what)) != FG_OK) return rc;
+    unsigned char *base = static_cast<unsigned char *>(ctx->frame_io.p);
+    float *d_rgb = reinterpret_cast<float *>(base);
+    float *d_offs = reinterpret_cast<float *>(base + rgb_b);
+    uint8_t *d_rgb8 = base + rgb_b + offs_b;
+    float *d_planes = planes ? reinterpret_cast<float *>(base + rgb_b + offs_b + out_b) : nullptr;
+    hipStream_t s = ctx->stream;
+    e = hipMemcpyAsync(d_rgb, rgb, nin * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_offs, offsets_xy, (size_t)d.n_samples * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    rc = frame_device(ctx, fd, d_rgb, d_offs, d_rgb8, d_planes, s, cancel, cancel_user);
+    if (rc != FG_OK) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    e = hipMemcpyAsync(rgb8, d_rgb8, nout * 3, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && planes)
+        e = hipMemcpyAsync(planes, d_planes, (size_t)nr * nout * 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    return read_status(ctx, s, what);
+}
+
+int fg_lambda_device(fg_ctx *ctx, const float *d_plane, uint32_t w, uint32_t h, float inv_e_pi_r2, float *d_norm,
+                     float *d_lambda, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_plane || !d_lambda) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (!std::isfinite(inv_e_pi_r2)) return set_error(ctx, FG_EINVAL, "inv_e_pi_r2 must be finite");
+    if ((size_t)w * h == 0) return FG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Lambda pre-pass");
+    int rc = scratch_acquire(ctx, "Lambda pre-pass");
+    if (rc != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->frame, 256, "Lambda pre-pass")) != FG_OK) return rc;
+    // the max word: the first word of the frame scratch
+    const hipStream_t s = reinterpret_cast<hipStream_t>(hip_stream);
+    e = fg::launch_lambda(d_plane, (long long)w * h, inv_e_pi_r2, static_cast<unsigned *>(ctx->frame.p), d_norm,
+                          d_lambda, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Lambda pre-pass");
+    return scratch_release(ctx, s, "Lambda pre-pass");
+}
+
+int fg_color_split_device(fg_ctx *ctx, int color_mode, const float *d_rgb, uint32_t w, uint32_t h, float *d_p0,
+                          float *d_p1, float *d_p2, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (color_mode != FG_COLOR_LUMA && color_mode != FG_COLOR_RGB)
+        return set_error(ctx, FG_EINVAL, "unknown colour mode");
+    if (!d_rgb || !d_p0 || !d_p1 || !d_p2) return set_error(ctx, FG_EINVAL, "null buffer");
+    if ((size_t)w * h == 0) return FG_OK;
+    const hipError_t e = fg::launch_split(color_mode, d_rgb, (long long)w * h, d_p0, d_p1, d_p2,
+                                          reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Colour split");
+}
+
+int fg_color_merge_device(fg_ctx *ctx, int color_mode, const float *d_p0, const float *d_p1, const float *d_p2,
+                          uint32_t w, uint32_t h, uint32_t cw, uint32_t ch, uint8_t *d_rgb8, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (color_mode != FG_COLOR_LUMA && color_mode != FG_COLOR_RGB)
+        return set_error(ctx, FG_EINVAL, "unknown colour mode");
+    if (!d_p0 || !d_p1 || !d_p2 || !d_rgb8) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (w > (1u << 24) || h > (1u << 24) || cw > (1u << 24) || ch > (1u << 24))
+        return set_error(ctx, FG_EINVAL, "image dimensions exceed 2^24");
+    if (color_mode == FG_COLOR_LUMA && (cw == 0 || ch == 0) && (size_t)w * h > 0)
+        return set_error(ctx, FG_EINVAL, "chroma planes are empty");
+    if ((size_t)w * h == 0) return FG_OK;
+    const hipError_t e = fg::launch_merge(color_mode, d_p0, d_p1, d_p2, (int)w, (int)h,
+                                          color_mode == FG_COLOR_RGB ? (int)w : (int)cw,
+                                          color_mode == FG_COLOR_RGB ? (int)h : (int)ch, d_rgb8,
+                                          reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Colour merge");
+}
+
+int fg_log_f32_device(fg_ctx *ctx, const float *d_in, int64_t n, float *d_out, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return set_error(ctx, FG_EINVAL, "invalid log arguments");
+    if (n == 0) return FG_OK;
+    const hipError_t e = fg::launch_log_f32(d_in, n, d_out, reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Log");
 }
 
 int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n, float c,

@@ -225,6 +225,11 @@ RenderStats dry_run_with_input_image_cancelable(const InputImage &input, const P
 std::pair<RgbImage, RenderStats> render_workspace(Workspace ws, const Params &p,
                                                   const CancelCheck *cancel,
                                                   std::vector<Plane> *planes_out);
+// The same result from an interleaved RGB f32 image with the whole frame on
+// the device (colour split, lambda pre-pass, integrator, merge: one upload,
+// one download, fg_render_frame); the ROI crop stays on the host.
+std::pair<RgbImage, RenderStats> render_rgb_frame(const float *rgb, size_t w, size_t h, const Params &p,
+                                                  const CancelCheck *cancel, std::vector<Plane> *planes_out);
 
 // image I/O (PNG 8-bit grey/RGB/RGBA via zlib, binary PGM/PPM, PFM)
 std::vector<float> load_image_rgb_f32(const std::string &path, size_t *w, size_t *h);

@@ -1,4 +1,5 @@
 // fg_host_capi.cpp -- C ABI of the host library (include/film_grain.h).
+#include <cmath>
 #include <cstring>
 
 #include "../../../include/film_grain.h"
@@ -196,6 +197,63 @@ int fgh_render_file(const fgh_cli_args *args, fgh_derived *stats, char *err, siz
         auto r = render_workspace(in.to_workspace(), p, nullptr, nullptr);
         save_image(r.first, p.output_path, resolve_format(p));
         fill(stats, p, derive_common(p, in.dimensions()), r.second.algorithm);
+        return FGH_OK;
+    });
+}
+
+int fgh_render_rgb_frame(const fgh_cli_args *args, const float *rgb, uint32_t w, uint32_t h, uint8_t *out_rgb8,
+                         size_t out_rgb8_cap, float *out_planes, size_t out_planes_cap, fgh_derived *stats,
+                         fgh_cancel_fn cancel, void *cancel_user, char *err, size_t errlen)
+{
+    return guarded(err, errlen, [&]() {
+        const Params p = to_params(args);
+        CancelCheck cc;
+        if (cancel) cc = [cancel, cancel_user]() { return cancel(cancel_user) != 0; };
+        std::vector<Plane> planes;
+        auto r = render_rgb_frame(rgb, w, h, p, cancel ? &cc : nullptr, out_planes ? &planes : nullptr);
+        const std::pair<size_t, size_t> dims =
+            p.roi ? std::pair<size_t, size_t>(p.roi->x1 - p.roi->x0, p.roi->y1 - p.roi->y0)
+                  : std::pair<size_t, size_t>(w, h);
+        fill(stats, p, derive_common(p, dims), r.second.algorithm);
+        if (out_rgb8) {
+            if (out_rgb8_cap < r.first.data.size()) {
+                put_err(err, errlen, "output buffer too small");
+                return FGH_ECAPACITY;
+            }
+            std::memcpy(out_rgb8, r.first.data.data(), r.first.data.size());
+        }
+        if (out_planes) {
+            size_t need = 0;
+            for (const auto &pl : planes) need += pl.data.size();
+            if (out_planes_cap < need) {
+                put_err(err, errlen, "plane buffer too small");
+                return FGH_ECAPACITY;
+            }
+            size_t off = 0;
+            for (const auto &pl : planes) {
+                std::memcpy(out_planes + off, pl.data.data(), pl.data.size() * sizeof(float));
+                off += pl.data.size();
+            }
+        }
+        return FGH_OK;
+    });
+}
+
+void fgh_logf_array(const float *x, size_t n, float *out)
+{
+    for (size_t i = 0; i < n; ++i) out[i] = std::log(x[i]);
+}
+
+int fgh_color_split(const float *rgb, uint32_t w, uint32_t h, uint32_t color_mode, float *p0, float *p1, float *p2)
+{
+    return guarded(nullptr, 0, [&]() {
+        const Workspace ws = Workspace::from_rgb_f32(rgb, w, h, color_mode == 1 ? ColorMode::Rgb : ColorMode::Luma);
+        const size_t n = (size_t)w * h;
+        const Plane *src[3] = {ws.kind == Workspace::Luma ? &ws.y : &ws.planes[0],
+                               ws.kind == Workspace::Luma ? &ws.cb : &ws.planes[1],
+                               ws.kind == Workspace::Luma ? &ws.cr : &ws.planes[2]};
+        float *dst[3] = {p0, p1, p2};
+        for (int i = 0; i < 3; ++i) std::memcpy(dst[i], src[i]->data.data(), n * sizeof(float));
         return FGH_OK;
     });
 }

@@ -1,6 +1,7 @@
 // fg_lib.cpp -- library entry points (src/lib.rs:57-186): derive -> choose ->
 // per-plane normalise / lambda / integrate -> recombine, with the integrator
 // arm running on the HIP backend through include/film_grain_hip.h.
+#include <algorithm>
 #include <memory>
 #include <mutex>
 
@@ -131,6 +132,66 @@ std::pair<RgbImage, RenderStats> render_workspace(Workspace ws, const Params &p,
     const RenderStats stats = make_stats(p, d, algo);
     check_cancel(cancel);
     return {ws.into_rgb_image(), stats};
+}
+
+std::pair<RgbImage, RenderStats> render_rgb_frame(const float *rgb, size_t w, size_t h, const Params &p,
+                                                  const CancelCheck *cancel, std::vector<Plane> *planes_out)
+{
+    check_cancel(cancel);
+    std::vector<float> crop;  // src/color.rs:215-231
+    if (p.roi) {
+        const Roi &r = *p.roi;
+        if (r.x1 > w || r.y1 > h) throw RenderError(RenderError::Message, "ROI exceeds image bounds");
+        const size_t cw = r.x1 - r.x0, ch = r.y1 - r.y0;
+        if (cw == 0 || ch == 0) throw RenderError(RenderError::Message, "ROI width and height must be positive");
+        crop.resize(cw * ch * 3);
+        for (size_t yy = 0; yy < ch; ++yy)
+            std::copy(rgb + ((r.y0 + yy) * w + r.x0) * 3, rgb + ((r.y0 + yy) * w + r.x0 + cw) * 3,
+                      crop.begin() + yy * cw * 3);
+        rgb = crop.data();
+        w = cw;
+        h = ch;
+    }
+    Derived d;
+    try {
+        d = derive_common(p, {w, h});
+    } catch (const RenderError &) {
+        throw;
+    } catch (const std::exception &e) {
+        throw RenderError(RenderError::Message, e.what());
+    }
+    const Algo algo = choose_algorithm(p, d);
+    if (p.device == Device::Cpu)
+        throw RenderError(RenderError::Unsupported,
+                          "unsupported: the CPU integrators are not part of this backend; use --device hip");
+    fg_ctx *ctx = hip_context();
+    fg_frame_desc fd{};
+    fd.plane = plane_desc(p, d);
+    fd.color_mode = p.color_mode == ColorMode::Rgb ? FG_COLOR_RGB : FG_COLOR_LUMA;
+    fd.algo = algo == Algo::Pixel ? FG_ALGO_PIXEL : FG_ALGO_GRAIN;
+    fd.inv_e_pi_r2 = d.inv_e_pi_r2;
+    const size_t nout = d.output_width * d.output_height, nr = p.color_mode == ColorMode::Rgb ? 3 : 1;
+    RgbImage img;
+    img.width = (uint32_t)d.output_width;
+    img.height = (uint32_t)d.output_height;
+    img.data.resize(nout * 3);
+    std::vector<float> planes(planes_out ? nr * nout : 0);
+    CancelThunk thunk{cancel};
+    const std::vector<float> &offs = algo == Algo::Pixel ? d.offsets_input : d.offsets;
+    const int rc = fg_render_frame(ctx, &fd, rgb, offs.data(), img.data.data(), planes_out ? planes.data() : nullptr,
+                                   cancel ? &CancelThunk::call : nullptr, &thunk);
+    if (rc == FG_ECANCELLED) throw RenderError(RenderError::Cancelled, "cancelled");
+    if (rc != FG_OK) {
+        const std::string msg = std::string("gpu error: ") + fg_last_error(ctx);
+        if (rc == FG_ENOMEM || rc == FG_EHIP) invalidate_context(ctx);
+        throw RenderError(RenderError::Gpu, msg);
+    }
+    if (planes_out)
+        for (size_t i = 0; i < nr; ++i)
+            planes_out->emplace_back(d.output_width, d.output_height,
+                                     std::vector<float>(planes.begin() + i * nout, planes.begin() + (i + 1) * nout));
+    check_cancel(cancel);
+    return {std::move(img), make_stats(p, d, algo)};
 }
 
 std::pair<RgbImage, RenderStats> render_to_image(const Params &p)

@@ -120,6 +120,12 @@ def _setup():
                                  C.c_size_t, fp, C.c_size_t, C.POINTER(DerivedC), C.CFUNCTYPE(C.c_int, C.c_void_p),
                                  C.c_void_p, C.c_char_p, C.c_size_t]
     L.fgh_render_rgb.restype = C.c_int
+    L.fgh_render_rgb_frame.argtypes = L.fgh_render_rgb.argtypes
+    L.fgh_render_rgb_frame.restype = C.c_int
+    L.fgh_logf_array.argtypes = [fp, C.c_size_t, fp]
+    L.fgh_logf_array.restype = None
+    L.fgh_color_split.argtypes = [fp, C.c_uint32, C.c_uint32, C.c_uint32, fp, fp, fp]
+    L.fgh_color_split.restype = C.c_int
     L.fgh_render_file.argtypes = [C.POINTER(CliArgsC), C.POINTER(DerivedC), C.c_char_p, C.c_size_t]
     L.fgh_render_file.restype = C.c_int
     L.fgh_dry_run.argtypes = [C.POINTER(CliArgsC), C.POINTER(DerivedC), C.c_char_p, C.c_size_t]
@@ -209,12 +215,35 @@ def lambda_plane(normalized: np.ndarray, inv_e_pi_r2: float) -> np.ndarray:
     return out
 
 
+def logf_array(x: np.ndarray) -> np.ndarray:
+    """The C library's logf, elementwise (the reference's f32::ln)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    _setup().fgh_logf_array(_f32(x), x.size, _f32(out))
+    return out
+
+
+def color_split(rgb: np.ndarray, color_mode: str) -> np.ndarray:
+    """Workspace planes of an (h, w, 3) f32 image: (3, h, w) as Y, Cb, Cr
+    (luma) or R, G, B (rgb)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    out = np.empty((3, h, w), dtype=np.float32)
+    rc = _setup().fgh_color_split(_f32(rgb), w, h, 1 if color_mode == "rgb" else 0, _f32(out[0]), _f32(out[1]),
+                                  _f32(out[2]))
+    if rc != FGH_OK:
+        raise RenderError(rc, "colour split failed")
+    return out
+
+
 def inverse_normal_cdf(p: float) -> float:
     return _setup().fgh_inverse_normal_cdf(p)
 
 
-def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None):
-    """Library-level render of an (h, w, 3) f32 image; returns (rgb8, planes, stats)."""
+def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None, frame_on_device: bool = False):
+    """Library-level render of an (h, w, 3) f32 image; returns (rgb8, planes,
+    stats).  frame_on_device: the colour split, lambda pre-pass and merge run
+    on the device too (one upload, one download); the result is the same."""
     L = _setup()
     rgb = np.ascontiguousarray(rgb, dtype=np.float32)
     h, w = rgb.shape[:2]
@@ -230,8 +259,9 @@ def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None):
     err = C.create_string_buffer(512)
     cb_type = C.CFUNCTYPE(C.c_int, C.c_void_p)
     cb = cb_type(lambda _u: 1 if cancel() else 0) if cancel else cb_type(0)
-    rc = L.fgh_render_rgb(C.byref(args.to_c()), _f32(rgb), w, h, out8.ctypes.data_as(C.POINTER(C.c_uint8)),
-                          out8.size, _f32(planes), planes.size, C.byref(stats), cb, None, err, 512)
+    fn = L.fgh_render_rgb_frame if frame_on_device else L.fgh_render_rgb
+    rc = fn(C.byref(args.to_c()), _f32(rgb), w, h, out8.ctypes.data_as(C.POINTER(C.c_uint8)),
+            out8.size, _f32(planes), planes.size, C.byref(stats), cb, None, err, 512)
     if rc != FGH_OK:
         raise RenderError(rc, err.value.decode())
     return out8, planes, stats

@@ -81,6 +81,16 @@ int fgh_render_rgb(const fgh_cli_args *args, const float *rgb, uint32_t w, uint3
                    uint8_t *out_rgb8, size_t out_rgb8_cap, float *out_planes, size_t out_planes_cap,
                    fgh_derived *stats, fgh_cancel_fn cancel, void *cancel_user, char *err,
                    size_t errlen);
+/* The same as fgh_render_rgb, bit for bit, with the whole frame on the device
+ * (colour split, lambda pre-pass, integrator and merge behind one upload and
+ * one download: fg_render_frame); the ROI crop stays on the host. */
+int fgh_render_rgb_frame(const fgh_cli_args *args, const float *rgb, uint32_t w, uint32_t h, uint8_t *out_rgb8,
+                         size_t out_rgb8_cap, float *out_planes, size_t out_planes_cap, fgh_derived *stats,
+                         fgh_cancel_fn cancel, void *cancel_user, char *err, size_t errlen);
+/* out[i] = logf(x[i]) of the C library (the reference's f32::ln). */
+void fgh_logf_array(const float *x, size_t n, float *out);
+/* Workspace::from_rgb_f32's planes: luma mode Y, Cb, Cr; rgb mode R, G, B. */
+int fgh_color_split(const float *rgb, uint32_t w, uint32_t h, uint32_t color_mode, float *p0, float *p1, float *p2);
 /* File to file (~ render, src/lib.rs:57-71). */
 int fgh_render_file(const fgh_cli_args *args, fgh_derived *stats, char *err, size_t errlen);
 /* ~ dry_run (src/lib.rs:95-98). */

@@ -228,6 +228,47 @@ int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t 
 int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
                        float beta1, float beta2, float eps, int step, void *hip_stream);
 
+/* ---- Frames: colour split, lambda pre-pass, render, merge on the device ---
+ * The reference prepares each plane on the host (src/color.rs:158-213 split,
+ * src/model.rs:228-265 normalize_plane and lambda_plane), renders it, and
+ * merges and quantises on the host (src/color.rs:66-114, 237-239).  These
+ * entry points do the same on the device, bit for bit: the same f32
+ * operations in the same order, and the C library's logf (which the
+ * reference's f32::ln calls) reproduced exactly on [1e-6, 1]. */
+#define FG_COLOR_LUMA 0 /* ColorMode::Luma: render Y, carry BT.709 Cb/Cr (resized nearest) */
+#define FG_COLOR_RGB 1  /* ColorMode::Rgb: render R, G, B */
+#define FG_ALGO_PIXEL 0
+#define FG_ALGO_GRAIN 1
+typedef struct {
+    fg_plane_desc plane;  /* shared by the frame's planes; in_w/in_h = the frame (after any ROI crop) */
+    uint32_t color_mode;  /* FG_COLOR_LUMA or FG_COLOR_RGB */
+    uint32_t algo;        /* FG_ALGO_PIXEL or FG_ALGO_GRAIN (choose_algorithm's result) */
+    float inv_e_pi_r2;    /* Derived::inv_e_pi_r2 */
+    uint32_t reserved;    /* 0 */
+} fg_frame_desc;
+/* One frame: d_rgb is in_h x in_w interleaved RGB f32 (as image's to_rgb32f
+ * gives); d_offsets as for the plane entry points of fd->algo; d_rgb8 gets
+ * out_h x out_w interleaved 8-bit RGB (into_rgb_image); d_planes, if not
+ * NULL, gets the rendered planes (3 for RGB, 1 for luma), out_h x out_w f32
+ * each.  Asynchronous on hip_stream. */
+int fg_render_frame_device(fg_ctx *ctx, const fg_frame_desc *fd, const float *d_rgb, const float *d_offsets_xy,
+                           uint8_t *d_rgb8, float *d_planes, void *hip_stream);
+/* The same from and to host memory: one upload of the frame and the offsets,
+ * one download of the result; cancel is polled between planes.  Synchronous. */
+int fg_render_frame(fg_ctx *ctx, const fg_frame_desc *fd, const float *rgb, const float *offsets_xy, uint8_t *rgb8,
+                    float *planes, fg_cancel_fn cancel, void *cancel_user);
+/* The steps on their own (device memory, asynchronous). */
+int fg_lambda_device(fg_ctx *ctx, const float *d_plane, uint32_t w, uint32_t h, float inv_e_pi_r2, float *d_norm,
+                     float *d_lambda, void *hip_stream); /* normalize_plane + lambda_plane; d_norm may be NULL */
+int fg_color_split_device(fg_ctx *ctx, int color_mode, const float *d_rgb, uint32_t w, uint32_t h, float *d_p0,
+                          float *d_p1, float *d_p2, void *hip_stream);
+/* Luma: d_p0 = Y at w x h, d_p1/d_p2 = Cb/Cr at cw x ch; RGB: all w x h. */
+int fg_color_merge_device(fg_ctx *ctx, int color_mode, const float *d_p0, const float *d_p1, const float *d_p2,
+                          uint32_t w, uint32_t h, uint32_t cw, uint32_t ch, uint8_t *d_rgb8, void *hip_stream);
+/* d_out[i] = logf(d_in[i]) exactly as the C library computes it, for inputs in
+ * [1e-6, 1] (other inputs get a result within one unit in the last place). */
+int fg_log_f32_device(fg_ctx *ctx, const float *d_in, int64_t n, float *d_out, void *hip_stream);
+
 /* Library version string. */
 const char *fg_version(void);
 
