@@ -17,6 +17,9 @@ Also reported on the same JSON line:
                 on a bounded random sample of 16x16 output tiles of the same
                 frame, on the host's cores; those tiles also spot-check parity.
 
+  frame         (default workload) the frame from host memory to host memory
+                (PCIe included): host split / lambda / merge against the
+                frame on the device, with identical output.
   adaptive      (default workload) the learned per-pixel sample budget on a
                 1920x1080 RGB frame: adaptive ms/frame at the RMSE fixed
                 N = 1024 reaches against the N = 4096 render, fixed-N ms/frame
@@ -192,6 +195,35 @@ def batch_bench(args):
 MLP_FWD_FLOPS = 2 * (12 * 64 + 64 * 64 + 64)
 MLP_GRAD_FLOPS = MLP_FWD_FLOPS + 2 * 64 + 2 * 64 + 2 * (2 * 64 * 64) + 2 * 12 * 64
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def frame_summary(reps: int = 3):
+    """The whole C2 frame from host memory to host memory (1920x1080 RGB f32
+    in, 3840x2160 8-bit RGB out, PCIe included): the host path (split, lambda
+    and merge on the host, one host-memory integrator call per plane) against
+    the frame on the device (one upload, split, lambda, integrators, merge,
+    one download).  Outputs must be identical."""
+    from film_grain_amd import host
+    from film_grain_amd.workloads import make_planes
+    wl = WORKLOADS["pixel-c2"]
+    rgb = np.ascontiguousarray(np.transpose(make_planes("pixel-c2"), (1, 2, 0)))
+    args = host.CliArgs(radius=wl["radius"], radius_stddev=wl["sd"], radius_dist=wl["dist"], zoom=wl["zoom"],
+                        iters=wl["iters"], seed=wl["seed"], algo="pixel", sigma=0.8, color_mode="rgb")
+    out = {}
+    res = {}
+    for name, on_dev in (("host_path_ms", False), ("frame_on_device_ms", True)):
+        host.render_rgb(args, rgb, frame_on_device=on_dev, with_planes=False)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res[name], _, _ = host.render_rgb(args, rgb, frame_on_device=on_dev, with_planes=False)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out[name] = round(float(np.median(ts)), 3)
+    out["identical"] = bool(np.array_equal(res["host_path_ms"], res["frame_on_device_ms"]))
+    out["workload"] = "pixel-c2 frame: 1920x1080 RGB f32 in host memory -> 3840x2160 RGB8 in host memory"
+    log(f"[bench] frame from host memory: host path {out['host_path_ms']} ms, on the device "
+        f"{out['frame_on_device_ms']} ms, identical {out['identical']}")
+    return out
 
 
 def adaptive_summary(steps: int = 3, warmup: int = 1):
@@ -448,6 +480,8 @@ def main():
     if parity is not None:
         line["parity_spot_check"] = parity
     ctx.close()
+    if args.workload == "pixel-c2":
+        line["frame"] = frame_summary()
     if args.workload == "pixel-c2" and not args.no_adaptive:
         line["adaptive"] = adaptive_summary()
     print(json.dumps(line), flush=True)

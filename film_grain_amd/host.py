@@ -240,10 +240,12 @@ def inverse_normal_cdf(p: float) -> float:
     return _setup().fgh_inverse_normal_cdf(p)
 
 
-def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None, frame_on_device: bool = False):
+def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None, frame_on_device: bool = False,
+               with_planes: bool = True):
     """Library-level render of an (h, w, 3) f32 image; returns (rgb8, planes,
-    stats).  frame_on_device: the colour split, lambda pre-pass and merge run
-    on the device too (one upload, one download); the result is the same."""
+    stats) (planes None without with_planes).  frame_on_device: the colour
+    split, lambda pre-pass and merge run on the device too (one upload, one
+    download); the result is the same."""
     L = _setup()
     rgb = np.ascontiguousarray(rgb, dtype=np.float32)
     h, w = rgb.shape[:2]
@@ -254,14 +256,15 @@ def render_rgb(args: CliArgs, rgb: np.ndarray, cancel=None, frame_on_device: boo
         d, _, _ = derive(args, x1 - x0, y1 - y0)
     nplanes = 3 if args.color_mode == "rgb" else 1
     out8 = np.zeros((d.out_h, d.out_w, 3), dtype=np.uint8)
-    planes = np.zeros((nplanes, d.out_h, d.out_w), dtype=np.float32)
+    planes = np.zeros((nplanes, d.out_h, d.out_w), dtype=np.float32) if with_planes else None
     stats = DerivedC()
     err = C.create_string_buffer(512)
     cb_type = C.CFUNCTYPE(C.c_int, C.c_void_p)
     cb = cb_type(lambda _u: 1 if cancel() else 0) if cancel else cb_type(0)
     fn = L.fgh_render_rgb_frame if frame_on_device else L.fgh_render_rgb
     rc = fn(C.byref(args.to_c()), _f32(rgb), w, h, out8.ctypes.data_as(C.POINTER(C.c_uint8)),
-            out8.size, _f32(planes), planes.size, C.byref(stats), cb, None, err, 512)
+            out8.size, _f32(planes) if with_planes else None, planes.size if with_planes else 0, C.byref(stats),
+            cb, None, err, 512)
     if rc != FGH_OK:
         raise RenderError(rc, err.value.decode())
     return out8, planes, stats
