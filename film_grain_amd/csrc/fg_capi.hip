@@ -701,15 +701,22 @@ bool sorted_grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, 
         *tw = ctx->cfg.grain_tile_w;
         *th = ctx->cfg.grain_tile_h;
     } else {
-        *tw = 0;
-        for (const auto &t : ladder) {
-            if ((size_t)t[0] * t[1] * words * 4 <= 32768 && fits(t[0], t[1])) {
-                *tw = t[0];
-                *th = t[1];
-                break;
-            }
-        }
-        if (*tw == 0) return false;
+        size_t i = 0, n = sizeof ladder / sizeof ladder[0];
+        while (i < n && !((size_t)ladder[i][0] * ladder[i][1] * words * 4 <= 32768 && fits(ladder[i][0], ladder[i][1])))
+            ++i;
+        if (i == n) return false;
+        // smaller tiles until the plane has at least four per CU, so the
+        // heaviest tile does not set the plane's time (512x512, N = 16: 32
+        // tiles of 128x64 take 8 ms)
+        auto tiles = [&](size_t k) {
+            return (long long)((d->out_w + ladder[k][0] - 1) / ladder[k][0]) *
+                   (long long)((d->out_h + ladder[k][1] - 1) / ladder[k][1]);
+        };
+        while (i + 1 < n && tiles(i) < 4LL * ctx->num_cus && ladder[i + 1][0] * ladder[i + 1][1] >= 64 &&
+               fits(ladder[i + 1][0], ladder[i + 1][1]))
+            ++i;
+        *tw = ladder[i][0];
+        *th = ladder[i][1];
     }
     *lds = (int)fg::grainwise_sorted_lds(*tw, *th, (int)d->n_samples, q);
     return true;

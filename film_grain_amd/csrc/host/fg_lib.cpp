@@ -194,10 +194,13 @@ std::pair<RgbImage, RenderStats> render_rgb_frame(const float *rgb, size_t w, si
     return {std::move(img), make_stats(p, d, algo)};
 }
 
+// File input renders the whole frame on the device (the same image as the
+// per-plane route through render_workspace, one upload and one download).
 std::pair<RgbImage, RenderStats> render_to_image(const Params &p)
 {
-    const InputImage in = InputImage::from_path(p.input_path, p.color_mode, p.roi);
-    return render_workspace(in.to_workspace(), p, nullptr, nullptr);
+    size_t w = 0, h = 0;
+    const std::vector<float> rgb = load_image_rgb_f32(p.input_path, &w, &h);
+    return render_rgb_frame(rgb.data(), w, h, p, nullptr, nullptr);
 }
 
 RenderStats render(const Params &p)
