@@ -34,7 +34,8 @@ EXPORTED = [
     "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy",
     "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
     "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
-    "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_version",
+    "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
+    "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
 ALGO_PIXEL, ALGO_GRAIN = 0, 1
@@ -154,6 +155,8 @@ def lib() -> C.CDLL:
     L.fg_color_split_device.argtypes = [vp, C.c_int, vp, u32, u32, vp, vp, vp, vp]
     L.fg_color_merge_device.argtypes = [vp, C.c_int, vp, vp, vp, u32, u32, u32, u32, vp, vp]
     L.fg_log_f32_device.argtypes = [vp, vp, C.c_int64, vp, vp]
+    L.fg_cancel.argtypes = [vp, C.c_int]
+    L.fg_cancel.restype = C.c_int
     for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
                  "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device",
                  "fg_render_frame", "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device",
@@ -283,6 +286,11 @@ class Context:
     def log_f32_device(self, x, out, stream=None):
         self._check(self._L.fg_log_f32_device(self.handle, C.c_void_p(x.data_ptr()), x.numel(),
                                               C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
+
+    def cancel(self, on: bool = True):
+        """Raise (or lower) the context's cancel flag: the integrators' tiles
+        not yet started return at once and the status reports FG_ECANCELLED."""
+        self._check(self._L.fg_cancel(self.handle, 1 if on else 0))
 
     def stream_status(self, stream=None):
         self._check(self._L.fg_stream_status(self.handle, C.c_void_p(_stream_handle(stream))))

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <map>
 #include <mutex>
 #include <string>
@@ -45,6 +46,10 @@ struct fg_ctx {
     // per-context device scratch: status word, stats counters, offsets range
     int *d_status = nullptr;
     unsigned long long *d_stats = nullptr;
+    // cancel flag: coherent host memory that every tile of the integrators
+    // reads once (host and device addresses)
+    int *h_cancel = nullptr;
+    int *d_cancel = nullptr;
     float4 *d_range = nullptr;
     // scratch of the per-plane grain generation: Poisson thresholds, bin
     // counts and starts, scan workspace, grain records
@@ -586,6 +591,7 @@ int pixel_args(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const
     a.offs = reinterpret_cast<const float2 *>(d_offs);
     a.out = d_out;
     a.status = d_status;
+    a.cancel = ctx->d_cancel;
     a.stats = ctx->d_stats;
     a.tile_list = nullptr;
     a.off_range = d_range;
@@ -738,6 +744,7 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     a.offs = reinterpret_cast<const float2 *>(d_offs);
     a.out = d_out;
     a.status = ctx->d_status;
+    a.cancel = ctx->d_cancel;
     a.stats = ctx->d_stats;
     a.seed = d->seed;
     a.log_mu = (double)d->log_mu;
@@ -803,6 +810,46 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     return FG_OK;
 }
 
+// Device status bits to a status code (no bits: FG_OK).
+int status_error(fg_ctx *ctx, int status, const char *what)
+{
+    if (status & fg::kStatusCancelled) return set_error(ctx, FG_ECANCELLED, "cancelled");
+    if (status & fg::kStatusNeighbourhood)
+        return set_error(ctx, FG_ENEIGHBOURHOOD,
+                         std::string(what) +
+                             ": the cell window of one output pixel does not fit in LDS; "
+                             "use a larger cell size");
+    if (status & fg::kStatusRegion)
+        return set_error(ctx, FG_EINVAL,
+                         std::string(what) + ": the cell region of one tile exceeds 65535 cells per side; "
+                                             "use a larger cell size");
+    if (status & fg::kStatusCapacity)
+        return set_error(ctx, FG_ECAPACITY, "batch grains exceed the planned capacity; create the batch again");
+    if (status != 0) return set_error(ctx, FG_EHIP, std::string(what) + ": device status " + std::to_string(status));
+    return FG_OK;
+}
+
+void set_cancel(fg_ctx *ctx, int on) { __atomic_store_n(ctx->h_cancel, on, __ATOMIC_SEQ_CST); }
+
+// Waits for stream s; while it runs, polls the caller's cancel callback and
+// raises the context's cancel flag when it asks, so that the integrators'
+// remaining tiles return at once (src/pixelwise.rs:28 checks per row).
+hipError_t wait_polling(fg_ctx *ctx, hipStream_t s, fg_cancel_fn cancel, void *user)
+{
+    if (!cancel) return hipStreamSynchronize(s);
+    bool raised = false;
+    for (;;) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q != hipErrorNotReady) return q;
+        if (!raised && cancel(user)) {
+            set_cancel(ctx, 1);
+            raised = true;
+        }
+        struct timespec ts = {0, 200000};  // 0.2 ms
+        nanosleep(&ts, nullptr);
+    }
+}
+
 int read_status(fg_ctx *ctx, hipStream_t s, const char *what)
 {
     int status = 0;
@@ -814,12 +861,7 @@ int read_status(fg_ctx *ctx, hipStream_t s, const char *what)
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_error(ctx, e, what);
     }
-    if (status & fg::kStatusNeighbourhood)
-        return set_error(ctx, FG_ENEIGHBOURHOOD,
-                         std::string(what) +
-                             ": the cell window of one output pixel does not fit in LDS; "
-                             "use a larger cell size");
-    return FG_OK;
+    return status_error(ctx, status, what);
 }
 
 bool finite_pairs(const float *xy, uint32_t n)
@@ -850,15 +892,19 @@ int render_host(fg_ctx *ctx, const fg_plane_desc *d, const float *lambda, const 
         return rc;
     if ((rc = ensure(ctx, &ctx->d_out, &ctx->cap_out, nout, what)) != FG_OK) return rc;
     hipStream_t s = ctx->stream;
+    set_cancel(ctx, 0);
     e = hipMemcpyAsync(ctx->d_lambda, lambda, nin * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ctx->d_offs, offs, (size_t)d->n_samples * 8, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     rc = fn(ctx, d, ctx->d_lambda, ctx->d_offs, ctx->d_out, s, false);
     if (rc != FG_OK) return rc;
-    e = hipMemcpyAsync(out, ctx->d_out, nout * 4, hipMemcpyDeviceToHost, s);
+    e = wait_polling(ctx, s, cancel, user);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->d_out, nout * 4, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    return read_status(ctx, s, what);
+    rc = read_status(ctx, s, what);
+    set_cancel(ctx, 0);
+    return rc;
 }
 
 int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
@@ -1172,6 +1218,11 @@ int fg_ctx_create(int device_ordinal, fg_ctx **out)
     if (e == hipSuccess) e = hipMalloc(&ctx->d_range, sizeof(float4));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_cancel, 64, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) {
+        *ctx->h_cancel = 0;
+        e = hipHostGetDevicePointer((void **)&ctx->d_cancel, ctx->h_cancel, 0);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         fg_ctx_destroy(ctx);
@@ -1190,6 +1241,7 @@ void fg_ctx_destroy(fg_ctx *ctx)
     (void)hipFree(ctx->d_status);
     (void)hipFree(ctx->d_stats);
     (void)hipFree(ctx->d_range);
+    if (ctx->h_cancel) (void)hipHostFree(ctx->h_cancel);
     if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
     for (hipEvent_t ev : ctx->tev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1198,6 +1250,13 @@ void fg_ctx_destroy(fg_ctx *ctx)
 }
 
 const char *fg_last_error(const fg_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int fg_cancel(fg_ctx *ctx, int on)
+{
+    if (!ctx) return FG_EINVAL;
+    set_cancel(ctx, on ? 1 : 0);
+    return FG_OK;
+}
 
 int fg_render_pixelwise(fg_ctx *ctx, const fg_plane_desc *desc, const float *lambda,
                         const float *offsets_input_xy, float *out, fg_cancel_fn cancel,
@@ -1393,10 +1452,7 @@ int fg_batch_status(fg_batch *b, void *hip_stream)
     hipError_t e = hipMemcpyAsync(&status, b->status, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, "Batch renderer");
-    if (status & fg::kStatusCapacity)
-        return set_error(ctx, FG_ECAPACITY, "batch grains exceed the planned capacity; create the batch again");
-    if (status != 0) return set_error(ctx, FG_EHIP, "batch renderer: device status " + std::to_string(status));
-    return FG_OK;
+    return status_error(ctx, status, "Batch renderer");
 }
 
 int fg_batch_info(const fg_batch *b, int *n_items, int *grid_blocks, int *graph_captured)
@@ -1495,16 +1551,22 @@ int fg_render_frame(fg_ctx *ctx, const fg_frame_desc *fd, const float *rgb, cons
     e = hipMemcpyAsync(d_rgb, rgb, nin * 12, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_offs, offsets_xy, (size_t)d.n_samples * 8, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
+    set_cancel(ctx, 0);
     rc = frame_device(ctx, fd, d_rgb, d_offs, d_rgb8, d_planes, s, cancel, cancel_user);
     if (rc != FG_OK) {
         (void)hipStreamSynchronize(s);
+        (void)read_status(ctx, s, what);
+        set_cancel(ctx, 0);
         return rc;
     }
-    e = hipMemcpyAsync(rgb8, d_rgb8, nout * 3, hipMemcpyDeviceToHost, s);
+    e = wait_polling(ctx, s, cancel, cancel_user);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgb8, d_rgb8, nout * 3, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && planes)
         e = hipMemcpyAsync(planes, d_planes, (size_t)nr * nout * 4, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    return read_status(ctx, s, what);
+    rc = read_status(ctx, s, what);
+    set_cancel(ctx, 0);
+    return rc;
 }
 
 int fg_lambda_device(fg_ctx *ctx, const float *d_plane, uint32_t w, uint32_t h, float inv_e_pi_r2, float *d_norm,

@@ -170,6 +170,22 @@ __device__ __forceinline__ void splat_drain(const PixelArgs &a, const SplatTile 
     }
 }
 
+// Cancellation between tiles (a workgroup reads the flag once per tile):
+// true when the host has raised *flag; the caller's status then records it.
+// The flag lives in coherent host memory, so the load goes to the system
+// scope.  Must be called by every thread of the workgroup (it synchronises).
+__device__ __forceinline__ bool tile_cancelled(const int *flag, int *status)
+{
+    __shared__ int s_cancel;
+    if (!flag) return false;
+    if (threadIdx.x == 0) {
+        s_cancel = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (s_cancel) atomicOr(status, kStatusCancelled);
+    }
+    __syncthreads();
+    return s_cancel != 0;
+}
+
 }  // namespace fg
 
 #endif

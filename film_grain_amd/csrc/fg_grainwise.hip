@@ -288,6 +288,7 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
     int tile = blockIdx.x;
     if (a.tile_list) tile = a.tile_list[tile];
     if (tile >= a.n_tiles) return;
+    if (tile_cancelled(a.cancel, a.status)) return;
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
     const int X1 = min(X0 + a.tile_w, a.out_w), Y1 = min(Y0 + a.tile_h, a.out_h);

@@ -24,6 +24,7 @@ struct PixelArgs {
     const float2 *offs;         // offsets_input, n_samples entries
     float *out;                 // out_h x out_w
     int *status;                // device status word
+    const int *cancel;          // optional: the context's host-visible cancel flag
     unsigned long long *stats;  // optional counters: points, tests (T), hits
     const int *tile_list;       // optional tile indices (batching / resubmission)
     const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
@@ -57,6 +58,7 @@ struct GrainArgs {
     const float2 *offs;  // offsets in output pixels, n_samples entries
     float *out;
     int *status;
+    const int *cancel;   // optional: the context's host-visible cancel flag
     unsigned long long *stats;  // optional counters: grains, (grain, sample) pairs, hits
     const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
     uint64_t seed;

@@ -447,6 +447,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     __syncthreads();
     if (tile >= a.n_tiles) return;
     if (a.grain_total && *a.grain_total > a.grain_cap) return;
+    if (tile_cancelled(a.cancel, a.status)) return;
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     SplatTile t;
     t.X0 = txi * a.tile_w;

@@ -116,6 +116,20 @@ int fg_render_pixelwise_adaptive_device(fg_ctx *ctx, const fg_plane_desc *desc, 
                                         const float *d_offsets_input_xy, const uint16_t *d_budget, float *d_out,
                                         void *hip_stream);
 
+/* Cancellation (src/pixelwise.rs:28 polls `cancel` per row; the viewer drops
+ * a stale job, src/bin/viewer.rs:952-965, 1006-1010).  The context owns one
+ * cancel flag in host memory that every tile of the integrators (single
+ * plane, grain-wise and the batch's persistent kernel) reads before it
+ * starts; a raised flag makes the remaining tiles return at once and the
+ * status report FG_ECANCELLED (fg_stream_status, fg_batch_status); the
+ * output is then incomplete.  fg_cancel(ctx, 1) raises it from any thread,
+ * fg_cancel(ctx, 0) lowers it for the next job.  The host-memory entry
+ * points (fg_render_pixelwise, fg_render_grainwise, fg_render_frame) lower
+ * it on entry, poll their callback while the device works, raise it when the
+ * callback asks, and lower it again on return.  No reference counterpart as
+ * an entry point. */
+int fg_cancel(fg_ctx *ctx, int on);
+
 /* Waits for `hip_stream` and returns the device-side status of the work
  * enqueued by the device-resident entry points since the previous call. */
 int fg_stream_status(fg_ctx *ctx, void *hip_stream);
