@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4", "adaptive-c5"])
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the learned-budget comparison appended to the default line")
+    ap.add_argument("--no-frame", action="store_true",
+                    help="skip the host-memory frame comparison appended to the default line")
     ap.add_argument("--batch-images", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -480,7 +482,7 @@ def main():
     if parity is not None:
         line["parity_spot_check"] = parity
     ctx.close()
-    if args.workload == "pixel-c2":
+    if args.workload == "pixel-c2" and not args.no_frame:
         line["frame"] = frame_summary()
     if args.workload == "pixel-c2" and not args.no_adaptive:
         line["adaptive"] = adaptive_summary()

@@ -496,6 +496,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     const float4 ext = *st.ext;
+    // the tile's 64-grain chunks go to its waves on demand (the first NW
+    // statically), so that a wave that drew sparse chunks takes more
+    __shared__ int s_next_chunk;
+    if (tid == 0) s_next_chunk = NW;
     __syncthreads();
 
     const float slack = 4e-3f + 2e-6f * (float)max(a.out_w, a.out_h);
@@ -569,7 +573,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             continue;
         }
         const uint32_t q = rs_ + (uint32_t)(c - cbase) * 64u + (uint32_t)lane;
-        c += NW;
+        {
+            int v = 0;
+            if (lane == 0) v = atomicAdd(&s_next_chunk, 1);
+            c = __builtin_amdgcn_readfirstlane(v);
+        }
         float4 G = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const bool in = q < re_;
         if (in) G = grains[q];
@@ -614,14 +622,12 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             m &= m - 1ull;
             const float ubx = lane_f(bx, l), uby = lane_f(by, l), urz = lane_f(rz, l);
             const int up0 = lane_i(p0, l), ulen = lane_i(len, l);
-            for (int i0 = 0; i0 < ulen; i0 += 64) {
-                const int i = i0 + lane;
-                const int pr = up0 + i;
-                // cyclic position: pr - N when pr >= N (as unsigned, pr - N
-                // wraps above pr otherwise); lanes past the run read the
-                // last sample and queue nothing
-                const int p = (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
-                const float2 X = sxy[p];
+            // A box narrower than one pixel (2 rz < 1) holds at most one
+            // pixel column and one row: decided once per grain.
+            const bool narrow = urz + urz < 1.0f;
+            // One sample of the run per lane (i: index in the run, p: its
+            // sorted position, X: its scaled offset).
+            auto step = [&](int i, int p, float2 X) {
                 // the candidate box (splat_box): a point of the grain's disk
                 // is a pixel in it; the run makes its column range non-empty
                 // up to the widening, and the drain's exact test settles the
@@ -632,8 +638,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 const bool cand = (i < ulen) & (yl <= yh) & (xl <= t.fx1) & (xh >= t.fx0) & (yl <= t.fy1) &
                                   (yh >= t.fy0);
                 const uint32_t base = (uint32_t)l | ((uint32_t)p << 6);
-                const bool multi = cand & ((xh > xl) | (yh > yl));
-                if (!__any(multi)) {
+                if (narrow || !__any(cand & ((xh > xl) | (yh > yl)))) {
                     const int tx = (int)(xl - t.fx0), ty = (int)(yl - t.fy0);
                     push(cand, base | ((uint32_t)tx << 17) | ((uint32_t)ty << 24));
                 } else {
@@ -651,6 +656,21 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         push(c, base | ((uint32_t)(tx + jx) << 17) | ((uint32_t)(ty + jy) << 24));
                     }
                 }
+            };
+            // cyclic position: pr - N when pr >= N (as unsigned, pr - N wraps
+            // above pr otherwise); lanes past the run read the last sample
+            // and queue nothing
+            auto position = [&](int pr) {
+                return (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
+            };
+            // two 64-sample halves per iteration: both halves' LDS reads are
+            // issued before either is tested
+            for (int i0 = 0; i0 < ulen; i0 += 128) {
+                const int ia = i0 + lane, ib = ia + 64;
+                const int pa = position(up0 + ia), pb = position(up0 + ib);
+                const float2 Xa = sxy[pa], Xb = sxy[pb];
+                step(ia, pa, Xa);
+                if (i0 + 64 < ulen) step(ib, pb, Xb);
             }
         }
         // the queue refers to this chunk's lanes: empty it before the next
