@@ -128,26 +128,73 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpLds &s, uint16_t *hbuf
     }
 }
 
+// Forward pass for inference, without LDS: the layers are computed
+// transposed, Z1^T = W1^T X^T (16x16x16: A = a 16-output block of W1^T,
+// B = X^T) and Z2^T = W2^T H1^T (16x16x32), so that a lane's accumulator
+// holds row r = lane % 16 of the activations, columns cb*16 + 4g + i
+// (g = lane / 16, i = 0..3).  Those are exactly the B operand layer 2 needs
+// (row r, eight k per lane) under a fixed permutation of k within each
+// 32-chunk: logical k = 8g + t is H1 column kc*32 + 4g + t (t < 4) or
+// kc*32 + 16 + 4g + t - 4 (t >= 4); W2^T's operand is loaded in the same
+// order, so the products are unchanged.  Weights, biases and w3 live in
+// registers for the whole kernel; each lane reads its row's four features
+// (8 bytes) from HBM.  Layer 3 sums the lane's 16 columns, then across the
+// four lanes of the row.
+__device__ __forceinline__ int mlp_h1_column(int kc, int g, int t)
+{
+    return kc * 32 + (t < 4 ? 4 * g + t : 16 + 4 * g + (t - 4));
+}
+
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *P, const uint16_t *X, long long n, float *out)
 {
-    __shared__ MlpLds s;
-    mlp_load(P, s);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const long long tiles = (n + 15) / 16;
-    for (long long t = (long long)blockIdx.x * 4 + wid; t < tiles; t += (long long)gridDim.x * 4) {
-        uint16_t *xt = s.x[wid];
-        for (int i = lane; i < 256; i += 64) {
-            const long long row = t * 16 + i / 16;
-            xt[i] = row < n ? X[row * 16 + (i % 16)] : (uint16_t)0;
+    v4s a1[4];
+    v8bf a2[4][2];
+    float b1[4][4], b2[4][4], w3[4][4];
+    for (int cb = 0; cb < 4; ++cb) {
+        for (int t = 0; t < 4; ++t) a1[cb][t] = (short)bf16_bits(P[kMlpW1 + (4 * g + t) * 64 + cb * 16 + r]);
+        for (int kc = 0; kc < 2; ++kc)
+            for (int t = 0; t < 8; ++t)
+                a2[cb][kc][t] =
+                    __builtin_bit_cast(__bf16, bf16_bits(P[kMlpW2 + mlp_h1_column(kc, g, t) * 64 + cb * 16 + r]));
+        for (int i = 0; i < 4; ++i) {
+            const int j = cb * 16 + 4 * g + i;
+            b1[cb][i] = P[kMlpB1 + j];
+            b2[cb][i] = P[kMlpB2 + j];
+            w3[cb][i] = bf16_float(bf16_bits(P[kMlpW3 + j]));
         }
-        v4f z1[4], z2[4];
-        float y[4];
-        mlp_tile_forward(s, s.h[wid], xt, z1, z2, y);
-        if (r == 0)
-            for (int i = 0; i < 4; ++i) {
-                const long long row = t * 16 + 4 * g + i;
-                if (row < n) out[row] = y[i];
-            }
+    }
+    const float b3 = P[kMlpB3];
+    const long long tiles = (n + 15) / 16;
+    const long long stride = (long long)gridDim.x * 4;
+    for (long long tt = (long long)blockIdx.x * 4 + wid; tt < tiles; tt += stride) {
+        const long long row = tt * 16 + r;
+        v4s x = {0, 0, 0, 0};
+        if (row < n) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(X + row * 16 + 4 * g);
+            x[0] = (short)(v.x & 0xFFFFu);
+            x[1] = (short)(v.x >> 16);
+            x[2] = (short)(v.y & 0xFFFFu);
+            x[3] = (short)(v.y >> 16);
+        }
+        // layer 1 (transposed) and the B operand of layer 2
+        v8bf h[2];
+        for (int cb = 0; cb < 4; ++cb) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[cb], x, c, 0, 0, 0);
+            for (int i = 0; i < 4; ++i)
+                h[cb >> 1][(cb & 1) * 4 + i] = __builtin_bit_cast(__bf16, bf16_bits(fmaxf(c[i] + b1[cb][i], 0.0f)));
+        }
+        // layer 2 (transposed) and layer 3
+        float acc = 0.0f;
+        for (int cb = 0; cb < 4; ++cb) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int kc = 0; kc < 2; ++kc) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[cb][kc], h[kc], c, 0, 0, 0);
+            for (int i = 0; i < 4; ++i) acc += fmaxf(c[i] + b2[cb][i], 0.0f) * w3[cb][i];
+        }
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        if (g == 0 && row < n) out[row] = acc + b3;
     }
 }
 
@@ -357,8 +404,9 @@ hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, floa
 
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s)
 {
+    // waves loop over 16-row tiles; enough waves to fill the device
     const long long tiles = (n + 15) / 16;
-    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 8192);
+    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 4096);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, n, out);
     return hipGetLastError();
 }
