@@ -26,7 +26,7 @@ two 2048-sample renders over the two halves of the 4096 offsets: each half
 is (covered count) * 2^-11, so their mean is (total count) * 2^-12, the
 4096-sample render, exactly in f32.
 
-Allocators compared (allocate_budgets): "learned" (the network), "binomial"
+Allocators compared (allocator_s, render_adaptive_frame): "learned" (the network), "binomial"
 (s = sqrt(u (1 - u)) of the input intensity: no model) and "oracle" (s from
 the N = 4096 render itself, which no renderer has before rendering: the
 bound on what any per-pixel allocation can gain).
