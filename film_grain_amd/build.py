@@ -33,6 +33,14 @@ DEVICE = [
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# FG_SANITIZE=address builds the host code (never the GPU code) with
+# AddressSanitizer; run Python with the sanitizer runtime preloaded
+# (libclang_rt.asan-x86_64.so of /opt/rocm/llvm) and ASAN_OPTIONS=detect_leaks=0.
+SANITIZE = os.environ.get("FG_SANITIZE", "")
+if SANITIZE:
+    COMMON = [f for f in COMMON if f != "-O3"] + ["-O1", "-g", "-Xarch_host", f"-fsanitize={SANITIZE}",
+                                                 "-Xarch_host", "-fno-omit-frame-pointer"]
+LINK_SANITIZE = ["-Xarch_host", f"-fsanitize={SANITIZE}", "-shared-libasan"] if SANITIZE else []
 
 HIP_SOURCES = ["fg_pixelwise.hip", "fg_pixelwise_hbm.hip", "fg_grainwise.hip", "fg_batch.hip", "fg_mlp.hip",
                "fg_frame.hip", "fg_capi.hip"]
@@ -84,7 +92,7 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, *DEVICE, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz", "-lpthread"])
+    _run([HIPCC, *DEVICE, "-shared", "-fPIC", *LINK_SANITIZE, "-o", str(tmp), *map(str, objs), "-lz", "-lpthread"])
     os.replace(tmp, LIB)
     cli_src = CSRC / "cli" / "filmgrain_main.cpp"
     if cli_src.exists():
