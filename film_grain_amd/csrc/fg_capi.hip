@@ -60,6 +60,7 @@ struct fg_ctx {
     DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
     DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
     DevBuf frame;             // frames: split planes, lambda, max word, rendered planes
+    DevBuf cells;             // grain-wise candidate lists (constant radius)
     DevBuf frame_io;          // frames from host memory: RGB in, offsets, 8-bit RGB out, planes out
     std::vector<int> level_n;
     std::vector<fg::SampleTable> level_tables;
@@ -119,6 +120,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->levels);
     free_buf(ctx->frame);
     free_buf(ctx->frame_io);
+    free_buf(ctx->cells);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -788,6 +790,24 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
         fg::SampleTable st{};
         mark(ctx, 0, s);
         hipError_t e = fg::launch_sample_table(a.offs, a.n_samples, 1.0f, q, ctx->samples.p, &st, s);
+        // constant radius with a narrow window (2R + 1/Q <= 0.35 output px):
+        // candidate lists per cell of the grain's fractional target, about
+        // N (2R + 1/Q)^2 samples each instead of a run of N (2R + 1/Q)
+        const int cq = 32;
+        const float r_out = std::min(d->radius_mean, d->rm) * d->zoom;
+        if (e == hipSuccess && d->dist_kind == 0 && r_out > 0.0f && 2.0f * r_out + 1.0f / (float)cq <= 0.35f) {
+            const size_t nstart = (size_t)cq * cq + 1;
+            if ((rc = ensure_bytes(ctx, ctx->cells, nstart * 4 + (size_t)cq * cq * a.n_samples * 2 + 64, what)) !=
+                FG_OK)
+                return rc;
+            uint32_t *cs = static_cast<uint32_t *>(ctx->cells.p);
+            uint16_t *cl = reinterpret_cast<uint16_t *>(cs + nstart);
+            const float c_base = (float)std::max(a.out_w, a.out_h) + a.r_out_max + 8.0f;
+            e = fg::launch_cell_lists(st, a.n_samples, cq, r_out, c_base, cs, cl, s);
+            a.cell_start = cs;
+            a.cell_list = cl;
+            a.cell_q = cq;
+        }
         if (e == hipSuccess) e = fg::launch_pixel_state(a, s);
         if (e == hipSuccess) e = fg::launch_grain_tile_weights(a, st, order.w, order.ids, s);
         if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);

@@ -400,7 +400,20 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
                 // every sample (bounds() is monotone in the target)
                 if (rad > 0.0f && R > 0.0f && floorf(((bx + ext.y) + R) - 0.5f) >= fx0 &&
                     ceilf(((bx + ext.x) - R) - 0.5f) <= fx1 && floorf(((by + ext.w) + R) - 0.5f) >= fy0 &&
-                    ceilf(((by + ext.z) - R) - 0.5f) <= fy1) {
+                    ceilf(((by + ext.z) - R) - 0.5f) <= fy1 && a.cell_q > 0) {
+                    // the candidate list of the cell of (A, B), the fractions
+                    // of the grain's x and y targets (build_cell_lists)
+                    const float ncx = R - (bx - 0.5f), ncy = R - (by - 0.5f);
+                    const float A = ncx - floorf(ncx), B = ncy - floorf(ncy);
+                    const float cq = (float)a.cell_q;
+                    const int qa = min(max((int)(A * cq), 0), a.cell_q - 1);
+                    const int qb = min(max((int)(B * cq), 0), a.cell_q - 1);
+                    const int c = qb * a.cell_q + qa;
+                    p0 = (int)a.cell_start[c];
+                    len = (int)a.cell_start[c + 1] - p0;
+                } else if (rad > 0.0f && R > 0.0f && floorf(((bx + ext.y) + R) - 0.5f) >= fx0 &&
+                           ceilf(((bx + ext.x) - R) - 0.5f) <= fx1 && floorf(((by + ext.w) + R) - 0.5f) >= fy0 &&
+                           ceilf(((by + ext.z) - R) - 0.5f) <= fy1) {
                     const float nc = R - (bx - 0.5f);
                     const float A = nc - floorf(nc);
                     const float lo = (A - (R + R)) - eps, hi = A + eps;
@@ -426,7 +439,10 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
             gR = R;
             for (int it = 0; __any(it < len); ++it) {
                 const int pr = p0 + it;
-                const int p = (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
+                // a list entry, or the cyclic run position (lanes past their
+                // list or run read a valid sample and queue nothing)
+                const int p = a.cell_q > 0 ? (it < len ? (int)a.cell_list[pr] : 0)
+                                           : (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
                 const float2 X = sxy[p];
                 const float txp = bx + X.x, typ = by + X.y;
                 // bounds() of src/grainwise.rs:126-142 before its clamp; the
@@ -469,6 +485,72 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
         for (int w = 0; w < words; ++w) count += __popc(pw[w]);
         a.out[(size_t)(Y0 + oy) * a.out_w + (X0 + ox)] = (float)count * a.inv_samples;
     }
+}
+
+// Candidate lists for grains of one constant output radius R: cell (qa, qb)
+// of a Q x Q grid over the fractions (A, B) of a grain's x and y targets
+// lists, in sorted position order, every sample whose fractions lie in
+// [qa/Q - 2R - eps, (qa+1)/Q + eps] and [qb/Q - 2R - eps, (qb+1)/Q + eps]
+// (mod 1), each widened by 2^-12.  A grain in the cell needs at most the
+// samples whose fraction lies in [A - 2R - eps, A + eps] (its box then has a
+// column: the per-grain run's argument) and in [B - 2R - eps, B + eps] (a
+// row), so the list is a superset of the candidates and the exact tests
+// downstream decide.  eps is the kernel's: C * 2^-20 + 2^-16 with C =
+// c_base + the offsets' largest magnitude.  One workgroup, one cell per
+// thread and pass; start has Q * Q + 1 entries, list room for Q * Q * N.
+__global__ __launch_bounds__(1024) void cell_lists_kernel(SampleTable st, int N, int Q, float R, float c_base,
+                                                          uint32_t *start, uint16_t *list)
+{
+    __shared__ uint32_t s_count[1024];
+    const float4 ext = *st.ext;
+    const float C = c_base + fmaxf(fmaxf(fabsf(ext.x), fabsf(ext.y)), fmaxf(fabsf(ext.z), fabsf(ext.w)));
+    const float eps = C * 9.5367431640625e-07f + 1.52587890625e-05f;
+    const float m = 2.44140625e-4f;  // 2^-12
+    const float w = ((R + R) + (eps + eps)) + (1.0f / (float)Q + (m + m));
+    const int cells = Q * Q;
+    auto member = [&](int c, int p) {
+        const float2 X = st.xy[p];
+        const float fx = X.x - floorf(X.x), fy = X.y - floorf(X.y);
+        const float lx = ((float)(c % Q) / (float)Q - (R + R)) - (eps + m);
+        const float ly = ((float)(c / Q) / (float)Q - (R + R)) - (eps + m);
+        float dx = fx - lx, dy = fy - ly;
+        dx -= floorf(dx);
+        dy -= floorf(dy);
+        return dx <= w && dy <= w;
+    };
+    uint32_t base = 0;
+    for (int c0 = 0; c0 < cells; c0 += 1024) {
+        const int c = c0 + (int)threadIdx.x;
+        uint32_t n = 0;
+        if (c < cells)
+            for (int p = 0; p < N; ++p) n += member(c, p) ? 1u : 0u;
+        s_count[threadIdx.x] = n;
+        __syncthreads();
+        // inclusive scan of the pass's counts (Hillis-Steele)
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t v = threadIdx.x >= (unsigned)o ? s_count[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s_count[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const uint32_t first = base + s_count[threadIdx.x] - n;
+        if (c < cells) {
+            start[c] = first;
+            uint32_t k = first;
+            for (int p = 0; p < N; ++p)
+                if (member(c, p)) list[k++] = (uint16_t)p;
+        }
+        base += s_count[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[cells] = base;
+}
+
+hipError_t launch_cell_lists(const SampleTable &st, int n, int q, float r_out, float c_base, uint32_t *start,
+                             uint16_t *list, hipStream_t s)
+{
+    hipLaunchKernelGGL(cell_lists_kernel, dim3(1), dim3(1024), 0, s, st, n, q, r_out, c_base, start, list);
+    return hipGetLastError();
 }
 
 size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets)

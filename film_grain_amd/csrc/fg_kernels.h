@@ -73,6 +73,13 @@ struct GrainArgs {
     uint32_t *pix_q;
     ulonglong2 *pix_s01, *pix_s23;
     const int *tile_list;  // optional launch order of the tiles (sorted kernel)
+    // optional candidate lists of the sorted kernel (constant radius): cell
+    // qb * cell_q + qa of a cell_q x cell_q grid over the grain's fractional
+    // target (A, B) lists sorted positions cell_list[cell_start[c] ..
+    // cell_start[c + 1]); cell_q = 0: the per-grain run instead
+    const uint32_t *cell_start;
+    const uint16_t *cell_list;
+    int cell_q;
 };
 
 // Writes (min x, max x, min y, max y) of n (x, y) pairs to *range (one workgroup).

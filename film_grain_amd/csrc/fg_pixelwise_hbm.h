@@ -58,6 +58,12 @@ size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 hipError_t launch_pixel_state(const GrainArgs &a, hipStream_t s);
 hipError_t launch_grainwise_sorted(const GrainArgs &a, const SampleTable &st, int dist_kind, int block,
                                    hipStream_t s);
+// Candidate lists of the sorted grain-wise kernel for one constant output
+// radius r_out (GrainArgs::cell_*): q x q cells, start (q * q + 1 entries),
+// list (room for q * q * n positions); c_base = max(out_w, out_h) + the
+// largest output radius + 8 (the kernel's widening bound without the offsets).
+hipError_t launch_cell_lists(const SampleTable &st, int n, int q, float r_out, float c_base, uint32_t *start,
+                             uint16_t *list, hipStream_t s);
 
 // Heaviest-first tile order: w[t] = the work of tile t (grains of the bins
 // covering its region), ids[t] = t; sort_tiles orders ids by w, descending

@@ -95,3 +95,27 @@ def test_random_planes_adaptive_uniform_budget(hip_ctx, oracle):
         bad = int(np.sum(out.cpu().numpy() != want))
         print(f"[random] {case.name} adaptive budget {b}: {bad} differ", flush=True)
         assert bad == 0, case.name
+
+
+@pytest.mark.parametrize("r_out", [0.01, 0.05, 0.12, 0.159, 0.1595, 0.17])
+def test_grainwise_candidate_lists_near_the_switch(hip_ctx, oracle, r_out):
+    """Constant output radii on both sides of the switch to per-cell
+    candidate lists (2 R + 1/32 <= 0.35 output pixels, so R <= 0.159375):
+    the sorted kernel gives the oracle's planes bit for bit, for sample
+    counts from 1 to 2048 and several zooms."""
+    rng = np.random.default_rng(int(r_out * 1e4))
+    for zoom, n in [(0.5, 2048), (1.0, 1), (1.7, 100), (3.0, 513)]:
+        w, h = 24, 19
+        # the oracle walks every (grain, sample) pair: about 1.4 / (pi r^2)
+        # grains per input pixel at the field's largest grey level
+        grains = w * h * 1.4 / (np.pi * (r_out / zoom) ** 2)
+        while n > 1 and (round(w * zoom) * round(h * zoom) * n > 2_000_000 or grains * n > 1e9):
+            n //= 2
+        u = synth.sinusoid_field(w, h, int(rng.integers(0, 1 << 30)), n_waves=3)
+        case = fgcases.build_case(oracle, f"cl_R{r_out}_z{zoom}_N{n}", u, r_out / zoom, 0.0, "const", zoom=zoom,
+                                  n_samples=n, sigma=0.8, seed=int(rng.integers(0, 1 << 40)))
+        want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
+        got = _render(hip_ctx, case, "grain", dict(grain_variant=2))
+        bad = int(np.sum(got != want))
+        print(f"[random] {case.name} grain sorted: {bad} differ", flush=True)
+        assert bad == 0, case.name

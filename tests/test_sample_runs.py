@@ -71,3 +71,57 @@ def test_run_contains_every_candidate(N, z, sd):
         walked += int(sel.sum())
         true += int(has_col.sum())
     assert walked <= 1.5 * true + 4 * M
+
+
+def cell_members(fx, fy, qa, qb, Q, R, eps):
+    """Membership of samples (fractions fx, fy) in the candidate list of
+    cell (qa, qb), in the f32 operation order of cell_lists_kernel in
+    fg_grainwise.hip."""
+    m = f32(2.0 ** -12)
+    w = f32(f32(f32(R + R) + f32(eps + eps)) + f32(f32(1) / f32(Q) + f32(m + m)))
+    lx = f32(f32(f32(qa) / f32(Q) - f32(R + R)) - f32(eps + m))
+    ly = f32(f32(f32(qb) / f32(Q) - f32(R + R)) - f32(eps + m))
+    dx = (fx - lx).astype(f32)
+    dy = (fy - ly).astype(f32)
+    dx = (dx - np.floor(dx)).astype(f32)
+    dy = (dy - np.floor(dy)).astype(f32)
+    return (dx <= w) & (dy <= w)
+
+
+@pytest.mark.parametrize("N,R,sd", [(1, 0.05, 0.8), (128, 0.05, 0.8), (513, 0.159, 1.5), (2048, 0.01, 0.4)])
+def test_cell_list_contains_every_candidate(N, R, sd):
+    """The grain-wise kernel's per-cell candidate lists (constant radius):
+    every sample whose f32 bounds() box (src/grainwise.rs:126-142) has both
+    a column and a row is in the list of the cell of the grain's target
+    fractions (A, B), for grain centres anywhere in a 4096-pixel plane and
+    at half-integers +- 1e-4."""
+    rng = np.random.default_rng(N)
+    X = rng.normal(0, sd, size=(N, 2)).astype(f32)
+    fx = (X[:, 0] - np.floor(X[:, 0])).astype(f32)
+    fy = (X[:, 1] - np.floor(X[:, 1])).astype(f32)
+    R = f32(R)
+    Q = 32
+    c = f32(f32(4096 + R + 8) + np.abs(X).max())
+    eps = f32(f32(c * f32(2.0 ** -20)) + f32(2.0 ** -16))
+    M = 3000
+    b = rng.uniform(-8, 4100, size=(M, 2)).astype(f32)
+    b[: M // 2] = (np.round(b[: M // 2] * 2) / 2 + rng.choice([-1e-4, 0, 1e-4], size=(M // 2, 2))).astype(f32)
+    listed = true = 0
+    for bx, by in b:
+        tx = (bx + X[:, 0]).astype(f32)
+        ty = (by + X[:, 1]).astype(f32)
+        col = np.ceil(((tx - R).astype(f32) - f32(0.5)).astype(f32)) <= np.floor(
+            ((tx + R).astype(f32) - f32(0.5)).astype(f32))
+        row = np.ceil(((ty - R).astype(f32) - f32(0.5)).astype(f32)) <= np.floor(
+            ((ty + R).astype(f32) - f32(0.5)).astype(f32))
+        ncx, ncy = f32(R - f32(bx - f32(0.5))), f32(R - f32(by - f32(0.5)))
+        A, B = f32(ncx - np.floor(ncx)), f32(ncy - np.floor(ncy))
+        qa = min(max(int(f32(A * f32(Q))), 0), Q - 1)
+        qb = min(max(int(f32(B * f32(Q))), 0), Q - 1)
+        sel = cell_members(fx, fy, qa, qb, Q, R, eps)
+        assert not np.any(col & row & ~sel), (N, float(R), float(bx), float(by))
+        listed += int(sel.sum())
+        true += int((col & row).sum())
+    # about N (2R + 1/Q)^2 per cell against N (2R)^2 true candidates
+    w = 2 * float(R) + 1.0 / Q + 2 * float(eps) + 2.0 ** -11
+    assert listed <= 1.5 * M * N * w * w + 4 * M, (listed, true)
