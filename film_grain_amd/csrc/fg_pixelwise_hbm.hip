@@ -625,6 +625,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             // A box narrower than one pixel (2 rz < 1) holds at most one
             // pixel column and one row: decided once per grain.
             const bool narrow = urz + urz < 1.0f;
+            // Such a grain with no cell window to check (rs > 0) takes the
+            // reference's disk test of its one pixel in the walk itself.
+            const float urs = lane_f(gr, l);
+            const bool direct = narrow && urs > 0.0f;
             // One sample of the run per lane (i: index in the run, p: its
             // sorted position, X: its scaled offset).
             auto step = [&](int i, int p, float2 X) {
@@ -665,6 +669,37 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             };
             // two 64-sample halves per iteration: both halves' LDS reads are
             // issued before either is tested
+            if (direct) {
+                const float ucx = lane_f(gcx, l), ucy = lane_f(gcy, l);
+                // A pixel whose point passes the disk test lies within rz of
+                // the sample's target (cxk, cyk) in each axis (the candidate
+                // box is a superset); with rz < 0.5 that is only the pixel
+                // nearest the target.  So the drain's test (same f32
+                // expression) of that one pixel, when it lies in the tile,
+                // sets exactly the bits the queue would have set.
+                auto test = [&](int i, float2 X, float2 O, int k) {
+                    const float px = rintf(ubx + X.x), py = rintf(uby + X.y);
+                    if ((i < ulen) & (px >= t.fx0) & (px <= t.fx1) & (py >= t.fy0) & (py <= t.fy1)) {
+                        const float xg = (px + 0.5f) * iz - O.x;
+                        const float yg = (py + 0.5f) * iz - O.y;
+                        const float dx = xg - ucx, dy = yg - ucy;
+                        if (dx * dx + dy * dy <= urs * urs) {
+                            const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
+                            atomicOr(&bits[(ty * t.TW + tx) * words + (k >> 5)], 1u << (k & 31));
+                        }
+                    }
+                };
+                for (int i0 = 0; i0 < ulen; i0 += 128) {
+                    const int ia = i0 + lane, ib = ia + 64;
+                    const int pa = position(up0 + ia), pb = position(up0 + ib);
+                    const float2 Xa = sxy[pa], Xb = sxy[pb];
+                    const float2 Oa = so[pa], Ob = so[pb];
+                    const int ka = sk[pa], kb = sk[pb];
+                    test(ia, Xa, Oa, ka);
+                    if (i0 + 64 < ulen) test(ib, Xb, Ob, kb);
+                }
+                continue;
+            }
             for (int i0 = 0; i0 < ulen; i0 += 128) {
                 const int ia = i0 + lane, ib = ia + 64;
                 const int pa = position(up0 + ia), pb = position(up0 + ib);

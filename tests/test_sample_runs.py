@@ -125,3 +125,50 @@ def test_cell_list_contains_every_candidate(N, R, sd):
     # about N (2R + 1/Q)^2 per cell against N (2R)^2 true candidates
     w = 2 * float(R) + 1.0 / Q + 2 * float(eps) + 2.0 ** -11
     assert listed <= 1.5 * M * N * w * w + 4 * M, (listed, true)
+
+
+@pytest.mark.parametrize("z,out_side", [(0.5, 2048), (1.0, 640), (2.0, 3840), (3.7, 4096)])
+def test_narrow_grain_passes_only_the_nearest_pixel(z, out_side):
+    """The splat kernel's direct test (fg_pixelwise_hbm.hip): for a grain with
+    2 rz < 1, rz = rad z + slack, a pixel whose point passes the reference's
+    f32 disk test (src/pixelwise.rs) is the pixel nearest the sample's
+    target cxk = (gx z - 0.5) + ox z, in both axes.  Checked in IEEE f32 on
+    the 6x6 pixels around every target, for radii up to the largest narrow
+    one and targets within 1e-3 of half-integers."""
+    rng = np.random.default_rng(int(z * 10) + out_side)
+    zf, iz = f32(z), f32(f32(1) / f32(z))
+    slack = f32(f32(4e-3) + f32(f32(2e-6) * f32(out_side)))
+    in_side = out_side / z
+    M, N = 3000, 64
+    g = rng.uniform(0, in_side, size=(M, 2)).astype(f32)
+    o = rng.normal(0, 0.8, size=(N, 2)).astype(f32)
+    r_max = float((f32(0.5) - slack) / zf)
+    rad = rng.uniform(0.0, 1.0, M) ** 0.3 * r_max
+    rad = np.minimum(rad, r_max * (1 - 1e-6)).astype(f32)
+    # half the grains: targets of sample 0 within 1e-3 of a half-integer
+    h = M // 2
+    t = (np.floor(g[:h] * zf) + f32(0.5) + rng.uniform(-1e-3, 1e-3, size=(h, 2))).astype(f32)
+    g[:h] = ((t - o[0] * zf + f32(0.5)) / zf).astype(f32)
+    X = (o * zf).astype(f32)
+    b = ((g * zf).astype(f32) - f32(0.5)).astype(f32)
+    rz = (f32(1) * (rad * zf).astype(f32) + slack).astype(f32)
+    assert np.all(rz + rz < f32(1))
+    checked = hits = 0
+    for i in range(M):
+        ck = (b[i][None, :] + X).astype(f32)  # (N, 2) targets
+        near = np.rint(ck).astype(f32)
+        base = np.floor(ck).astype(np.int64)
+        for dxp in range(-2, 4):
+            for dyp in range(-2, 4):
+                cx = (base[:, 0] + dxp).astype(f32)
+                cy = (base[:, 1] + dyp).astype(f32)
+                xg = (((cx + f32(0.5)).astype(f32) * iz).astype(f32) - o[:, 0]).astype(f32)
+                yg = (((cy + f32(0.5)).astype(f32) * iz).astype(f32) - o[:, 1]).astype(f32)
+                dx = (xg - g[i, 0]).astype(f32)
+                dy = (yg - g[i, 1]).astype(f32)
+                inside = ((dx * dx).astype(f32) + (dy * dy).astype(f32)).astype(f32) <= f32(rad[i] * rad[i])
+                bad = inside & ((cx != near[:, 0]) | (cy != near[:, 1]))
+                assert not np.any(bad), (z, out_side, i, float(rad[i]))
+                hits += int(inside.sum())
+                checked += N
+    assert hits > 0
