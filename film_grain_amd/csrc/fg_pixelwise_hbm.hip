@@ -359,7 +359,7 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {
     const size_t words = ((size_t)n_samples + 31) / 32;
-    return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 8) * 2 +
+    return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 16) +
            align16((size_t)n_samples * 2) + align16((size_t)(qbuckets + 1) * 2);
 }
 
@@ -471,15 +471,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
     const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
 
-    // ---- LDS: bitset | xy[N] | o[N] | k[N] | qidx[Q + 1] ----
+    // ---- LDS: bitset | (xy, o)[N] | k[N] | qidx[Q + 1] ----
+    // a sample's scaled offset and its offset in input pixels side by side:
+    // the direct test reads both with one 16-byte load
     const int tile_words = a.tile_w * a.tile_h * words;
     size_t off = 0;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
     off += align16((size_t)tile_words * 4);
-    float2 *sxy = reinterpret_cast<float2 *>(lds + off);
-    off += align16((size_t)N * 8);
-    float2 *so = reinterpret_cast<float2 *>(lds + off);
-    off += align16((size_t)N * 8);
+    float4 *sxo = reinterpret_cast<float4 *>(lds + off);
+    off += align16((size_t)N * 16);
     uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)N * 2);
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
@@ -490,8 +490,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     }
     for (int w = tid; w < tile_words; w += BLOCK) bits[w] = 0u;
     for (int k = tid; k < N; k += BLOCK) {
-        sxy[k] = st.xy[k];
-        so[k] = st.o[k];
+        const float2 X = st.xy[k], O = st.o[k];
+        sxo[k] = make_float4(X.x, X.y, O.x, O.y);
         sk[k] = st.k[k];
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
@@ -527,7 +527,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const uint32_t cell = (uint32_t)__shfl((int)gcell, l, 64);
         if (lane >= n) return;
         const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
-        const float2 O = so[p];
+        const float4 XO = sxo[p];
+        const float2 O = make_float2(XO.z, XO.w);
         const int k = sk[p];
         const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
         const float yg = ((float)(t.Y0 + ty) + 0.5f) * iz - O.y;
@@ -677,35 +678,42 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 // nearest the target.  So the drain's test (same f32
                 // expression) of that one pixel, when it lies in the tile,
                 // sets exactly the bits the queue would have set.
-                auto test = [&](int i, float2 X, float2 O, int k) {
-                    const float px = rintf(ubx + X.x), py = rintf(uby + X.y);
-                    if ((i < ulen) & (px >= t.fx0) & (px <= t.fx1) & (py >= t.fy0) & (py <= t.fy1)) {
-                        const float xg = (px + 0.5f) * iz - O.x;
-                        const float yg = (py + 0.5f) * iz - O.y;
+                const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
+                auto test = [&](int i, float4 XO, int k) {
+                    const float px = rintf(ubx + XO.x), py = rintf(uby + XO.y);
+                    const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
+                    if ((i < ulen) & ((unsigned)tx < (unsigned)tw) & ((unsigned)ty < (unsigned)th)) {
+                        const float xg = (px + 0.5f) * iz - XO.z;
+                        const float yg = (py + 0.5f) * iz - XO.w;
                         const float dx = xg - ucx, dy = yg - ucy;
                         if (dx * dx + dy * dy <= urs * urs) {
-                            const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
-                            atomicOr(&bits[(ty * t.TW + tx) * words + (k >> 5)], 1u << (k & 31));
+                            // tile coordinates and words are far below 2^24
+                            const uint32_t px_i = __umul24((uint32_t)ty, (uint32_t)t.TW) + (uint32_t)tx;
+                            atomicOr(&bits[__umul24(px_i, (uint32_t)words) + ((uint32_t)k >> 5)], 1u << (k & 31));
                         }
                     }
                 };
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
                     const int pa = position(up0 + ia), pb = position(up0 + ib);
-                    const float2 Xa = sxy[pa], Xb = sxy[pb];
-                    const float2 Oa = so[pa], Ob = so[pb];
+                    const float4 XOa = sxo[pa], XOb = sxo[pb];
                     const int ka = sk[pa], kb = sk[pb];
-                    test(ia, Xa, Oa, ka);
-                    if (i0 + 64 < ulen) test(ib, Xb, Ob, kb);
+                    // all six reads issue together, ahead of the tests (an
+                    // empty asm that takes the values as operands keeps the
+                    // compiler from moving the offset and index reads into
+                    // the branches after the dependent ones)
+                    asm volatile("" ::"v"(XOa.z), "v"(XOa.w), "v"(ka), "v"(XOb.z), "v"(XOb.w), "v"(kb));
+                    test(ia, XOa, ka);
+                    if (i0 + 64 < ulen) test(ib, XOb, kb);
                 }
                 continue;
             }
             for (int i0 = 0; i0 < ulen; i0 += 128) {
                 const int ia = i0 + lane, ib = ia + 64;
                 const int pa = position(up0 + ia), pb = position(up0 + ib);
-                const float2 Xa = sxy[pa], Xb = sxy[pb];
-                step(ia, pa, Xa);
-                if (i0 + 64 < ulen) step(ib, pb, Xb);
+                const float4 Xa = sxo[pa], Xb = sxo[pb];
+                step(ia, pa, make_float2(Xa.x, Xa.y));
+                if (i0 + 64 < ulen) step(ib, pb, make_float2(Xb.x, Xb.y));
             }
         }
         // the queue refers to this chunk's lanes: empty it before the next
