@@ -457,7 +457,14 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
                 const int tx = (int)(cxl - fx0), ty = (int)(cyl - fy0);
                 const bool multi = cand & ((cxh > cxl) | (cyh > cyl));
                 if (!__any(multi)) {
-                    push(cand, base | ((uint32_t)tx << 17) | ((uint32_t)ty << 24));
+                    // one pixel per candidate: the drain's test of it, run
+                    // here on the lane's own grain (same f32 expression)
+                    if (cand) {
+                        const int k = sk[p];
+                        const float dx = (cxl + 0.5f) - txp, dy = (cyl + 0.5f) - typ;
+                        if (dx * dx + dy * dy <= R * R)
+                            atomicOr(&bits[(ty * a.tile_w + tx) * words + (k >> 5)], 1u << (k & 31));
+                    }
                 } else {
                     const int bw = cand ? (int)(cxh - cxl) + 1 : 0, bh = cand ? (int)(cyh - cyl) + 1 : 0;
                     const int nb = bw * bh;
