@@ -60,7 +60,7 @@ struct fg_ctx {
     DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
     DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
     DevBuf frame;             // frames: split planes, lambda, max word, rendered planes
-    DevBuf cells;             // grain-wise candidate lists (constant radius)
+    DevBuf cells;             // candidate lists: grain-wise constant radius, splat direct test
     DevBuf frame_io;          // frames from host memory: RGB in, offsets, 8-bit RGB out, planes out
     std::vector<int> level_n;
     std::vector<fg::SampleTable> level_tables;
@@ -571,6 +571,32 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
         // the host arrays above must outlive the copies
         if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
+    if (e == hipSuccess && !a.budget && a.n_samples > 64 && a.n_samples <= 2048) {
+        // candidate lists for the direct test: classes of 1/64 output pixel
+        // up to the narrow limit rz < 0.5 and the plane's largest rz
+        const int cq = 32;
+        const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
+        const float rz_max = std::min(a.rm * a.zoom + slack, 0.5f);
+        const int classes = std::max(1, std::min(32, (int)std::ceil(rz_max * 64.0f)));
+        const int ew = (int)std::ceil((2.0 * (double)st.eps + std::ldexp(1.0, -11)) * cq + 0.01);
+        const size_t ncell = (size_t)classes * cq * cq;
+        const size_t entries = fg::splat_lists_entries(cq, classes, ew, a.n_samples);
+        if (entries < ((size_t)1 << 31)) {
+            const size_t head = 256;  // per-class fill counters
+            if ((rc = ensure_bytes(ctx, ctx->cells, head + ncell * 8 + entries * 2 + 64, what)) != FG_OK)
+                return rc;
+            unsigned char *base = static_cast<unsigned char *>(ctx->cells.p);
+            uint32_t *fill = reinterpret_cast<uint32_t *>(base);
+            uint2 *cs = reinterpret_cast<uint2 *>(base + head);
+            uint16_t *cl = reinterpret_cast<uint16_t *>(cs + ncell);
+            e = fg::launch_splat_lists(st, cq, classes, ew, fill, cs, cl, a.status, s);
+            a.cl_cell = cs;
+            a.cl_list = cl;
+            a.cl_q = cq;
+            a.cl_classes = classes;
+            a.cl_inv_dz = (float)(2 * cq);
+        }
+    }
     if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a, g, start, order.w, order.ids, s);
     if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);
     a.tile_list = order.order;
@@ -844,7 +870,9 @@ int status_error(fg_ctx *ctx, int status, const char *what)
                          std::string(what) + ": the cell region of one tile exceeds 65535 cells per side; "
                                              "use a larger cell size");
     if (status & fg::kStatusCapacity)
-        return set_error(ctx, FG_ECAPACITY, "batch grains exceed the planned capacity; create the batch again");
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": planned capacity exceeded (a batch's grains, or the splat's "
+                                             "candidate lists); for a batch, create it again");
     if (status != 0) return set_error(ctx, FG_EHIP, std::string(what) + ": device status " + std::to_string(status));
     return FG_OK;
 }

@@ -50,6 +50,15 @@ struct PixelArgs {
     const uint16_t *budget;
     const uint8_t *tile_level;
     const void *level_tables;
+    // HBM splat, optional (cl_q == 0: off): candidate lists of the sorted
+    // sample order per (radius class, cell of the grain's centre fractions),
+    // for narrow grains that take the direct disk test.  Class k holds grains
+    // with rz < (k + 1) / cl_inv_dz; list (k, c) is cl_cell[k * cl_q^2 + c]
+    // = (start, length) in cl_list.
+    const uint2 *cl_cell;
+    const uint16_t *cl_list;
+    int cl_q, cl_classes;
+    float cl_inv_dz;
 };
 
 // Grain-wise integrator launch (src/grainwise.rs:12-124).

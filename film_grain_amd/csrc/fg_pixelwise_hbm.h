@@ -40,6 +40,15 @@ size_t sample_table_bytes(int n, int qbuckets);
 hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
                                hipStream_t s);
 
+// Candidate lists of the HBM splat (PixelArgs::cl_*): q * q cells per class,
+// class k of radius (k + 1) / (2 q).  e is an integer >= (2 st.eps + 2^-11)
+// q; list needs splat_lists_entries(q, classes, e, st.n) entries, cell
+// classes * q * q (start, length) pairs, fill classes counters (cleared here).
+// A class beyond its room sets kStatusCapacity in *status.
+size_t splat_lists_entries(int q, int classes, int e, int n);
+hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
+                              uint16_t *list, int *status, hipStream_t s);
+
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s);
 // bin_mask: four 64-bit words per bin, the nonempty cells (bin_grains reads them).
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,

@@ -356,6 +356,105 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
 // ---------------------------------------------------------------------------
 // splat from HBM grains
 // ---------------------------------------------------------------------------
+// Candidate lists for the direct test.  A grain's candidate box has a column
+// only if frac(X) lies within rz + eps of c = frac(-bx), bx = cx * zoom - 0.5
+// (c is exact in f32, and so is its cell floor(c * q) for q a power of two):
+// the f32 box and cxk = bx + X take a few roundings of at most C * 2^-24
+// each, which eps covers.  Likewise rows with frac(Y) and frac(-by).  The
+// list of class k (rz < R = (k + 1) dz) and cell (qa, qb) holds, in sorted
+// order, every sample with frac(X) in [qa/q - R - eps, (qa+1)/q + R + eps]
+// and frac(Y) in [qb/q - R - eps, (qb+1)/q + R + eps] (mod 1), each widened
+// by 2^-12 for the rounding of this test and of the class.  With dz =
+// 1 / (2 q) a window is (k + 2 + (2 eps + 2^-11) q) / q wide, so a sample
+// lies in at most splat_list_span(k, q, e) cells per axis when e >=
+// (2 eps + 2^-11) q, and class k needs at most n * span^2 list entries.
+// Blocks of 256 cells of one class; both passes test every sample, and each
+// block reserves its cells' room in the class's region with one atomic.
+__host__ __device__ inline uint32_t splat_list_span(int k, int q, int e)
+{
+    return (uint32_t)min(q, k + 4 + e);
+}
+
+__host__ __device__ inline size_t splat_list_base(int k, int q, int e, int n)
+{
+    size_t b = 0;
+    for (int j = 0; j < k; ++j) b += (size_t)n * splat_list_span(j, q, e) * splat_list_span(j, q, e);
+    return b;
+}
+
+__global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q, int e, uint32_t *fill, uint2 *cell,
+                                                          uint16_t *list, int *status)
+{
+    __shared__ float2 s_f[2048];
+    __shared__ uint32_t s_scan[256];
+    __shared__ uint32_t s_base;
+    const int cells = q * q, parts = (cells + 255) / 256, N = st.n;
+    const int k = blockIdx.x / parts, c = (blockIdx.x - k * parts) * 256 + (int)threadIdx.x;
+    for (int p = threadIdx.x; p < N; p += 256) {
+        const float2 X = st.xy[p];
+        s_f[p] = make_float2(X.x - floorf(X.x), X.y - floorf(X.y));
+    }
+    __syncthreads();
+    const float m = 2.44140625e-4f;  // 2^-12
+    const float R = (float)(k + 1) / (float)(2 * q);
+    const float w = ((R + R) + (st.eps + st.eps)) + (1.0f / (float)q + (m + m));
+    const float lx = ((float)(c % q) / (float)q - R) - (st.eps + m);
+    const float ly = ((float)(c / q) / (float)q - R) - (st.eps + m);
+    auto member = [&](int p) {
+        const float2 f = s_f[p];
+        float dx = f.x - lx, dy = f.y - ly;
+        dx -= floorf(dx);
+        dy -= floorf(dy);
+        return dx <= w && dy <= w;
+    };
+    uint32_t n = 0;
+    if (c < cells)
+        for (int p = 0; p < N; ++p) n += member(p) ? 1u : 0u;
+    // the block's cells take one contiguous range of the class's room
+    s_scan[threadIdx.x] = n;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t v = (int)threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    __shared__ bool s_fits;
+    if (threadIdx.x == 255) {
+        const uint32_t span = splat_list_span(k, q, e);
+        const uint32_t off = atomicAdd(&fill[k], s_scan[255]);
+        s_base = (uint32_t)splat_list_base(k, q, e, N) + off;
+        // the room argument above; a block beyond it writes nothing and
+        // fails the render instead
+        s_fits = (size_t)off + s_scan[255] <= (size_t)N * span * span;
+        if (!s_fits) atomicOr(status, kStatusCapacity);
+    }
+    __syncthreads();
+    if (c < cells && s_fits) {
+        uint32_t pos = s_base + s_scan[threadIdx.x] - n;
+        cell[(size_t)k * cells + c] = make_uint2(pos, n);
+        for (int p = 0; p < N; ++p)
+            if (member(p)) list[pos++] = (uint16_t)p;
+    }
+}
+
+size_t splat_lists_entries(int q, int classes, int e, int n)
+{
+    return splat_list_base(classes, q, e, n);
+}
+
+hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
+                              uint16_t *list, int *status, hipStream_t s)
+{
+    if (q < 1 || q * q > 65536 || classes < 1 || st.n > 2048) return hipErrorInvalidValue;
+    hipError_t err = hipMemsetAsync(fill, 0, sizeof(uint32_t) * classes, s);
+    if (err != hipSuccess) return err;
+    const int parts = (q * q + 255) / 256;
+    hipLaunchKernelGGL(splat_lists_kernel, dim3(classes * parts), dim3(256), 0, s, st, q, e, fill, cell, list,
+                       status);
+    return hipGetLastError();
+}
+
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {
     const size_t words = ((size_t)n_samples + 31) / 32;
@@ -613,14 +712,44 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 len = end - p0;
             }
         }
+        // a narrow grain that takes the direct test and has a radius class:
+        // its cell's candidate list replaces the run
+        bool listed = false;
+        if (keep && a.cl_q > 0 && rad <= safe_r && rz + rz < 1.0f) {
+            const int k = (int)(rz * a.cl_inv_dz);
+            if (k < a.cl_classes) {
+                const float ncx = -bx, ncy = -by;
+                const float fx = ncx - floorf(ncx), fy = ncy - floorf(ncy);
+                const float qf = (float)a.cl_q;
+                const int qa = min((int)(fx * qf), a.cl_q - 1), qb = min((int)(fy * qf), a.cl_q - 1);
+                const uint2 se = a.cl_cell[(size_t)k * (a.cl_q * a.cl_q) + qb * a.cl_q + qa];
+                p0 = (int)se.x;
+                len = (int)se.y;
+                listed = true;
+            }
+        }
+        const unsigned long long lmask = __ballot(listed);
         gcx = G.x;
         gcy = G.y;
         gr = rad <= safe_r ? rad : -rad;
         gcell = pk;
         unsigned long long m = __ballot(keep);
+        // the first 128 list entries of the next listed grain, loaded while
+        // the current one is tested
+        int na = 0, nb = 0;
+        auto prefetch = [&](int l) {
+            if ((lmask >> l) & 1ull) {
+                const int q0 = lane_i(p0, l), qn = lane_i(len, l);
+                na = lane < qn ? (int)a.cl_list[q0 + lane] : 0;
+                nb = lane + 64 < qn ? (int)a.cl_list[q0 + 64 + lane] : 0;
+            }
+        };
+        if (m) prefetch(__ffsll((long long)m) - 1);
         while (m) {
             const int l = __ffsll((long long)m) - 1;
             m &= m - 1ull;
+            const int ca = na, cb = nb;
+            if (m) prefetch(__ffsll((long long)m) - 1);
             const float ubx = lane_f(bx, l), uby = lane_f(by, l), urz = lane_f(rz, l);
             const int up0 = lane_i(p0, l), ulen = lane_i(len, l);
             // A box narrower than one pixel (2 rz < 1) holds at most one
@@ -693,9 +822,20 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         }
                     }
                 };
+                const bool ulisted = (lmask >> l) & 1ull;
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
-                    const int pa = position(up0 + ia), pb = position(up0 + ib);
+                    int pa, pb;
+                    if (ulisted && i0 == 0) {
+                        pa = ca;
+                        pb = cb;
+                    } else if (ulisted) {
+                        pa = ia < ulen ? (int)a.cl_list[up0 + ia] : 0;
+                        pb = ib < ulen ? (int)a.cl_list[up0 + ib] : 0;
+                    } else {
+                        pa = position(up0 + ia);
+                        pb = position(up0 + ib);
+                    }
                     const float4 XOa = sxo[pa], XOb = sxo[pb];
                     const int ka = sk[pa], kb = sk[pb];
                     // all six reads issue together, ahead of the tests (an

@@ -172,3 +172,72 @@ def test_narrow_grain_passes_only_the_nearest_pixel(z, out_side):
                 hits += int(inside.sum())
                 checked += N
     assert hits > 0
+
+
+def splat_list_members(fx, fy, k, qa, qb, q, eps):
+    """Membership of samples in the splat candidate list (class k, cell
+    (qa, qb)), in the f32 operation order of splat_lists_kernel in
+    fg_pixelwise_hbm.hip."""
+    m = f32(2.0 ** -12)
+    R = f32(f32(k + 1) / f32(2 * q))
+    w = f32(f32(f32(R + R) + f32(eps + eps)) + f32(f32(f32(1) / f32(q)) + f32(m + m)))
+    lx = f32(f32(f32(f32(qa) / f32(q)) - R) - f32(eps + m))
+    ly = f32(f32(f32(f32(qb) / f32(q)) - R) - f32(eps + m))
+    dx = (fx - lx).astype(f32)
+    dy = (fy - ly).astype(f32)
+    dx = (dx - np.floor(dx)).astype(f32)
+    dy = (dy - np.floor(dy)).astype(f32)
+    return (dx <= w) & (dy <= w)
+
+
+def splat_list_span(k, q, e):
+    return min(q, k + 4 + e)
+
+
+@pytest.mark.parametrize("N,z,sd,out_side", [(100, 2.0, 0.4, 3840), (256, 2.0, 0.8, 3840), (513, 0.7, 1.5, 1024),
+                                              (2048, 3.0, 0.8, 4096)])
+def test_splat_list_contains_every_candidate(N, z, sd, out_side):
+    """The splat's per-(radius class, cell) candidate lists: every sample
+    whose f32 candidate box (cxk -+ rz, cxk = (cx z - 0.5) + ox z) has a
+    column and a row is in the list of the grain's class floor(rz * 64) and
+    of the cell of its centre fractions (frac(-bx), frac(-by)); and each
+    class's lists fit the room splat_list_span gives them."""
+    rng = np.random.default_rng(N + out_side)
+    q = 32
+    zf = f32(z)
+    o = rng.normal(0, sd, size=(N, 2)).astype(f32)
+    X = (o * zf).astype(f32)
+    fx = (X[:, 0] - np.floor(X[:, 0])).astype(f32)
+    fy = (X[:, 1] - np.floor(X[:, 1])).astype(f32)
+    in_side = out_side / z
+    c_in = in_side + 0.4 + float(np.abs(o).max()) + 4.0
+    c_out = (c_in + float(np.abs(o).max()) + 0.4) * z + 8.0
+    eps = f32(c_out * 2.0 ** -20 + 2.0 ** -16)
+    e = int(np.ceil((2.0 * float(eps) + 2.0 ** -11) * q + 0.01))
+    slack = f32(f32(4e-3) + f32(f32(2e-6) * f32(out_side)))
+    M = 2500
+    g = rng.uniform(-2, in_side + 2, size=(M, 2)).astype(f32)
+    h = M // 2  # targets of sample 0 near half-integers
+    t = (np.floor(g[:h] * zf) + f32(0.5) + rng.uniform(-1e-4, 1e-4, size=(h, 2))).astype(f32)
+    g[:h] = ((t - X[0] + f32(0.5)) / zf).astype(f32)
+    rad = (rng.uniform(0.0, 1.0, M) * (0.5 - float(slack)) / z).astype(f32)
+    for i in range(M):
+        bx = f32(f32(g[i, 0] * zf) - f32(0.5))
+        by = f32(f32(g[i, 1] * zf) - f32(0.5))
+        rz = f32(f32(rad[i] * zf) + slack)
+        if not rz + rz < f32(1):
+            continue
+        cx = (bx + X[:, 0]).astype(f32)
+        cy = (by + X[:, 1]).astype(f32)
+        col = np.ceil((cx - rz).astype(f32)) <= np.floor((cx + rz).astype(f32))
+        row = np.ceil((cy - rz).astype(f32)) <= np.floor((cy + rz).astype(f32))
+        k = int(f32(rz * f32(2 * q)))
+        ncx, ncy = f32(-bx), f32(-by)
+        ax, ay = f32(ncx - np.floor(ncx)), f32(ncy - np.floor(ncy))
+        qa, qb = min(int(f32(ax * f32(q))), q - 1), min(int(f32(ay * f32(q))), q - 1)
+        sel = splat_list_members(fx, fy, k, qa, qb, q, eps)
+        assert not np.any(col & row & ~sel), (N, z, i, float(rz))
+    # the room per class: no list set exceeds n * span^2 entries
+    for k in range(32):
+        total = sum(int(splat_list_members(fx, fy, k, c % q, c // q, q, eps).sum()) for c in range(q * q))
+        assert total <= N * splat_list_span(k, q, e) ** 2, (k, total)
