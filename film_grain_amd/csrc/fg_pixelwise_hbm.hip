@@ -680,7 +680,14 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         }
         float4 G = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const bool in = q < re_;
-        if (in) G = grains[q];
+        // grain records stream through once per tile (the halo re-reads of
+        // other tiles come much later): non-temporal, so that they do not
+        // push the candidate lists out of L2
+        if (in) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(grains + q));
+            G = make_float4(v.x, v.y, v.z, v.w);
+        }
         const uint32_t pk = __float_as_uint(G.w);
         const int ci = (int)(pk & 0xFFFFu), cj = (int)(pk >> 16);
         const float rad = G.z;
