@@ -536,7 +536,7 @@ __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_re
 // persistent batch kernel (which therefore give identical planes).  lds is the
 // workgroup's dynamic LDS (a.lds_bytes), s_queue its per-wave queues.  Starts
 // with a barrier, so a workgroup may call it for tile after tile.
-template <int BLOCK>
+template <int BLOCK, bool LISTS>
 __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st_all,
                                            const uint32_t *bin_start, const float4 *grains, int tile,
                                            unsigned char *lds, uint32_t (*s_queue)[kQueueCap])
@@ -722,7 +722,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         // a narrow grain that takes the direct test and has a radius class:
         // its cell's candidate list replaces the run
         bool listed = false;
-        if (keep && a.cl_q > 0 && rad <= safe_r && rz + rz < 1.0f) {
+        if (LISTS && keep && a.cl_q > 0 && rad <= safe_r && rz + rz < 1.0f) {
             const int k = (int)(rz * a.cl_inv_dz);
             if (k < a.cl_classes) {
                 const float ncx = -bx, ncy = -by;
@@ -745,7 +745,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         // the current one is tested
         int na = 0, nb = 0;
         auto prefetch = [&](int l) {
-            if ((lmask >> l) & 1ull) {
+            if (LISTS && ((lmask >> l) & 1ull)) {
                 const int q0 = lane_i(p0, l), qn = lane_i(len, l);
                 na = lane < qn ? (int)a.cl_list[q0 + lane] : 0;
                 nb = lane + 64 < qn ? (int)a.cl_list[q0 + 64 + lane] : 0;
@@ -829,7 +829,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         }
                     }
                 };
-                const bool ulisted = (lmask >> l) & 1ull;
+                const bool ulisted = LISTS && ((lmask >> l) & 1ull);
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
                     int pa, pb;
@@ -898,7 +898,7 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
     const int tile = a.tile_list ? a.tile_list[blockIdx.x] : (int)blockIdx.x;
-    splat_tile<BLOCK>(a, geo, st, bin_start, grains, tile, lds, s_queue);
+    splat_tile<BLOCK, true>(a, geo, st, bin_start, grains, tile, lds, s_queue);
 }
 
 // Persistent batch kernel: every workgroup takes (plane, tile) items from the
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
         // (generic loads kept them in VGPRs: 91 VGPRs against 68).
         const auto *cp = (const __attribute__((address_space(4))) BatchPlane *)planes + pl;
         const BatchPlane &p = *(const BatchPlane *)cp;
-        splat_tile<BLOCK>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
+        splat_tile<BLOCK, false>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
     }
 }
 
