@@ -536,13 +536,57 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
+    // Candidate lists for the direct test, one set per sample table of more
+    // than 64 samples (at most 64 the run is one step anyway): classes of
+    // 1/64 output pixel up to the narrow limit rz < 0.5 and the plane's
+    // largest rz, 32 x 32 cells.
+    const int cq = 32;
+    const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
+    const int classes = std::max(1, std::min(32, (int)std::ceil(std::min(a.rm * a.zoom + slack, 0.5f) * 64.0f)));
+    const int ew = (int)std::ceil((2.0 * (double)st.eps + std::ldexp(1.0, -11)) * cq + 0.01);
+    auto list_bytes = [&](int n) -> size_t {
+        if (n <= 64 || n > 2048) return 0;
+        const size_t b = 256 + (size_t)classes * cq * cq * 8 + fg::splat_lists_entries(cq, classes, ew, n) * 2;
+        return (b + 255) & ~(size_t)255;
+    };
+    std::vector<int> &ln = ctx->level_n;
+    ln.clear();
+    if (a.budget) {
+        for (int l = 16; l < a.n_samples; l *= 2) ln.push_back(l);
+    }
+    ln.push_back(a.n_samples);
+    size_t lists_total = 0;
+    for (int l : ln) lists_total += list_bytes(l);
+    if (lists_total > 0 && lists_total < ((size_t)1 << 32)) {
+        if ((rc = ensure_bytes(ctx, ctx->cells, lists_total, what)) != FG_OK) return rc;
+        a.cl_q = cq;
+        a.cl_classes = classes;
+        a.cl_inv_dz = (float)(2 * cq);
+    }
+    unsigned char *lists_at = static_cast<unsigned char *>(ctx->cells.p);
+    // builds table t's lists at lists_at (fill counters | cells | entries)
+    auto build_lists = [&](fg::SampleTable &t) -> hipError_t {
+        const size_t b = list_bytes(t.n);
+        if (a.cl_q == 0 || b == 0) return hipSuccess;
+        uint32_t *fill = reinterpret_cast<uint32_t *>(lists_at);
+        uint2 *cs = reinterpret_cast<uint2 *>(lists_at + 256);
+        uint16_t *cl = reinterpret_cast<uint16_t *>(cs + (size_t)classes * cq * cq);
+        t.cl_cell = cs;
+        t.cl_list = cl;
+        lists_at += b;
+        return fg::launch_splat_lists(t, cq, classes, ew, fill, cs, cl, a.status, s);
+    };
+    if (e == hipSuccess && !a.budget) {
+        // the splat kernel reads its one table from device memory
+        if ((rc = ensure_bytes(ctx, ctx->levels, 256, what)) != FG_OK) return rc;
+        fg::SampleTable *d_table = static_cast<fg::SampleTable *>(ctx->levels.p);
+        e = build_lists(st);
+        if (e == hipSuccess) e = fg::launch_store_table(st, d_table, s);
+        a.level_tables = d_table;
+    }
     if (e == hipSuccess && a.budget) {
         // adaptive sampling: one sample table per level (the first 16, 32,
         // ... samples, then all), and each tile's level
-        std::vector<int> &ln = ctx->level_n;
-        ln.clear();
-        for (int l = 16; l < a.n_samples; l *= 2) ln.push_back(l);
-        ln.push_back(a.n_samples);
         const int nl = (int)ln.size();
         size_t bytes = 0;
         for (int l : ln) bytes += (fg::sample_table_bytes(l, sample_buckets((uint32_t)l)) + 255) & ~(size_t)255;
@@ -558,6 +602,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
             e = fg::launch_sample_table(a.offs, ln[i], a.zoom, sample_buckets((uint32_t)ln[i]), p, &lt, s);
             lt.eps = st.eps;  // bounds of the whole offset set hold for every prefix
             lt.safe_r = st.safe_r;
+            if (e == hipSuccess) e = build_lists(lt);
             p += (fg::sample_table_bytes(ln[i], sample_buckets((uint32_t)ln[i])) + 255) & ~(size_t)255;
         }
         uint8_t *d_tile_level = p;
@@ -570,32 +615,6 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
         if (e == hipSuccess) e = fg::launch_tile_levels(a, d_level_n, nl, d_tile_level, s);
         // the host arrays above must outlive the copies
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-    }
-    if (e == hipSuccess && !a.budget && a.n_samples > 64 && a.n_samples <= 2048) {
-        // candidate lists for the direct test: classes of 1/64 output pixel
-        // up to the narrow limit rz < 0.5 and the plane's largest rz
-        const int cq = 32;
-        const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
-        const float rz_max = std::min(a.rm * a.zoom + slack, 0.5f);
-        const int classes = std::max(1, std::min(32, (int)std::ceil(rz_max * 64.0f)));
-        const int ew = (int)std::ceil((2.0 * (double)st.eps + std::ldexp(1.0, -11)) * cq + 0.01);
-        const size_t ncell = (size_t)classes * cq * cq;
-        const size_t entries = fg::splat_lists_entries(cq, classes, ew, a.n_samples);
-        if (entries < ((size_t)1 << 31)) {
-            const size_t head = 256;  // per-class fill counters
-            if ((rc = ensure_bytes(ctx, ctx->cells, head + ncell * 8 + entries * 2 + 64, what)) != FG_OK)
-                return rc;
-            unsigned char *base = static_cast<unsigned char *>(ctx->cells.p);
-            uint32_t *fill = reinterpret_cast<uint32_t *>(base);
-            uint2 *cs = reinterpret_cast<uint2 *>(base + head);
-            uint16_t *cl = reinterpret_cast<uint16_t *>(cs + ncell);
-            e = fg::launch_splat_lists(st, cq, classes, ew, fill, cs, cl, a.status, s);
-            a.cl_cell = cs;
-            a.cl_list = cl;
-            a.cl_q = cq;
-            a.cl_classes = classes;
-            a.cl_inv_dz = (float)(2 * cq);
-        }
     }
     if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a, g, start, order.w, order.ids, s);
     if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);

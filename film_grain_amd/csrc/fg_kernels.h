@@ -46,17 +46,16 @@ struct PixelArgs {
     // n in [1, n_samples] (the first n offsets, as a render with N = n uses),
     // per tile the index of the sample table covering its largest count, and
     // those tables (struct SampleTable, one per level).  budget == nullptr:
-    // every pixel uses all n_samples.
+    // every pixel uses all n_samples, and level_tables holds the one table.
     const uint16_t *budget;
     const uint8_t *tile_level;
     const void *level_tables;
     // HBM splat, optional (cl_q == 0: off): candidate lists of the sorted
     // sample order per (radius class, cell of the grain's centre fractions),
-    // for narrow grains that take the direct disk test.  Class k holds grains
+    // for narrow grains that take the direct disk test; the lists of each
+    // sample table are SampleTable::cl_cell / cl_list.  Class k holds grains
     // with rz < (k + 1) / cl_inv_dz; list (k, c) is cl_cell[k * cl_q^2 + c]
     // = (start, length) in cl_list.
-    const uint2 *cl_cell;
-    const uint16_t *cl_list;
     int cl_q, cl_classes;
     float cl_inv_dz;
 };

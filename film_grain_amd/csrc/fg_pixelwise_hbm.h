@@ -32,6 +32,10 @@ struct SampleTable {
     float eps;             // widening of the fraction interval (covers f32 rounding of the box)
     float safe_r;          // a grain of radius <= safe_r that passes the disk test of a point
                            // lies in that point's cell window
+    // the splat's candidate lists over this order (launch_splat_lists), or
+    // nullptr: (start, length) per (radius class, cell), and the entries
+    const uint2 *cl_cell;
+    const uint16_t *cl_list;
 };
 
 // Bytes of the device buffer behind a SampleTable for n samples.
@@ -46,6 +50,9 @@ hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbucke
 // classes * q * q (start, length) pairs, fill classes counters (cleared here).
 // A class beyond its room sets kStatusCapacity in *status.
 size_t splat_lists_entries(int q, int classes, int e, int n);
+// Writes t to device memory at dst (the single-plane splat reads its table
+// from PixelArgs::level_tables).
+hipError_t launch_store_table(const SampleTable &t, SampleTable *dst, hipStream_t s);
 hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
                               uint16_t *list, int *status, hipStream_t s);
 

@@ -438,6 +438,17 @@ __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q,
     }
 }
 
+__global__ void store_table_kernel(SampleTable t, SampleTable *dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = t;
+}
+
+hipError_t launch_store_table(const SampleTable &t, SampleTable *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(store_table_kernel, dim3(1), dim3(64), 0, s, t, dst);
+    return hipGetLastError();
+}
+
 size_t splat_lists_entries(int q, int classes, int e, int n)
 {
     return splat_list_base(classes, q, e, n);
@@ -563,8 +574,19 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
     const int fast = a.div_fast;
     // adaptive sampling: the tile's level table holds its first N samples
-    const SampleTable st =
-        a.budget ? reinterpret_cast<const SampleTable *>(a.level_tables)[a.tile_level[tile]] : st_all;
+    // The single-plane kernel (LISTS) reads its table from a.level_tables:
+    // the tile's level's table, or the one table of a fixed-N render.  It is
+    // read through the constant address space (the tile is uniform, so the
+    // level and the table's fields become scalar loads, as kernel arguments)
+    // and never selected at run time against st_all, which would make the
+    // compiler keep a copy of the walk for each.  The batch kernel passes its
+    // plane's table as st_all.
+    const SampleTable *lt = &st_all;
+    if (LISTS) {
+        const int lvl = a.budget ? ((const __attribute__((address_space(4))) uint8_t *)a.tile_level)[tile] : 0;
+        lt = (const SampleTable *)((const __attribute__((address_space(4))) SampleTable *)a.level_tables + lvl);
+    }
+    const SampleTable st = *lt;  // a copy: the walk's LDS atomics cannot make it reload
     const int N = st.n, words = a.words, Q = st.qbuckets;
     const TileRegion reg = tile_region(a, geo, t.X0, t.Y0, t.X1, t.Y1);
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
@@ -722,14 +744,14 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         // a narrow grain that takes the direct test and has a radius class:
         // its cell's candidate list replaces the run
         bool listed = false;
-        if (LISTS && keep && a.cl_q > 0 && rad <= safe_r && rz + rz < 1.0f) {
+        if (LISTS && keep && st.cl_cell && rad <= safe_r && rz + rz < 1.0f) {
             const int k = (int)(rz * a.cl_inv_dz);
             if (k < a.cl_classes) {
                 const float ncx = -bx, ncy = -by;
                 const float fx = ncx - floorf(ncx), fy = ncy - floorf(ncy);
                 const float qf = (float)a.cl_q;
                 const int qa = min((int)(fx * qf), a.cl_q - 1), qb = min((int)(fy * qf), a.cl_q - 1);
-                const uint2 se = a.cl_cell[(size_t)k * (a.cl_q * a.cl_q) + qb * a.cl_q + qa];
+                const uint2 se = st.cl_cell[(size_t)k * (a.cl_q * a.cl_q) + qb * a.cl_q + qa];
                 p0 = (int)se.x;
                 len = (int)se.y;
                 listed = true;
@@ -747,8 +769,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         auto prefetch = [&](int l) {
             if (LISTS && ((lmask >> l) & 1ull)) {
                 const int q0 = lane_i(p0, l), qn = lane_i(len, l);
-                na = lane < qn ? (int)a.cl_list[q0 + lane] : 0;
-                nb = lane + 64 < qn ? (int)a.cl_list[q0 + 64 + lane] : 0;
+                na = lane < qn ? (int)st.cl_list[q0 + lane] : 0;
+                nb = lane + 64 < qn ? (int)st.cl_list[q0 + 64 + lane] : 0;
             }
         };
         if (m) prefetch(__ffsll((long long)m) - 1);
@@ -837,8 +859,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         pa = ca;
                         pb = cb;
                     } else if (ulisted) {
-                        pa = ia < ulen ? (int)a.cl_list[up0 + ia] : 0;
-                        pb = ib < ulen ? (int)a.cl_list[up0 + ib] : 0;
+                        pa = ia < ulen ? (int)st.cl_list[up0 + ia] : 0;
+                        pb = ib < ulen ? (int)st.cl_list[up0 + ib] : 0;
                     } else {
                         pa = position(up0 + ia);
                         pb = position(up0 + ib);
