@@ -368,8 +368,9 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
 // 1 / (2 q) a window is (k + 2 + (2 eps + 2^-11) q) / q wide, so a sample
 // lies in at most splat_list_span(k, q, e) cells per axis when e >=
 // (2 eps + 2^-11) q, and class k needs at most n * span^2 list entries.
-// Blocks of 256 cells of one class; both passes test every sample, and each
-// block reserves its cells' room in the class's region with one atomic.
+// Blocks of 64 cells of one class, four threads per cell; both passes test
+// every sample, and each block reserves its cells' room in the class's
+// region with one atomic.
 __host__ __device__ inline uint32_t splat_list_span(int k, int q, int e)
 {
     return (uint32_t)min(q, k + 4 + e);
@@ -385,11 +386,17 @@ __host__ __device__ inline size_t splat_list_base(int k, int q, int e, int n)
 __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q, int e, uint32_t *fill, uint2 *cell,
                                                           uint16_t *list, int *status)
 {
+    // 64 cells per block, each split over 4 adjacent threads that take the
+    // four quarters of the sample order: the scan over the block in thread
+    // order keeps every list in sorted order
     __shared__ float2 s_f[2048];
     __shared__ uint32_t s_scan[256];
     __shared__ uint32_t s_base;
-    const int cells = q * q, parts = (cells + 255) / 256, N = st.n;
-    const int k = blockIdx.x / parts, c = (blockIdx.x - k * parts) * 256 + (int)threadIdx.x;
+    __shared__ bool s_fits;
+    const int cells = q * q, parts = (cells + 63) / 64, N = st.n;
+    const int k = blockIdx.x / parts, c = (blockIdx.x - k * parts) * 64 + ((int)threadIdx.x >> 2);
+    const int quarter = threadIdx.x & 3, qn = (N + 3) / 4;
+    const int p_lo = quarter * qn, p_hi = min(N, p_lo + qn);
     for (int p = threadIdx.x; p < N; p += 256) {
         const float2 X = st.xy[p];
         s_f[p] = make_float2(X.x - floorf(X.x), X.y - floorf(X.y));
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q,
     };
     uint32_t n = 0;
     if (c < cells)
-        for (int p = 0; p < N; ++p) n += member(p) ? 1u : 0u;
+        for (int p = p_lo; p < p_hi; ++p) n += member(p) ? 1u : 0u;
     // the block's cells take one contiguous range of the class's room
     s_scan[threadIdx.x] = n;
     __syncthreads();
@@ -419,7 +426,6 @@ __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q,
         s_scan[threadIdx.x] += v;
         __syncthreads();
     }
-    __shared__ bool s_fits;
     if (threadIdx.x == 255) {
         const uint32_t span = splat_list_span(k, q, e);
         const uint32_t off = atomicAdd(&fill[k], s_scan[255]);
@@ -432,8 +438,11 @@ __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q,
     __syncthreads();
     if (c < cells && s_fits) {
         uint32_t pos = s_base + s_scan[threadIdx.x] - n;
-        cell[(size_t)k * cells + c] = make_uint2(pos, n);
-        for (int p = 0; p < N; ++p)
+        if (quarter == 0) {
+            const uint32_t len = s_scan[threadIdx.x + 3] - (s_scan[threadIdx.x] - n);
+            cell[(size_t)k * cells + c] = make_uint2(pos, len);
+        }
+        for (int p = p_lo; p < p_hi; ++p)
             if (member(p)) list[pos++] = (uint16_t)p;
     }
 }
@@ -460,7 +469,7 @@ hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, 
     if (q < 1 || q * q > 65536 || classes < 1 || st.n > 2048) return hipErrorInvalidValue;
     hipError_t err = hipMemsetAsync(fill, 0, sizeof(uint32_t) * classes, s);
     if (err != hipSuccess) return err;
-    const int parts = (q * q + 255) / 256;
+    const int parts = (q * q + 63) / 64;
     hipLaunchKernelGGL(splat_lists_kernel, dim3(classes * parts), dim3(256), 0, s, st, q, e, fill, cell, list,
                        status);
     return hipGetLastError();
