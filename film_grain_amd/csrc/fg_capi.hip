@@ -599,9 +599,13 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
         ctx->level_tables.assign(nl, fg::SampleTable{});
         for (int i = 0; i < nl && e == hipSuccess; ++i) {
             fg::SampleTable &lt = ctx->level_tables[i];
-            e = fg::launch_sample_table(a.offs, ln[i], a.zoom, sample_buckets((uint32_t)ln[i]), p, &lt, s);
-            lt.eps = st.eps;  // bounds of the whole offset set hold for every prefix
-            lt.safe_r = st.safe_r;
+            if (ln[i] == a.n_samples) {
+                lt = st;  // the last level is the full table
+            } else {
+                e = fg::launch_sample_prefix(st, ln[i], sample_buckets((uint32_t)ln[i]), p, &lt, s);
+                lt.eps = st.eps;  // bounds of the whole offset set hold for every prefix
+                lt.safe_r = st.safe_r;
+            }
             if (e == hipSuccess) e = build_lists(lt);
             p += (fg::sample_table_bytes(ln[i], sample_buckets((uint32_t)ln[i])) + 255) & ~(size_t)255;
         }

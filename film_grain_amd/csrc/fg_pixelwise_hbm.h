@@ -43,6 +43,10 @@ size_t sample_table_bytes(int n, int qbuckets);
 // Builds the table in `buf` (sample_table_bytes bytes) and points *t at it.
 hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
                                hipStream_t s);
+// The table of the first n samples of `full` (full.n <= 2048), built by
+// filtering full's order instead of sorting again.
+hipError_t launch_sample_prefix(const SampleTable &full, int n, int qbuckets, void *buf, SampleTable *t,
+                                hipStream_t s);
 
 // Candidate lists of the HBM splat (PixelArgs::cl_*): q * q cells per class,
 // class k of radius (k + 1) / (2 q).  e is an integer >= (2 st.eps + 2^-11)

@@ -315,6 +315,37 @@ __global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable
     if (threadIdx.x == 0) *const_cast<float4 *>(t.ext) = s_r[0];
 }
 
+// The table of the first n samples from the table of all of them: the full
+// order is sorted by (fraction, index), so keeping the entries with index < n
+// in that order sorts the prefix by the same key.  One workgroup of 1024
+// threads, full.n <= 2048.
+__global__ __launch_bounds__(1024) void sample_prefix_kernel(SampleTable full, int n, SampleTable t)
+{
+    __shared__ uint32_t s_scan[1024];
+    const int i0 = 2 * (int)threadIdx.x, N = full.n;
+    const uint32_t f0 = (i0 < N && full.k[i0] < n) ? 1u : 0u;
+    const uint32_t f1 = (i0 + 1 < N && full.k[i0 + 1] < n) ? 1u : 0u;
+    s_scan[threadIdx.x] = f0 + f1;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = (int)threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t pos = s_scan[threadIdx.x] - (f0 + f1);
+    for (int j = 0; j < 2; ++j) {
+        const int i = i0 + j;
+        if (j == 0 ? f0 : f1) {
+            const_cast<float2 *>(t.xy)[pos] = full.xy[i];
+            const_cast<float2 *>(t.o)[pos] = full.o[i];
+            const_cast<uint16_t *>(t.k)[pos] = full.k[i];
+            const_cast<float *>(t.key)[pos] = full.key[i];
+            ++pos;
+        }
+    }
+}
+
 __host__ __device__ static inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 size_t sample_table_bytes(int n, int qbuckets)
@@ -329,6 +360,18 @@ hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbucke
     sample_table_layout(buf, n, qbuckets, t);
     hipLaunchKernelGGL(sample_sort_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, offs,
                        n, zoom, *t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sample_index_kernel, dim3(1), dim3(kBlock), 0, s, n, *t);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_prefix(const SampleTable &full, int n, int qbuckets, void *buf, SampleTable *t,
+                                hipStream_t s)
+{
+    if (full.n > 2048 || n > full.n) return hipErrorInvalidValue;
+    sample_table_layout(buf, n, qbuckets, t);
+    hipLaunchKernelGGL(sample_prefix_kernel, dim3(1), dim3(1024), 0, s, full, n, *t);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sample_index_kernel, dim3(1), dim3(kBlock), 0, s, n, *t);
