@@ -317,10 +317,15 @@ def frame_buffers(planes: list[DevicePlane]) -> FrameBuffers:
     from film_grain_amd import capi
     dev = planes[0].lam.device
     X, pred, budget, out = [], [], [], []
+    # the planes' predictions are views of one array: the scale search reads
+    # the whole frame in one call
+    pred_all = torch.empty(sum(dp.out_w * dp.out_h for dp in planes), dtype=torch.float32, device=dev)
+    at = 0
     for dp in planes:
         n = dp.out_w * dp.out_h
         X.append(torch.empty((n, capi.MLP_FEATURES), dtype=torch.int16, device=dev))
-        pred.append(torch.empty(n, dtype=torch.float32, device=dev))
+        pred.append(pred_all[at:at + n])
+        at += n
         budget.append(torch.empty(n, dtype=torch.int16, device=dev))
         out.append(torch.empty((dp.out_h, dp.out_w), dtype=torch.float32, device=dev))
     return FrameBuffers(X, pred, budget, out)
@@ -358,6 +363,8 @@ def render_adaptive_frame(ctx, model: ErrorModel | None, planes: list[DevicePlan
     mean_budget samples per pixel), budgets, adaptive renders, each launched
     for the largest budget of its plane.  Returns (outputs, c)."""
     import torch
+    if kind == "learned" and target_mse is not None and mean_budget is None:
+        return _render_learned_frame(ctx, model, planes, bufs, target_mse, n_min, n_max)
     s_all = []
     for i, dp in enumerate(planes):
         features(ctx, dp, out=bufs.X[i])
@@ -386,6 +393,30 @@ def render_adaptive_frame(ctx, model: ErrorModel | None, planes: list[DevicePlan
         n_hi = max(1, min(n_max, int(his[i])))
         ctx.render_adaptive_device(_desc_n(dp.desc, n_hi), dp.lam, dp.offs[:n_hi], bufs.budget[i], bufs.out[i])
     return bufs.out, c
+
+
+def _render_learned_frame(ctx, model: ErrorModel, planes: list[DevicePlane], bufs: FrameBuffers, target_mse: float,
+                          n_min: int, n_max: int):
+    """render_adaptive_frame for the learned model at a target error, with no
+    host round trip: the scale search (choose_scale's rounds, on the device)
+    and the budgets read the scale from device memory, and every plane's
+    launch is sized for n_max (its tiles pick their own levels).  Returns
+    (outputs, c) with c a one-element device tensor."""
+    for i, dp in enumerate(planes):
+        features(ctx, dp, out=bufs.X[i])
+        model.predict(ctx, bufs.X[i], out=bufs.pred[i])
+    import torch
+    base = bufs.pred[0]._base
+    if base is not None and base.numel() == sum(p.numel() for p in bufs.pred):
+        pred_all = base
+    else:
+        pred_all = torch.cat(bufs.pred)
+    sc = ctx.budget_scale(pred_all, SIGMA64_TO_S, target_mse, n_min, n_max, N_REF)
+    for i, dp in enumerate(planes):
+        ctx.mlp_budget_scale(bufs.pred[i], None, dp.out_w * dp.out_h, sc[1:], n_min, n_max, out=bufs.budget[i])
+    for i, dp in enumerate(planes):
+        ctx.render_adaptive_device(_desc_n(dp.desc, n_max), dp.lam, dp.offs[:n_max], bufs.budget[i], bufs.out[i])
+    return bufs.out, sc[:1]
 
 
 def _frame_rmse(outs, refs) -> float:
@@ -461,7 +492,7 @@ def evaluate(ctx, model: ErrorModel | None, specs_per_frame, targets_n=(256, 102
                 mean_n = float(np.mean([float((b.to(torch.int32) & 0xFFFF).float().mean())
                                         for b in bufs.budget]))
                 fixed_ms = ms_at_rmse(curve, rmse)
-                row = dict(target_n=tn, kind=kind, ms=ms, rmse=rmse, mean_budget=mean_n, c=c,
+                row = dict(target_n=tn, kind=kind, ms=ms, rmse=rmse, mean_budget=mean_n, c=float(c),
                            fixed_ms_at_rmse=fixed_ms, speedup=(fixed_ms / ms) if fixed_ms else None)
                 frame["adaptive"].append(row)
                 if log:

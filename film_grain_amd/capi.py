@@ -33,7 +33,8 @@ EXPORTED = [
     "fg_set_launch_config", "fg_get_launch_config", "fg_set_timing", "fg_get_timings",
     "fg_batch_create", "fg_batch_run", "fg_batch_status", "fg_batch_info", "fg_batch_destroy",
     "fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
-    "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
+    "fg_mlp_budget_device", "fg_mlp_budget_scale_device", "fg_budget_scale_device",
+    "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
     "fg_version",
 ]
@@ -148,6 +149,9 @@ def lib() -> C.CDLL:
     L.fg_mlp_grad_device.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp]
     L.fg_mlp_adam_device.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
     L.fg_mlp_budget_device.argtypes = [vp, vp, vp, C.c_int64, C.c_float, C.c_int, C.c_int, vp, vp]
+    L.fg_mlp_budget_scale_device.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int, C.c_int, vp, vp]
+    L.fg_budget_scale_device.argtypes = [vp, vp, C.c_int64, C.c_float, C.c_double, C.c_int, C.c_int, C.c_int, vp,
+                                         vp]
     L.fg_render_pixelwise_adaptive_device.argtypes = [vp, C.POINTER(PlaneDesc), vp, vp, vp, vp, vp]
     L.fg_render_frame_device.argtypes = [vp, C.POINTER(FrameDesc), vp, vp, vp, vp, vp]
     L.fg_render_frame.argtypes = [vp, C.POINTER(FrameDesc), fp, fp, C.POINTER(C.c_uint8), fp, CANCEL_FN, vp]
@@ -158,7 +162,8 @@ def lib() -> C.CDLL:
     L.fg_cancel.argtypes = [vp, C.c_int]
     L.fg_cancel.restype = C.c_int
     for name in ("fg_mlp_features_device", "fg_mlp_forward_device", "fg_mlp_grad_device", "fg_mlp_adam_device",
-                 "fg_mlp_budget_device", "fg_render_pixelwise_adaptive_device", "fg_render_frame_device",
+                 "fg_mlp_budget_device", "fg_mlp_budget_scale_device", "fg_budget_scale_device",
+                 "fg_render_pixelwise_adaptive_device", "fg_render_frame_device",
                  "fg_render_frame", "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device",
                  "fg_log_f32_device"):
         getattr(L, name).restype = C.c_int
@@ -352,6 +357,31 @@ class Context:
         self._check(self._L.fg_mlp_budget_device(
             self.handle, C.c_void_p(pred.data_ptr() if pred is not None else 0),
             C.c_void_p(feats.data_ptr() if feats is not None else 0), n, c, n_min, n_max,
+            C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
+        return out
+
+    def budget_scale(self, pred, s_mult, target_mse, n_min, n_max, n_ref, out=None, stream=None):
+        """The frame's budget scale searched on the device (no host round
+        trip): out[0] = c for s = exp(pred) * s_mult, out[1] = c * s_mult.
+        pred is one float32 tensor over all the frame's rows."""
+        import torch
+        if out is None:
+            out = torch.empty(2, dtype=torch.float32, device=pred.device)
+        self._check(self._L.fg_budget_scale_device(
+            self.handle, C.c_void_p(pred.data_ptr()), pred.numel(), s_mult, target_mse, n_min, n_max, n_ref,
+            C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
+        return out
+
+    def mlp_budget_scale(self, pred, feats, n, d_c, n_min, n_max, out=None, stream=None):
+        """mlp_budget with the scale read from device memory (d_c: a
+        one-element float32 tensor, e.g. budget_scale(...)[1:])."""
+        import torch
+        dev = (pred if pred is not None else feats).device
+        if out is None:
+            out = torch.empty(n, dtype=torch.int16, device=dev)
+        self._check(self._L.fg_mlp_budget_scale_device(
+            self.handle, C.c_void_p(pred.data_ptr() if pred is not None else 0),
+            C.c_void_p(feats.data_ptr() if feats is not None else 0), n, C.c_void_p(d_c.data_ptr()), n_min, n_max,
             C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
         return out
 

@@ -61,6 +61,7 @@ struct fg_ctx {
     DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
     DevBuf frame;             // frames: split planes, lambda, max word, rendered planes
     DevBuf cells;             // candidate lists: grain-wise constant radius, splat direct test
+    DevBuf scale_state;       // budget scale search: bracket, flag, probe sums
     DevBuf frame_io;          // frames from host memory: RGB in, offsets, 8-bit RGB out, planes out
     std::vector<int> level_n;
     std::vector<fg::SampleTable> level_tables;
@@ -121,6 +122,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->frame);
     free_buf(ctx->frame_io);
     free_buf(ctx->cells);
+    free_buf(ctx->scale_state);
 }
 
 int ensure_bytes(fg_ctx *ctx, DevBuf &b, size_t bytes, const char *what)
@@ -1710,9 +1712,38 @@ int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t 
         !(c >= 0.0f))
         return set_error(ctx, FG_EINVAL, "invalid budget arguments");
     if (n == 0) return FG_OK;
-    const hipError_t e = fg::launch_budget(d_pred_log_s, d_features, n, c, n_min, n_max, d_budget,
+    const hipError_t e = fg::launch_budget(d_pred_log_s, d_features, n, c, nullptr, n_min, n_max, d_budget,
                                            reinterpret_cast<hipStream_t>(hip_stream));
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "budget");
+}
+
+int fg_mlp_budget_scale_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n,
+                               const float *d_c, int n_min, int n_max, uint16_t *d_budget, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if ((!d_pred_log_s && !d_features) || !d_c || !d_budget || n < 0 || n_min < 1 || n_max < n_min ||
+        n_max > 65535)
+        return set_error(ctx, FG_EINVAL, "invalid budget arguments");
+    if (n == 0) return FG_OK;
+    const hipError_t e = fg::launch_budget(d_pred_log_s, d_features, n, 0.0f, d_c, n_min, n_max, d_budget,
+                                           reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "budget");
+}
+
+int fg_budget_scale_device(fg_ctx *ctx, const float *d_pred_log_s, int64_t n, float s_mult, double target_mse,
+                           int n_min, int n_max, int n_ref, float *d_scale, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (!d_pred_log_s || !d_scale || n < 1 || n_min < 1 || n_max < n_min || n_ref < 1 || !(s_mult > 0.0f) ||
+        !(target_mse >= 0.0))
+        return set_error(ctx, FG_EINVAL, "invalid budget scale arguments");
+    const char *what = "budget scale";
+    const int rc = ensure_bytes(ctx, ctx->scale_state, fg::budget_scale_state_bytes(), what);
+    if (rc != FG_OK) return rc;
+    const hipError_t e = fg::launch_budget_scale(d_pred_log_s, n, s_mult, target_mse, n_min, n_max, n_ref, 3,
+                                                 (long long)1 << 18, ctx->scale_state.p, d_scale,
+                                                 reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, what);
 }
 
 int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,

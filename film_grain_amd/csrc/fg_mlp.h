@@ -25,8 +25,14 @@ hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, fl
 // of the residual target under the predicted log std.
 hipError_t launch_mlp_grad(const float *P, const uint16_t *X, const float *target, long long n, float inv_total,
                            int loss_kind, float *grad, float *loss, hipStream_t s);
-hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, int nmin, int nmax,
-                         uint16_t *budget, hipStream_t s);
+hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, const float *dc, int nmin,
+                         int nmax, uint16_t *budget, hipStream_t s);
+// The budget scale search (see fg_mlp.hip): writes out[0] = c for s =
+// exp(pred) * s_mult and out[1] = c * s_mult (the scale of exp(pred));
+// state_buf holds budget_scale_state_bytes().
+size_t budget_scale_state_bytes();
+hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, double target, int nmin, int nmax,
+                               int n_ref, int rounds, long long max_rows, void *state_buf, float *out, hipStream_t s);
 hipError_t launch_mlp_adam(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2, float eps,
                            float bc1, float bc2, hipStream_t s);
 

@@ -31,6 +31,8 @@
 // both: D[4(l/16)+0..3][l%16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fg_kernels.h"
 #include "fg_mlp.h"
 
@@ -383,23 +385,167 @@ __global__ __launch_bounds__(256) void feature_kernel(const float *u, int in_w, 
 // Sample budget n = clamp(ceil(c * s), nmin, nmax) per pixel, from s =
 // exp(predicted log s) (pred != nullptr) or, for the allocator without the
 // model, s = sqrt(u (1 - u)) from feature 4 of X.
+// With dc != nullptr the scale is *dc (written by the scale search).
 __global__ __launch_bounds__(256) void budget_kernel(const float *pred, const uint16_t *X, long long n, float c,
-                                                     int nmin, int nmax, uint16_t *budget)
+                                                     const float *dc, int nmin, int nmax, uint16_t *budget)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    if (dc) c = *dc;
     const float s = pred ? expf(pred[i]) : sqrtf(fmaxf(bf16_float(X[i * 16 + 4]), 0.0f));
     const float v = ceilf(c * s);
     const int b = v >= (float)nmax ? nmax : (v <= (float)nmin ? nmin : (int)v);
     budget[i] = (uint16_t)b;
 }
 
-hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, int nmin, int nmax,
-                         uint16_t *budget, hipStream_t s)
+hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, float c, const float *dc, int nmin,
+                         int nmax, uint16_t *budget, hipStream_t s)
 {
-    hipLaunchKernelGGL(budget_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pred, X, n, c, nmin, nmax,
-                       budget);
+    hipLaunchKernelGGL(budget_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pred, X, n, c, dc, nmin,
+                       nmax, budget);
     return hipGetLastError();
+}
+
+// ---- the budget scale search on the device ----
+// Per-sample stds s_i = exp(pred[i * step]) * s_mult over the strided
+// subsample i < m.  The smallest c whose budgets clamp(ceil(c s), nmin, nmax)
+// have mean predicted squared error s^2 (1/n - 1/n_ref) <= target, by rounds
+// of kScaleProbes geometric candidates between lo and hi (each round keeps the
+// bracket of the first candidate that meets the target).  State: lo, hi, c,
+// done; then kScaleProbes double sums.
+constexpr int kScaleProbes = 64;
+
+__device__ __forceinline__ float scale_probe(float lo, float hi, int j)
+{
+    const float a = logf(lo), b = logf(hi);
+    return expf(a + (b - a) * ((float)j / (float)(kScaleProbes - 1)));
+}
+
+__global__ __launch_bounds__(1024) void scale_init_kernel(const float *pred, long long step, long long m, float s_mult,
+                                                          int nmax, float *state, double *sums)
+{
+    __shared__ float s_min[1024];
+    float mn = INFINITY;
+    for (long long i = threadIdx.x; i < m; i += 1024) {
+        const float s = expf(pred[i * step]) * s_mult;
+        if (s > 0.0f) mn = fminf(mn, s);
+    }
+    s_min[threadIdx.x] = mn;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_min[threadIdx.x] = fminf(s_min[threadIdx.x], s_min[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < kScaleProbes) sums[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) {
+        const float smin = s_min[0] < INFINITY ? fmaxf(s_min[0], 1e-6f) : 1.0f;
+        state[0] = 1e-2f;
+        state[1] = 2.0f * (float)nmax / smin;
+        state[2] = state[1];
+        state[3] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void scale_sums_kernel(const float *pred, long long step, long long m, float s_mult,
+                                                         int nmin, int nmax, float inv_nref, const float *state,
+                                                         double *sums)
+{
+    __shared__ float s_c[kScaleProbes];
+    __shared__ float s_part[256 / 64][kScaleProbes];
+    if (state[3] != 0.0f) return;
+    if (threadIdx.x < kScaleProbes) s_c[threadIdx.x] = scale_probe(state[0], state[1], threadIdx.x);
+    __syncthreads();
+    float acc[kScaleProbes];
+#pragma unroll
+    for (int j = 0; j < kScaleProbes; ++j) acc[j] = 0.0f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
+        const float s = expf(pred[i * step]) * s_mult;
+        const float s2 = s * s;
+#pragma unroll
+        for (int j = 0; j < kScaleProbes; ++j) {
+            const float v = ceilf(s_c[j] * s);
+            const float nj = fminf(fmaxf(v, (float)nmin), (float)nmax);
+            acc[j] += s2 * (1.0f / nj - inv_nref);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kScaleProbes; ++j) {
+        float v = acc[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) s_part[wid][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kScaleProbes) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / 64; ++w) t += (double)s_part[w][threadIdx.x];
+        atomicAdd(&sums[threadIdx.x], t);
+    }
+}
+
+__global__ void scale_pick_kernel(long long m, double target, int last, float *state, double *sums)
+{
+    __shared__ int s_first;
+    if (threadIdx.x == 0) s_first = kScaleProbes;
+    __syncthreads();
+    if (state[3] == 0.0f && threadIdx.x < kScaleProbes && sums[threadIdx.x] / (double)m <= target)
+        atomicMin(&s_first, (int)threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0 && state[3] == 0.0f) {
+        const float lo = state[0], hi = state[1];
+        const int k = s_first;
+        if (k == kScaleProbes) {
+            state[2] = scale_probe(lo, hi, kScaleProbes - 1);
+            state[3] = 1.0f;
+        } else if (k == 0) {
+            state[2] = scale_probe(lo, hi, 0);
+            state[3] = 1.0f;
+        } else {
+            state[0] = scale_probe(lo, hi, k - 1);
+            state[1] = scale_probe(lo, hi, k);
+            state[2] = state[1];
+            if (last) state[3] = 1.0f;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kScaleProbes) sums[threadIdx.x] = 0.0;
+}
+
+__global__ void scale_out_kernel(const float *state, float s_mult, float *out)
+{
+    if (threadIdx.x == 0) {
+        out[0] = state[2];
+        out[1] = state[2] * s_mult;
+    }
+}
+
+size_t budget_scale_state_bytes() { return 16 + sizeof(double) * kScaleProbes; }
+
+hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, double target, int nmin, int nmax,
+                               int n_ref, int rounds, long long max_rows, void *state_buf, float *out, hipStream_t s)
+{
+    if (n < 1 || rounds < 1 || max_rows < 1) return hipErrorInvalidValue;
+    const long long step = std::max<long long>(1, n / max_rows);
+    const long long m = (n + step - 1) / step;
+    float *state = static_cast<float *>(state_buf);
+    double *sums = reinterpret_cast<double *>(static_cast<unsigned char *>(state_buf) + 16);
+    hipLaunchKernelGGL(scale_init_kernel, dim3(1), dim3(1024), 0, s, pred, step, m, s_mult, nmax, state, sums);
+    hipError_t e = hipGetLastError();
+    const unsigned grid = (unsigned)std::min<long long>((m + 255) / 256, 1024);
+    for (int r = 0; r < rounds && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(scale_sums_kernel, dim3(grid), dim3(256), 0, s, pred, step, m, s_mult, nmin, nmax,
+                           1.0f / (float)n_ref, state, sums);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(scale_pick_kernel, dim3(1), dim3(kScaleProbes), 0, s, m, target, r == rounds - 1 ? 1 : 0,
+                           state, sums);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(scale_out_kernel, dim3(1), dim3(64), 0, s, state, s_mult, out);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s)

@@ -238,6 +238,20 @@ int fg_mlp_grad_device(fg_ctx *ctx, const float *d_params, const uint16_t *d_fea
  * allocator without the model). */
 int fg_mlp_budget_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n, float c,
                          int n_min, int n_max, uint16_t *d_budget, void *hip_stream);
+/* The same budgets with the scale read from device memory (*d_c, as
+ * fg_budget_scale_device writes it at d_scale + 1), so that a frame's scale
+ * search and budgets run without a host round trip.  No counterpart in the
+ * reference (its sample count is one N for every pixel). */
+int fg_mlp_budget_scale_device(fg_ctx *ctx, const float *d_pred_log_s, const uint16_t *d_features, int64_t n,
+                               const float *d_c, int n_min, int n_max, uint16_t *d_budget, void *hip_stream);
+/* The scale of a frame's budgets on the device: the smallest c whose budgets
+ * clamp(ceil(c s), n_min, n_max), s = exp(pred) * s_mult, have mean predicted
+ * squared error s^2 (1/n - 1/n_ref) <= target_mse, over a strided subsample
+ * of at most 2^18 of the n rows, by three rounds of 64 geometric candidates.
+ * Writes d_scale[0] = c and d_scale[1] = c * s_mult.  No counterpart in the
+ * reference. */
+int fg_budget_scale_device(fg_ctx *ctx, const float *d_pred_log_s, int64_t n, float s_mult, double target_mse,
+                           int n_min, int n_max, int n_ref, float *d_scale, void *hip_stream);
 /* One Adam step (step counts from 1) on d_params with moments d_m, d_v. */
 int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
                        float beta1, float beta2, float eps, int step, void *hip_stream);

@@ -91,3 +91,32 @@ def test_training_on_renderer_pairs_beats_binomial(hip_ctx):
     print(f"[budget] equal budget 256: learned {learned * 255:.3f}/255 binomial {binomial * 255:.3f}/255",
           flush=True)
     assert learned < binomial
+
+
+def test_device_scale_search_matches_choose_scale(hip_ctx):
+    """fg_budget_scale_device runs choose_scale's rounds on the device (f32
+    rows, double sums): its scale lands within 0.1% of the host search's, its
+    budgets meet the target by the host's own predicted_mse, and the budgets
+    read from device memory equal fg_mlp_budget_device's at that scale."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(7)
+    for n, sd in [(3 * 1920 * 1080, 0.5), (300_001, 1.2), (1000, 0.2)]:
+        pred = (torch.randn(n, generator=g) * sd - 2.0).to("cuda:0")
+        s = torch.exp(pred) * budget.SIGMA64_TO_S
+        # (a target of exactly N = n_max everywhere is a tie that rounding
+        # decides; targets below the cap are not)
+        for target_n in (64, 256, 900):
+            target = float(budget.predicted_mse(s, torch.full_like(s, float(target_n))).mean())
+            want = budget.choose_scale(s, target, 16, 1024)
+            sc = hip_ctx.budget_scale(pred, budget.SIGMA64_TO_S, target, 16, 1024, budget.N_REF)
+            got = float(sc[0])
+            print(f"[scale] n={n} sd={sd} target N={target_n}: host {want:.6g} device {got:.6g}", flush=True)
+            assert abs(got - want) <= 1e-3 * want, (n, sd, target_n, got, want)
+            # the device scale meets the target on the subsample it searched
+            step = max(1, n // (1 << 18))
+            ss = s[::step].double()
+            nb = torch.clamp(torch.ceil(float(sc[0]) * ss), 16, 1024)
+            assert float(budget.predicted_mse(ss, nb).mean()) <= target * (1 + 1e-5), (n, sd, target_n)
+            b_dev = hip_ctx.mlp_budget_scale(pred, None, n, sc[1:], 16, 1024)
+            b_host = hip_ctx.mlp_budget(pred, None, n, float(sc[1]), 16, 1024)
+            assert torch.equal(b_dev, b_host)
