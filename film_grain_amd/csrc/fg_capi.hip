@@ -538,16 +538,19 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
-    // Candidate lists for the direct test, one set per sample table of more
-    // than 64 samples (at most 64 the run is one step anyway): classes of
-    // 1/64 output pixel up to the narrow limit rz < 0.5 and the plane's
-    // largest rz, 32 x 32 cells.
+    // Candidate lists for the direct test, one set per sample table whose
+    // x-run for a grain of the mean radius exceeds one 64-sample step:
+    // classes of 1/64 output pixel up to the narrow limit rz < 0.5 and the
+    // plane's largest rz, 32 x 32 cells.
     const int cq = 32;
     const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
     const int classes = std::max(1, std::min(32, (int)std::ceil(std::min(a.rm * a.zoom + slack, 0.5f) * 64.0f)));
     const int ew = (int)std::ceil((2.0 * (double)st.eps + std::ldexp(1.0, -11)) * cq + 0.01);
     auto list_bytes = [&](int n) -> size_t {
         if (n <= 64 || n > 2048) return 0;
+        // no lists where a grain of the mean radius walks one step anyway
+        const double run = (double)n * (2.0 * ((double)a.r_const * a.zoom + slack) + 2.0 * st.eps);
+        if (run <= 64.0) return 0;
         const size_t b = 256 + (size_t)classes * cq * cq * 8 + fg::splat_lists_entries(cq, classes, ew, n) * 2;
         return (b + 255) & ~(size_t)255;
     };
