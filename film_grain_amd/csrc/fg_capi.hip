@@ -847,7 +847,7 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
         // constant radius with a narrow window (2R + 1/Q <= 0.35 output px):
         // candidate lists per cell of the grain's fractional target, about
         // N (2R + 1/Q)^2 samples each instead of a run of N (2R + 1/Q)
-        const int cq = 32;
+        const int cq = 64;
         const float r_out = std::min(d->radius_mean, d->rm) * d->zoom;
         if (e == hipSuccess && d->dist_kind == 0 && r_out > 0.0f && 2.0f * r_out + 1.0f / (float)cq <= 0.35f) {
             const size_t nstart = (size_t)cq * cq + 1;

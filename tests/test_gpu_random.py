@@ -97,10 +97,10 @@ def test_random_planes_adaptive_uniform_budget(hip_ctx, oracle):
         assert bad == 0, case.name
 
 
-@pytest.mark.parametrize("r_out", [0.01, 0.05, 0.12, 0.159, 0.1595, 0.17])
+@pytest.mark.parametrize("r_out", [0.01, 0.05, 0.12, 0.167, 0.1675, 0.18])
 def test_grainwise_candidate_lists_near_the_switch(hip_ctx, oracle, r_out):
     """Constant output radii on both sides of the switch to per-cell
-    candidate lists (2 R + 1/32 <= 0.35 output pixels, so R <= 0.159375):
+    candidate lists (2 R + 1/64 <= 0.35 output pixels, so R <= 0.1671875):
     the sorted kernel gives the oracle's planes bit for bit, for sample
     counts from 1 to 2048 and several zooms."""
     rng = np.random.default_rng(int(r_out * 1e4))

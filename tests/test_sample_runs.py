@@ -88,8 +88,9 @@ def cell_members(fx, fy, qa, qb, Q, R, eps):
     return (dx <= w) & (dy <= w)
 
 
+@pytest.mark.parametrize("Q", [32, 64])
 @pytest.mark.parametrize("N,R,sd", [(1, 0.05, 0.8), (128, 0.05, 0.8), (513, 0.159, 1.5), (2048, 0.01, 0.4)])
-def test_cell_list_contains_every_candidate(N, R, sd):
+def test_cell_list_contains_every_candidate(N, R, sd, Q):
     """The grain-wise kernel's per-cell candidate lists (constant radius):
     every sample whose f32 bounds() box (src/grainwise.rs:126-142) has both
     a column and a row is in the list of the cell of the grain's target
@@ -100,7 +101,6 @@ def test_cell_list_contains_every_candidate(N, R, sd):
     fx = (X[:, 0] - np.floor(X[:, 0])).astype(f32)
     fy = (X[:, 1] - np.floor(X[:, 1])).astype(f32)
     R = f32(R)
-    Q = 32
     c = f32(f32(4096 + R + 8) + np.abs(X).max())
     eps = f32(f32(c * f32(2.0 ** -20)) + f32(2.0 ** -16))
     M = 3000
