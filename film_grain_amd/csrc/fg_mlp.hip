@@ -411,8 +411,7 @@ hipError_t launch_budget(const float *pred, const uint16_t *X, long long n, floa
 // subsample i < m.  The smallest c whose budgets clamp(ceil(c s), nmin, nmax)
 // have mean predicted squared error s^2 (1/n - 1/n_ref) <= target, by rounds
 // of kScaleProbes geometric candidates between lo and hi (each round keeps the
-// bracket of the first candidate that meets the target).  State: lo, hi, c,
-// done; then kScaleProbes double sums.
+// bracket of the first candidate that meets the target).
 constexpr int kScaleProbes = 64;
 
 __device__ __forceinline__ float scale_probe(float lo, float hi, int j)
@@ -421,24 +420,32 @@ __device__ __forceinline__ float scale_probe(float lo, float hi, int j)
     return expf(a + (b - a) * ((float)j / (float)(kScaleProbes - 1)));
 }
 
-__global__ __launch_bounds__(1024) void scale_init_kernel(const float *pred, long long step, long long m, float s_mult,
-                                                          int nmax, float *state, double *sums)
+// State: lo, hi, c, done (floats), the bits of the smallest positive s (a
+// positive float orders as its bits do), then the probe sums.
+__global__ void scale_clear_kernel(float *state, double *sums)
 {
-    __shared__ float s_min[1024];
+    if (threadIdx.x < kScaleProbes) sums[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) reinterpret_cast<unsigned int *>(state)[4] = 0x7F800000u;  // +inf
+}
+
+__global__ __launch_bounds__(256) void scale_min_kernel(const float *pred, long long step, long long m, float s_mult,
+                                                        float *state)
+{
     float mn = INFINITY;
-    for (long long i = threadIdx.x; i < m; i += 1024) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
         const float s = expf(pred[i * step]) * s_mult;
         if (s > 0.0f) mn = fminf(mn, s);
     }
-    s_min[threadIdx.x] = mn;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s_min[threadIdx.x] = fminf(s_min[threadIdx.x], s_min[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x < kScaleProbes) sums[threadIdx.x] = 0.0;
+    for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o, 64));
+    if ((threadIdx.x & 63) == 0 && mn < INFINITY)
+        atomicMin(reinterpret_cast<unsigned int *>(state) + 4, __float_as_uint(mn));
+}
+
+__global__ void scale_setup_kernel(int nmax, float *state)
+{
     if (threadIdx.x == 0) {
-        const float smin = s_min[0] < INFINITY ? fmaxf(s_min[0], 1e-6f) : 1.0f;
+        const float mn = __uint_as_float(reinterpret_cast<const unsigned int *>(state)[4]);
+        const float smin = mn < INFINITY ? fmaxf(mn, 1e-6f) : 1.0f;
         state[0] = 1e-2f;
         state[1] = 2.0f * (float)nmax / smin;
         state[2] = state[1];
@@ -519,7 +526,7 @@ __global__ void scale_out_kernel(const float *state, float s_mult, float *out)
     }
 }
 
-size_t budget_scale_state_bytes() { return 16 + sizeof(double) * kScaleProbes; }
+size_t budget_scale_state_bytes() { return 32 + sizeof(double) * kScaleProbes; }
 
 hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, double target, int nmin, int nmax,
                                int n_ref, int rounds, long long max_rows, void *state_buf, float *out, hipStream_t s)
@@ -528,10 +535,18 @@ hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, dou
     const long long step = std::max<long long>(1, n / max_rows);
     const long long m = (n + step - 1) / step;
     float *state = static_cast<float *>(state_buf);
-    double *sums = reinterpret_cast<double *>(static_cast<unsigned char *>(state_buf) + 16);
-    hipLaunchKernelGGL(scale_init_kernel, dim3(1), dim3(1024), 0, s, pred, step, m, s_mult, nmax, state, sums);
-    hipError_t e = hipGetLastError();
+    double *sums = reinterpret_cast<double *>(static_cast<unsigned char *>(state_buf) + 32);
     const unsigned grid = (unsigned)std::min<long long>((m + 255) / 256, 1024);
+    hipLaunchKernelGGL(scale_clear_kernel, dim3(1), dim3(kScaleProbes), 0, s, state, sums);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(scale_min_kernel, dim3(grid), dim3(256), 0, s, pred, step, m, s_mult, state);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(scale_setup_kernel, dim3(1), dim3(64), 0, s, nmax, state);
+        e = hipGetLastError();
+    }
     for (int r = 0; r < rounds && e == hipSuccess; ++r) {
         hipLaunchKernelGGL(scale_sums_kernel, dim3(grid), dim3(256), 0, s, pred, step, m, s_mult, nmin, nmax,
                            1.0f / (float)n_ref, state, sums);
