@@ -818,11 +818,16 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         // the first 128 list entries of the next listed grain, loaded while
         // the current one is tested
         int na = 0, nb = 0;
+        // (a lane past the list reads its last entry, which its i < len
+        // test then ignores: no branches around the loads)
         auto prefetch = [&](int l) {
             if (LISTS && ((lmask >> l) & 1ull)) {
                 const int q0 = lane_i(p0, l), qn = lane_i(len, l);
-                na = lane < qn ? (int)st.cl_list[q0 + lane] : 0;
-                nb = lane + 64 < qn ? (int)st.cl_list[q0 + 64 + lane] : 0;
+                if (qn > 0) {
+                    const uint32_t last = (uint32_t)(q0 + qn - 1);
+                    na = (int)st.cl_list[min((uint32_t)(q0 + lane), last)];
+                    nb = (int)st.cl_list[min((uint32_t)(q0 + 64 + lane), last)];
+                }
             }
         };
         if (m) prefetch(__ffsll((long long)m) - 1);
@@ -911,8 +916,9 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         pa = ca;
                         pb = cb;
                     } else if (ulisted) {
-                        pa = ia < ulen ? (int)st.cl_list[up0 + ia] : 0;
-                        pb = ib < ulen ? (int)st.cl_list[up0 + ib] : 0;
+                        const uint32_t last = (uint32_t)(up0 + ulen - 1);
+                        pa = (int)st.cl_list[min((uint32_t)(up0 + ia), last)];
+                        pb = (int)st.cl_list[min((uint32_t)(up0 + ib), last)];
                     } else {
                         pa = position(up0 + ia);
                         pb = position(up0 + ib);
