@@ -496,6 +496,43 @@ hipError_t tile_order_sort(fg_ctx *ctx, int n, const TileOrder &t, hipStream_t s
     return fg::sort_tiles(ctx->tile_temp.p, &temp, t.w, t.w_sorted, t.ids, t.order, n, s);
 }
 
+// Candidate lists of the splat's direct test for one sample table: classes
+// of 1/64 output pixel up to the narrow limit rz < 0.5 and the plane's
+// largest rz, 32 x 32 cells, and only where a grain of the mean radius walks
+// more than one 64-sample step of its x-run.
+struct ListPlan {
+    int q = 32, classes = 0, ew = 0;
+    size_t bytes(int n) const  // fill counters | (start, length) cells | entries
+    {
+        if (classes == 0 || n <= 64 || n > 2048 || !use(n)) return 0;
+        const size_t b = 256 + (size_t)classes * q * q * 8 + fg::splat_lists_entries(q, classes, ew, n) * 2;
+        return (b + 255) & ~(size_t)255;
+    }
+    bool use(int n) const { return (double)n * run_per_sample > 64.0; }
+    double run_per_sample = 0.0;
+};
+
+ListPlan list_plan(const fg::PixelArgs &a, float eps)
+{
+    ListPlan lp;
+    const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
+    lp.classes = std::max(1, std::min(32, (int)std::ceil(std::min(a.rm * a.zoom + slack, 0.5f) * 64.0f)));
+    lp.ew = (int)std::ceil((2.0 * (double)eps + std::ldexp(1.0, -11)) * lp.q + 0.01);
+    lp.run_per_sample = 2.0 * ((double)a.r_const * a.zoom + slack) + 2.0 * eps;
+    return lp;
+}
+
+// Builds table t's lists in `at` (lp.bytes(t.n) bytes) and points t at them.
+hipError_t build_lists(const ListPlan &lp, fg::SampleTable &t, unsigned char *at, int *status, hipStream_t s)
+{
+    uint32_t *fill = reinterpret_cast<uint32_t *>(at);
+    uint2 *cs = reinterpret_cast<uint2 *>(at + 256);
+    uint16_t *cl = reinterpret_cast<uint16_t *>(cs + (size_t)lp.classes * lp.q * lp.q);
+    t.cl_cell = cs;
+    t.cl_list = cl;
+    return fg::launch_splat_lists(t, lp.q, lp.classes, lp.ew, fill, cs, cl, status, s);
+}
+
 // Per-plane generation into HBM, then the splat-from-HBM kernel.
 int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
                   hipStream_t s)
@@ -538,22 +575,8 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
     if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
-    // Candidate lists for the direct test, one set per sample table whose
-    // x-run for a grain of the mean radius exceeds one 64-sample step:
-    // classes of 1/64 output pixel up to the narrow limit rz < 0.5 and the
-    // plane's largest rz, 32 x 32 cells.
-    const int cq = 32;
-    const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
-    const int classes = std::max(1, std::min(32, (int)std::ceil(std::min(a.rm * a.zoom + slack, 0.5f) * 64.0f)));
-    const int ew = (int)std::ceil((2.0 * (double)st.eps + std::ldexp(1.0, -11)) * cq + 0.01);
-    auto list_bytes = [&](int n) -> size_t {
-        if (n <= 64 || n > 2048) return 0;
-        // no lists where a grain of the mean radius walks one step anyway
-        const double run = (double)n * (2.0 * ((double)a.r_const * a.zoom + slack) + 2.0 * st.eps);
-        if (run <= 64.0) return 0;
-        const size_t b = 256 + (size_t)classes * cq * cq * 8 + fg::splat_lists_entries(cq, classes, ew, n) * 2;
-        return (b + 255) & ~(size_t)255;
-    };
+    // candidate lists for the direct test, one set per sample table
+    const ListPlan lp = list_plan(a, st.eps);
     std::vector<int> &ln = ctx->level_n;
     ln.clear();
     if (a.budget) {
@@ -561,31 +584,26 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     }
     ln.push_back(a.n_samples);
     size_t lists_total = 0;
-    for (int l : ln) lists_total += list_bytes(l);
+    for (int l : ln) lists_total += lp.bytes(l);
     if (lists_total > 0 && lists_total < ((size_t)1 << 32)) {
         if ((rc = ensure_bytes(ctx, ctx->cells, lists_total, what)) != FG_OK) return rc;
-        a.cl_q = cq;
-        a.cl_classes = classes;
-        a.cl_inv_dz = (float)(2 * cq);
+        a.cl_q = lp.q;
+        a.cl_classes = lp.classes;
+        a.cl_inv_dz = (float)(2 * lp.q);
     }
     unsigned char *lists_at = static_cast<unsigned char *>(ctx->cells.p);
-    // builds table t's lists at lists_at (fill counters | cells | entries)
-    auto build_lists = [&](fg::SampleTable &t) -> hipError_t {
-        const size_t b = list_bytes(t.n);
+    auto build_lists_for = [&](fg::SampleTable &t) -> hipError_t {
+        const size_t b = lp.bytes(t.n);
         if (a.cl_q == 0 || b == 0) return hipSuccess;
-        uint32_t *fill = reinterpret_cast<uint32_t *>(lists_at);
-        uint2 *cs = reinterpret_cast<uint2 *>(lists_at + 256);
-        uint16_t *cl = reinterpret_cast<uint16_t *>(cs + (size_t)classes * cq * cq);
-        t.cl_cell = cs;
-        t.cl_list = cl;
+        const hipError_t err = build_lists(lp, t, lists_at, a.status, s);
         lists_at += b;
-        return fg::launch_splat_lists(t, cq, classes, ew, fill, cs, cl, a.status, s);
+        return err;
     };
     if (e == hipSuccess && !a.budget) {
         // the splat kernel reads its one table from device memory
         if ((rc = ensure_bytes(ctx, ctx->levels, 256, what)) != FG_OK) return rc;
         fg::SampleTable *d_table = static_cast<fg::SampleTable *>(ctx->levels.p);
-        e = build_lists(st);
+        e = build_lists_for(st);
         if (e == hipSuccess) e = fg::launch_store_table(st, d_table, s);
         a.level_tables = d_table;
     }
@@ -611,7 +629,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
                 lt.eps = st.eps;  // bounds of the whole offset set hold for every prefix
                 lt.safe_r = st.safe_r;
             }
-            if (e == hipSuccess) e = build_lists(lt);
+            if (e == hipSuccess) e = build_lists_for(lt);
             p += (fg::sample_table_bytes(ln[i], sample_buckets((uint32_t)ln[i])) + 255) & ~(size_t)255;
         }
         uint8_t *d_tile_level = p;
@@ -1073,6 +1091,8 @@ struct fg_batch {
     int n = 0;
     std::vector<void *> allocs;
     std::vector<fg::BatchPlane> planes;
+    std::vector<ListPlan> list_plans;       // per plane: the splat's candidate lists
+    std::vector<unsigned char *> list_mem;  // per plane: their buffer, or nullptr
     std::vector<int> kind;             // radius law per plane
     std::vector<double *> T;           // thresholds per plane
     std::vector<long long> bin_off;    // first bin of each plane in the batch's bins
@@ -1145,6 +1165,12 @@ hipError_t batch_enqueue(fg_batch *b, hipStream_t s)
         fg::SampleTable st{};
         if (e == hipSuccess)
             e = fg::launch_sample_table(bp.a.offs, bp.a.n_samples, bp.a.zoom, b->qb[p], b->samples[p], &st, s);
+        if (e == hipSuccess && b->list_mem[p]) {
+            // the plane record's table (its eps, and the list pointers the
+            // records already hold) over the table just written
+            fg::SampleTable t = bp.st;
+            e = build_lists(b->list_plans[p], t, b->list_mem[p], b->status, s);
+        }
     }
     if (e == hipSuccess) e = fg::launch_batch_weights(b->d_planes, b->n, b->weights, b->n_items, s);
     if (e == hipSuccess) e = fg::launch_batch_order(b->d_planes, b->n, b->weights, b->n_items, b->items, nullptr, s);
@@ -1186,6 +1212,8 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     b->bin_off.resize(n);
     b->samples.resize(n);
     b->qb.resize(n);
+    b->list_plans.resize(n);
+    b->list_mem.resize(n);
     std::vector<int> lds_p(n);
     long long items = 0;
     for (int p = 0; p < n; ++p) {
@@ -1211,6 +1239,19 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
         if ((rc = batch_alloc(b, &b->samples[p], fg::sample_table_bytes((int)d->n_samples, b->qb[p]), what)) != FG_OK)
             return rc;
         fg::sample_table_layout(b->samples[p], (int)d->n_samples, b->qb[p], &bp.st);
+        // candidate lists, built with the plane's table on every run
+        b->list_plans[p] = list_plan(bp.a, bp.st.eps);
+        b->list_mem[p] = nullptr;
+        const size_t lb = b->list_plans[p].bytes((int)d->n_samples);
+        if (lb > 0 && lb < ((size_t)1 << 32)) {
+            if ((rc = batch_alloc(b, (void **)&b->list_mem[p], lb, what)) != FG_OK) return rc;
+            const ListPlan &lp = b->list_plans[p];
+            bp.st.cl_cell = reinterpret_cast<const uint2 *>(b->list_mem[p] + 256);
+            bp.st.cl_list = reinterpret_cast<const uint16_t *>(bp.st.cl_cell + (size_t)lp.classes * lp.q * lp.q);
+            bp.a.cl_q = lp.q;
+            bp.a.cl_classes = lp.classes;
+            bp.a.cl_inv_dz = (float)(2 * lp.q);
+        }
     }
     if (items > 0x7fffffffLL || b->total_bins >= 0xffffffffLL)
         return set_error(ctx, FG_EINVAL, "batch too large");

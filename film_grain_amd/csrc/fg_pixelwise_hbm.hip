@@ -510,7 +510,8 @@ hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, 
                               uint16_t *list, int *status, hipStream_t s)
 {
     if (q < 1 || q * q > 65536 || classes < 1 || st.n > 2048) return hipErrorInvalidValue;
-    hipError_t err = hipMemsetAsync(fill, 0, sizeof(uint32_t) * classes, s);
+    // a kernel, not hipMemsetAsync: batches capture this into a graph
+    hipError_t err = launch_fill_u32(fill, classes, 0u, s);
     if (err != hipSuccess) return err;
     const int parts = (q * q + 63) / 64;
     hipLaunchKernelGGL(splat_lists_kernel, dim3(classes * parts), dim3(256), 0, s, st, q, e, fill, cell, list,
@@ -599,7 +600,10 @@ __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_re
 // persistent batch kernel (which therefore give identical planes).  lds is the
 // workgroup's dynamic LDS (a.lds_bytes), s_queue its per-wave queues.  Starts
 // with a barrier, so a workgroup may call it for tile after tile.
-template <int BLOCK, bool LISTS>
+// LISTS: walk the candidate lists of the tile's table (SampleTable::cl_*).
+// DEV_TABLE: read the table from a.level_tables (the single-plane kernel)
+// rather than st_all (the batch kernel's plane record).
+template <int BLOCK, bool LISTS, bool DEV_TABLE>
 __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st_all,
                                            const uint32_t *bin_start, const float4 *grains, int tile,
                                            unsigned char *lds, uint32_t (*s_queue)[kQueueCap])
@@ -626,7 +630,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const float rm = a.rm, delta = a.delta, rd = a.rdelta, iz = a.inv_zoom, z = a.zoom;
     const int fast = a.div_fast;
     // adaptive sampling: the tile's level table holds its first N samples
-    // The single-plane kernel (LISTS) reads its table from a.level_tables:
+    // The single-plane kernel (DEV_TABLE) reads its table from a.level_tables:
     // the tile's level's table, or the one table of a fixed-N render.  It is
     // read through the constant address space (the tile is uniform, so the
     // level and the table's fields become scalar loads, as kernel arguments)
@@ -634,7 +638,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     // compiler keep a copy of the walk for each.  The batch kernel passes its
     // plane's table as st_all.
     const SampleTable *lt = &st_all;
-    if (LISTS) {
+    if (DEV_TABLE) {
         const int lvl = a.budget ? ((const __attribute__((address_space(4))) uint8_t *)a.tile_level)[tile] : 0;
         lt = (const SampleTable *)((const __attribute__((address_space(4))) SampleTable *)a.level_tables + lvl);
     }
@@ -978,7 +982,7 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
     const int tile = a.tile_list ? a.tile_list[blockIdx.x] : (int)blockIdx.x;
-    splat_tile<BLOCK, true>(a, geo, st, bin_start, grains, tile, lds, s_queue);
+    splat_tile<BLOCK, true, true>(a, geo, st, bin_start, grains, tile, lds, s_queue);
 }
 
 // Persistent batch kernel: every workgroup takes (plane, tile) items from the
@@ -1006,7 +1010,7 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
         // (generic loads kept them in VGPRs: 91 VGPRs against 68).
         const auto *cp = (const __attribute__((address_space(4))) BatchPlane *)planes + pl;
         const BatchPlane &p = *(const BatchPlane *)cp;
-        splat_tile<BLOCK, false>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
+        splat_tile<BLOCK, true, false>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
     }
 }
 
