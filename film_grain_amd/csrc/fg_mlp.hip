@@ -565,9 +565,12 @@ hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, dou
 
 hipError_t launch_mlp_forward(const float *P, const uint16_t *X, long long n, float *out, hipStream_t s)
 {
-    // waves loop over 16-row tiles; enough waves to fill the device
+    // waves loop over 16-row tiles.  Each wave first loads the whole network
+    // into registers (138 VGPRs: 3 waves per SIMD), so the grid is one
+    // resident round of waves (3 workgroups of 4 waves per CU on 256 CUs)
+    // and each wave amortises that load over many tiles.
     const long long tiles = (n + 15) / 16;
-    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 4096);
+    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 768);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, P, X, n, out);
     return hipGetLastError();
 }
