@@ -182,12 +182,14 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
 // 0.5)).  With the samples ordered by frac(X_k) (SampleTable, built from the
 // output-pixel offsets) those samples are one cyclic run, found through the
 // bucket index; the interval is widened by the table's eps, so the run holds
-// every such sample.  Each lane owns one input pixel of the tile's halo
-// region: it draws that pixel's grains in the reference's order and, per
-// grain, walks its run; a (grain, sample) pair whose bounds() box is
-// non-empty and meets the tile goes to the wave's queue, one entry per pixel
-// of the box inside the tile, and the queue is drained through the
-// reference's disk test into the tile's LDS bitset.
+// every such sample.  For a constant radius with a narrow window, per-cell
+// candidate lists over both fractions (cell_lists_kernel) replace the run.
+// Each lane owns one input pixel of the tile's halo region: it draws that
+// pixel's grains in the reference's order and, per grain, walks its list or
+// run.  A (grain, sample) pair whose bounds() box, clipped to the tile, is one
+// pixel takes the reference's disk test in the lane; larger boxes go to the
+// wave's queue, one entry per pixel of the box inside the tile, drained
+// through the same test.  Hits go to the tile's LDS bitset.
 // Poisson pre-pass (src/grainwise.rs:37-45): per input pixel with lambda > 0,
 // pixel_rng and the grain count; the count and the stream's state after the
 // count draws are stored so that the splat kernel resumes each pixel's stream

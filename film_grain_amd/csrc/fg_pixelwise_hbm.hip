@@ -14,13 +14,19 @@
 //               draws run on lanes that all have work
 //   samples     the N sample offsets ordered by the fraction of their scaled
 //               x offset, with a bucket index over that fraction
+//   lists       candidate lists over that order per (radius class, cell of a
+//               grain's centre fractions): the samples whose box can hold a
+//               pixel column and row (splat_lists_kernel)
 //   splat       per output tile, every wave streams 64-grain chunks of the
-//               tile's bins on its own; for each grain the lanes walk the one
-//               cyclic run of the sample order whose x fraction can put a
-//               pixel column in the grain's candidate box, queue the (grain,
-//               sample) pairs whose box meets the tile, and drain the queue 64
-//               pairs at a time through the exact reference test into an LDS
-//               bitset of (pixel, sample) coverage.
+//               tile's bins on its own; for each grain the lanes walk its
+//               list, or else the one cyclic run of the sample order whose x
+//               fraction can put a pixel column in the grain's candidate box.
+//               A narrow grain with no cell window to check tests the pixel
+//               nearest each sample's target with the exact reference test
+//               in the lane; other grains queue the (grain, sample, pixel)
+//               candidates whose box meets the tile and drain the queue 64 at
+//               a time through the same test.  Hits go to an LDS bitset of
+//               (pixel, sample) coverage.
 // A grain record is (cx, cy, r, cell) with cell = (i - Ix0) | (j - Jy0) << 16.
 #include <hipcub/hipcub.hpp>
 
