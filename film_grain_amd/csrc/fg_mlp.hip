@@ -540,16 +540,20 @@ hipError_t launch_budget_scale(const float *pred, long long n, float s_mult, dou
     hipLaunchKernelGGL(scale_clear_kernel, dim3(1), dim3(kScaleProbes), 0, s, state, sums);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(scale_min_kernel, dim3(grid), dim3(256), 0, s, pred, step, m, s_mult, state);
+        hipLaunchKernelGGL(scale_min_kernel, dim3(std::min(grid, 128u)), dim3(256), 0, s, pred, step, m, s_mult,
+                           state);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(scale_setup_kernel, dim3(1), dim3(64), 0, s, nmax, state);
         e = hipGetLastError();
     }
+    // each thread of the sums kernel folds several rows before the 64-probe
+    // reduction across its wave, which costs about as much as 8 rows
+    const unsigned sums_grid = (unsigned)std::min<long long>((m + 2047) / 2048, 128);
     for (int r = 0; r < rounds && e == hipSuccess; ++r) {
-        hipLaunchKernelGGL(scale_sums_kernel, dim3(grid), dim3(256), 0, s, pred, step, m, s_mult, nmin, nmax,
-                           1.0f / (float)n_ref, state, sums);
+        hipLaunchKernelGGL(scale_sums_kernel, dim3(std::max(sums_grid, 1u)), dim3(256), 0, s, pred, step, m, s_mult,
+                           nmin, nmax, 1.0f / (float)n_ref, state, sums);
         e = hipGetLastError();
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(scale_pick_kernel, dim3(1), dim3(kScaleProbes), 0, s, m, target, r == rounds - 1 ? 1 : 0,
