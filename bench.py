@@ -72,9 +72,21 @@ def measured_traffic(workload, kernel, cfg):
     return None if rec is None else rec["traffic_bytes"]
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     """Times the oracle on random 16x16 output tiles of plane 0 and checks the
-    GPU frame on the same tiles."""
+    GPU frame on the same tiles; also a one-thread rate on a smaller slice of
+    the same kind of sample, and the host's CPU model."""
     from oracle import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     od = O.derive(wl["radius"], wl["sd"], wl["dist"], wl["zoom"], 0.8, wl["iters"], seed=wl["seed"],
@@ -96,10 +108,13 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
                         seed=wl["seed"], in_w=crop, in_h=crop)
         pc = O.plane(od_c, wl["iters"], wl["seed"], wl["zoom"], wl["dist"], wl["radius"])
         n_crops = 0
+        first = None
         while t_used < budget_s and n_crops < 64:
             x0 = int(rng.integers(0, wl["w"] - crop + 1))
             y0 = int(rng.integers(0, wl["h"] - crop + 1))
             lam_c = np.ascontiguousarray(lam0[y0:y0 + crop, x0:x0 + crop])
+            if first is None:
+                first = lam_c
             t0 = time.perf_counter()
             O.render_grainwise(pc, lam_c, offs, threads=threads)
             t_used += time.perf_counter() - t0
@@ -107,8 +122,12 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
             n_crops += 1
         sample = (f"grain-wise oracle on {n_crops} random {crop}x{crop} input crops of the frame "
                   f"({n_px * wl['iters']:.3g} pixel-samples, {t_used:.1f} s)")
+        t0 = time.perf_counter()
+        O.render_grainwise(pc, first, offs, threads=1)
+        one = pc.out_w * pc.out_h * wl["iters"] / (time.perf_counter() - t0) / 1e6
         return dict(value=round(n_px * wl["iters"] / t_used / 1e6, 3), unit="Mpix·samples/s", cores=threads,
-                    kind="port", sample=sample), None
+                    kind="port", sample=sample, one_thread_value=round(one, 3),
+                    one_thread_sample=f"the first of those crops", cpu_model=_cpu_model()), None
     while t_used < budget_s:
         x0 = int(rng.integers(0, od.out_w - tile + 1))
         y0 = int(rng.integers(0, od.out_h - tile + 1))
@@ -124,8 +143,21 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     value = n_px * wl["iters"] / t_used / 1e6
     sample = (f"{n_tiles} random {tile}x{tile} output tiles of colour plane 0 "
               f"({n_px * wl['iters']:.3g} pixel-samples, {t_used:.1f} s)")
+    # one thread, on the first 24 tiles of the same random sequence
+    rng1 = np.random.default_rng(2024)
+    n1, t1 = 0, 0.0
+    for _ in range(min(24, n_tiles)):
+        x0 = int(rng1.integers(0, od.out_w - tile + 1))
+        y0 = int(rng1.integers(0, od.out_h - tile + 1))
+        t0 = time.perf_counter()
+        O.render_pixelwise(p, lam0, offs, region=(x0, y0, x0 + tile, y0 + tile), threads=1)
+        t1 += time.perf_counter() - t0
+        n1 += tile * tile
+    one = n1 * wl["iters"] / t1 / 1e6 if t1 > 0 else None
     return dict(value=round(value, 3), unit="Mpix·samples/s", cores=threads, kind="port",
-                sample=sample), dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
+                sample=sample, one_thread_value=None if one is None else round(one, 3),
+                one_thread_sample=f"the first {min(24, n_tiles)} of those tiles", cpu_model=_cpu_model()), \
+        dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
 
 
 def batch_bench(args):
