@@ -529,7 +529,7 @@ size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {
     const size_t words = ((size_t)n_samples + 31) / 32;
     return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 16) +
-           align16((size_t)n_samples * 2) + align16((size_t)(qbuckets + 1) * 2);
+           align16((size_t)n_samples * 8) + align16((size_t)(qbuckets + 1) * 2);
 }
 
 
@@ -654,17 +654,21 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
     const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
 
-    // ---- LDS: bitset | (xy, o)[N] | k[N] | qidx[Q + 1] ----
+    // ---- LDS: bitset | (xy, o)[N] | (word, bit)[N] | qidx[Q + 1] ----
     // a sample's scaled offset and its offset in input pixels side by side:
-    // the direct test reads both with one 16-byte load
+    // the direct test reads both with one 16-byte load.  Sample k's bit of
+    // tile pixel i is bit k % 32 of bitset word i * words + k / 32; per
+    // sorted position the LDS holds that word's byte offset for pixel 0 and
+    // the bit's mask, so a hit's address is one multiply-add.
     const int tile_words = a.tile_w * a.tile_h * words;
+    const uint32_t w4 = (uint32_t)words * 4u;
     size_t off = 0;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
     off += align16((size_t)tile_words * 4);
     float4 *sxo = reinterpret_cast<float4 *>(lds + off);
     off += align16((size_t)N * 16);
-    uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
-    off += align16((size_t)N * 2);
+    uint2 *sw = reinterpret_cast<uint2 *>(lds + off);
+    off += align16((size_t)N * 8);
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)(Q + 1) * 2);
     if (off > (size_t)a.lds_bytes) {
@@ -675,7 +679,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     for (int k = tid; k < N; k += BLOCK) {
         const float2 X = st.xy[k], O = st.o[k];
         sxo[k] = make_float4(X.x, X.y, O.x, O.y);
-        sk[k] = st.k[k];
+        const uint32_t ki = st.k[k];
+        sw[k] = make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     const float4 ext = *st.ext;
@@ -712,7 +717,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
         const float4 XO = sxo[p];
         const float2 O = make_float2(XO.z, XO.w);
-        const int k = sk[p];
+        const uint2 wm = sw[p];
         const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
         const float yg = ((float)(t.Y0 + ty) + 0.5f) * iz - O.y;
         const float dx = xg - cx, dy = yg - cy;
@@ -722,7 +727,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             if (rs > 0.0f ||
                 (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
                  floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)))
-                atomicOr(&bits[(ty * t.TW + tx) * words + (k >> 5)], 1u << (k & 31));
+                atomicOr(reinterpret_cast<uint32_t *>(lds + ((uint32_t)(ty * t.TW + tx) * w4 + wm.x)), wm.y);
         }
     };
 
@@ -904,7 +909,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 // expression) of that one pixel, when it lies in the tile,
                 // sets exactly the bits the queue would have set.
                 const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
-                auto test = [&](int i, float4 XO, int k) {
+                auto test = [&](int i, float4 XO, uint2 wm) {
                     const float px = rintf(ubx + XO.x), py = rintf(uby + XO.y);
                     const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
                     if ((i < ulen) & ((unsigned)tx < (unsigned)tw) & ((unsigned)ty < (unsigned)th)) {
@@ -912,9 +917,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         const float yg = (py + 0.5f) * iz - XO.w;
                         const float dx = xg - ucx, dy = yg - ucy;
                         if (dx * dx + dy * dy <= urs * urs) {
-                            // tile coordinates and words are far below 2^24
+                            // tile coordinates and the bitset's bytes are far
+                            // below 2^24
                             const uint32_t px_i = __umul24((uint32_t)ty, (uint32_t)t.TW) + (uint32_t)tx;
-                            atomicOr(&bits[__umul24(px_i, (uint32_t)words) + ((uint32_t)k >> 5)], 1u << (k & 31));
+                            atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24(px_i, w4) + wm.x)), wm.y);
                         }
                     }
                 };
@@ -934,14 +940,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                         pb = position(up0 + ib);
                     }
                     const float4 XOa = sxo[pa], XOb = sxo[pb];
-                    const int ka = sk[pa], kb = sk[pb];
-                    // all six reads issue together, ahead of the tests (an
+                    const uint2 wa = sw[pa], wb = sw[pb];
+                    // all the reads issue together, ahead of the tests (an
                     // empty asm that takes the values as operands keeps the
                     // compiler from moving the offset and index reads into
                     // the branches after the dependent ones)
-                    asm volatile("" ::"v"(XOa.z), "v"(XOa.w), "v"(ka), "v"(XOb.z), "v"(XOb.w), "v"(kb));
-                    test(ia, XOa, ka);
-                    if (i0 + 64 < ulen) test(ib, XOb, kb);
+                    asm volatile("" ::"v"(XOa.z), "v"(XOa.w), "v"(wa.x), "v"(wa.y), "v"(XOb.z), "v"(XOb.w),
+                                 "v"(wb.x), "v"(wb.y));
+                    test(ia, XOa, wa);
+                    if (i0 + 64 < ulen) test(ib, XOb, wb);
                 }
                 continue;
             }
