@@ -744,6 +744,30 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         }
     };
 
+    // The direct test of a narrow grain whose disk test implies its cell
+    // window (rs > 0) at the sample with offsets XO and bitset word/bit wm.
+    // A pixel whose point passes the disk test lies within rz of the
+    // sample's target (bx + X.x, by + X.y) in each axis (the candidate box
+    // is a superset); with rz < 0.5 that is only the pixel nearest the
+    // target.  So the drain's test (same f32 expression, rs2 = r * r) of
+    // that one pixel, when it lies in the tile, sets exactly the bits the
+    // queue would have set.  The grain's values may differ between lanes.
+    const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
+    auto test = [&](bool ok, float4 XO, uint2 wm, float gbx, float gby, float gcx_, float gcy_, float rs2) {
+        const float px = rintf(gbx + XO.x), py = rintf(gby + XO.y);
+        const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
+        if (ok & ((unsigned)tx < (unsigned)tw) & ((unsigned)ty < (unsigned)th)) {
+            const float xg = (px + 0.5f) * iz - XO.z;
+            const float yg = (py + 0.5f) * iz - XO.w;
+            const float dx = xg - gcx_, dy = yg - gcy_;
+            if (dx * dx + dy * dy <= rs2) {
+                // tile coordinates and the bitset's bytes are far below 2^24
+                const uint32_t px_i = __umul24((uint32_t)ty, (uint32_t)t.TW) + (uint32_t)tx;
+                atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24(px_i, w4) + wm.x)), wm.y);
+            }
+        }
+    };
+
     // ---- each wave walks the region's bin rows in 64-grain chunks ----
     int row = by0, cbase = 0, nch = 0;
     uint32_t rs_ = 0, re_ = 0;
@@ -824,33 +848,50 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 listed = true;
             }
         }
-        const unsigned long long lmask = __ballot(listed);
         gcx = G.x;
         gcy = G.y;
         gr = rad <= safe_r ? rad : -rad;
         gcell = pk;
-        unsigned long long m = __ballot(keep);
-        // the first 128 list entries of the next listed grain, loaded while
-        // the current one is tested
-        int na = 0, nb = 0;
-        // (a lane past the list reads its last entry, which its i < len
-        // test then ignores: no branches around the loads)
-        auto prefetch = [&](int l) {
-            if (LISTS && ((lmask >> l) & 1ull)) {
-                const int q0 = lane_i(p0, l), qn = lane_i(len, l);
-                if (qn > 0) {
-                    const uint32_t last = (uint32_t)(q0 + qn - 1);
-                    na = (int)st.cl_list[min((uint32_t)(q0 + lane), last)];
-                    nb = (int)st.cl_list[min((uint32_t)(q0 + 64 + lane), last)];
+        // the listed grains (all narrow, all with rs > 0) take the
+        // concatenated walk below; the others the per-grain walk after it
+        unsigned long long m = __ballot(keep && !listed);
+        unsigned long long lrem = LISTS ? __ballot(listed && len > 0) : 0ull;
+        if (LISTS && lrem) {
+            // The listed grains' candidate lists, concatenated in lane order,
+            // are walked 64 entries a step: stream entry e belongs to the
+            // last listed grain whose first entry excl is at most e, and is
+            // its list entry cl_list[p0 + e - excl].
+            const uint32_t fl = listed ? (uint32_t)len : 0u;
+            const uint32_t incl = wave_inclusive_sum(fl);
+            const int total = lane_i((int)incl, 63);
+            const int excl = (int)(incl - fl);
+            const int dl = p0 - excl;
+            const float rs2 = rad * rad;
+            int carry = 0;  // the last grain that starts before the step
+            for (int s0 = 0; s0 < total; s0 += 64) {
+                const int e = s0 + lane;
+                int owner = carry;
+                while (lrem) {
+                    const int g = __ffsll((long long)lrem) - 1;
+                    const int ex = lane_i(excl, g);
+                    if (ex >= s0 + 64) break;
+                    owner = e >= ex ? g : owner;
+                    carry = g;
+                    lrem &= lrem - 1ull;
                 }
+                const bool ok = e < total;
+                // lanes past the stream read its last entry and test nothing
+                const int pos = __shfl(dl, owner, 64) + min(e, total - 1);
+                const float obx = __shfl(bx, owner, 64), oby = __shfl(by, owner, 64);
+                const float ocx = __shfl(G.x, owner, 64), ocy = __shfl(G.y, owner, 64);
+                const float ors2 = __shfl(rs2, owner, 64);
+                const int p = (int)st.cl_list[pos];
+                test(ok, sxo[p], sw[p], obx, oby, ocx, ocy, ors2);
             }
-        };
-        if (m) prefetch(__ffsll((long long)m) - 1);
+        }
         while (m) {
             const int l = __ffsll((long long)m) - 1;
             m &= m - 1ull;
-            const int ca = na, cb = nb;
-            if (m) prefetch(__ffsll((long long)m) - 1);
             const float ubx = lane_f(bx, l), uby = lane_f(by, l), urz = lane_f(rz, l);
             const int up0 = lane_i(p0, l), ulen = lane_i(len, l);
             // A box narrower than one pixel (2 rz < 1) holds at most one
@@ -901,44 +942,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             // two 64-sample halves per iteration: both halves' LDS reads are
             // issued before either is tested
             if (direct) {
-                const float ucx = lane_f(gcx, l), ucy = lane_f(gcy, l);
-                // A pixel whose point passes the disk test lies within rz of
-                // the sample's target (cxk, cyk) in each axis (the candidate
-                // box is a superset); with rz < 0.5 that is only the pixel
-                // nearest the target.  So the drain's test (same f32
-                // expression) of that one pixel, when it lies in the tile,
-                // sets exactly the bits the queue would have set.
-                const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
-                auto test = [&](int i, float4 XO, uint2 wm) {
-                    const float px = rintf(ubx + XO.x), py = rintf(uby + XO.y);
-                    const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
-                    if ((i < ulen) & ((unsigned)tx < (unsigned)tw) & ((unsigned)ty < (unsigned)th)) {
-                        const float xg = (px + 0.5f) * iz - XO.z;
-                        const float yg = (py + 0.5f) * iz - XO.w;
-                        const float dx = xg - ucx, dy = yg - ucy;
-                        if (dx * dx + dy * dy <= urs * urs) {
-                            // tile coordinates and the bitset's bytes are far
-                            // below 2^24
-                            const uint32_t px_i = __umul24((uint32_t)ty, (uint32_t)t.TW) + (uint32_t)tx;
-                            atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24(px_i, w4) + wm.x)), wm.y);
-                        }
-                    }
-                };
-                const bool ulisted = LISTS && ((lmask >> l) & 1ull);
+                const float ucx = lane_f(gcx, l), ucy = lane_f(gcy, l), urs2 = urs * urs;
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
-                    int pa, pb;
-                    if (ulisted && i0 == 0) {
-                        pa = ca;
-                        pb = cb;
-                    } else if (ulisted) {
-                        const uint32_t last = (uint32_t)(up0 + ulen - 1);
-                        pa = (int)st.cl_list[min((uint32_t)(up0 + ia), last)];
-                        pb = (int)st.cl_list[min((uint32_t)(up0 + ib), last)];
-                    } else {
-                        pa = position(up0 + ia);
-                        pb = position(up0 + ib);
-                    }
+                    const int pa = position(up0 + ia), pb = position(up0 + ib);
                     const float4 XOa = sxo[pa], XOb = sxo[pb];
                     const uint2 wa = sw[pa], wb = sw[pb];
                     // all the reads issue together, ahead of the tests (an
@@ -947,8 +954,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     // the branches after the dependent ones)
                     asm volatile("" ::"v"(XOa.z), "v"(XOa.w), "v"(wa.x), "v"(wa.y), "v"(XOb.z), "v"(XOb.w),
                                  "v"(wb.x), "v"(wb.y));
-                    test(ia, XOa, wa);
-                    if (i0 + 64 < ulen) test(ib, XOb, wb);
+                    test(ia < ulen, XOa, wa, ubx, uby, ucx, ucy, urs2);
+                    if (i0 + 64 < ulen) test(ib < ulen, XOb, wb, ubx, uby, ucx, ucy, urs2);
                 }
                 continue;
             }
@@ -969,7 +976,6 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     __syncthreads();
 
     // ---- popcount epilogue: one store per output pixel ----
-    const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
     for (int p = tid; p < tw * th; p += BLOCK) {
         const int ox = p % tw, oy = p / tw;
         const uint32_t *pw = bits + ((size_t)oy * a.tile_w + ox) * words;
