@@ -868,7 +868,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             const int dl = p0 - excl;
             const float rs2 = rad * rad;
             int carry = 0;  // the last grain that starts before the step
-            for (int s0 = 0; s0 < total; s0 += 64) {
+            // the owner of each lane's entry of the step from s0, and the
+            // entry's sorted position (lanes past the stream read its last
+            // entry and test nothing)
+            auto owner_of = [&](int s0) {
                 const int e = s0 + lane;
                 int owner = carry;
                 while (lrem) {
@@ -879,14 +882,24 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     carry = g;
                     lrem &= lrem - 1ull;
                 }
-                const bool ok = e < total;
-                // lanes past the stream read its last entry and test nothing
-                const int pos = __shfl(dl, owner, 64) + min(e, total - 1);
+                return owner;
+            };
+            auto entry = [&](int s0, int owner) {
+                return (int)st.cl_list[__shfl(dl, owner, 64) + min(s0 + lane, total - 1)];
+            };
+            // the next step's entries load while the current step is tested
+            int own = owner_of(0);
+            int pn = entry(0, own);
+            for (int s0 = 0; s0 < total; s0 += 64) {
+                const int p = pn, owner = own;
+                if (s0 + 64 < total) {
+                    own = owner_of(s0 + 64);
+                    pn = entry(s0 + 64, own);
+                }
                 const float obx = __shfl(bx, owner, 64), oby = __shfl(by, owner, 64);
                 const float ocx = __shfl(G.x, owner, 64), ocy = __shfl(G.y, owner, 64);
                 const float ors2 = __shfl(rs2, owner, 64);
-                const int p = (int)st.cl_list[pos];
-                test(ok, sxo[p], sw[p], obx, oby, ocx, ocy, ors2);
+                test(s0 + lane < total, sxo[p], sw[p], obx, oby, ocx, ocy, ors2);
             }
         }
         while (m) {
