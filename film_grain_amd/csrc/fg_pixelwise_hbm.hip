@@ -601,6 +601,13 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// v of lane l (a per-lane index, 0 <= l < 64); ba = 4 l, the byte address
+// ds_bpermute takes
+__device__ __forceinline__ int from_lane_i(int v, int ba) { return __builtin_amdgcn_ds_bpermute(ba, v); }
+__device__ __forceinline__ float from_lane_f(float v, int ba)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(ba, __float_as_int(v)));
+}
 
 // One output tile of one plane: the body of the single-plane kernel and of the
 // persistent batch kernel (which therefore give identical planes).  lds is the
@@ -884,8 +891,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 }
                 return owner;
             };
+            // (unsigned: list positions are non-negative, so the load takes
+            // a 32-bit offset from the list's base)
             auto entry = [&](int s0, int owner) {
-                return (int)st.cl_list[__shfl(dl, owner, 64) + min(s0 + lane, total - 1)];
+                const uint32_t pos = (uint32_t)from_lane_i(dl, owner << 2) + (uint32_t)min(s0 + lane, total - 1);
+                return (int)st.cl_list[pos];
             };
             // the next step's entries load while the current step is tested
             int own = owner_of(0);
@@ -896,10 +906,16 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     own = owner_of(s0 + 64);
                     pn = entry(s0 + 64, own);
                 }
-                const float obx = __shfl(bx, owner, 64), oby = __shfl(by, owner, 64);
-                const float ocx = __shfl(G.x, owner, 64), ocy = __shfl(G.y, owner, 64);
-                const float ors2 = __shfl(rs2, owner, 64);
-                test(s0 + lane < total, sxo[p], sw[p], obx, oby, ocx, ocy, ors2);
+                const int ba = owner << 2;
+                const float obx = from_lane_f(bx, ba), oby = from_lane_f(by, ba);
+                const float ocx = from_lane_f(G.x, ba), ocy = from_lane_f(G.y, ba);
+                const float ors2 = from_lane_f(rs2, ba);
+                const float4 XO = sxo[p];
+                const uint2 wm = sw[p];
+                // both reads issue before the test's branches (as in the
+                // per-grain walk)
+                asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
+                test(lane < total - s0, XO, wm, obx, oby, ocx, ocy, ors2);
             }
         }
         while (m) {
