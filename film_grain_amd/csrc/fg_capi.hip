@@ -287,8 +287,14 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
                                     {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
     const int q = sample_buckets(d->n_samples);
     const size_t words = (d->n_samples + 31) / 32;
-    const size_t limit = (size_t)ctx->max_lds - (size_t)splat_hbm_block(ctx) / 64 * fg::kQueueCap * 4;
+    const size_t queue = (size_t)splat_hbm_block(ctx) / 64 * fg::kQueueCap * 4;
+    const size_t limit = (size_t)ctx->max_lds - queue;
     auto fits = [&](int w, int h) { return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) <= limit; };
+    // room for two workgroups per CU (at N = 2048 one workgroup per CU made
+    // the splat 2x slower)
+    auto fits2 = [&](int w, int h) {
+        return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) + queue <= (size_t)ctx->max_lds / 2;
+    };
     if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 && ctx->cfg.pix_tile_w <= 128 &&
         ctx->cfg.pix_tile_h <= 256 && fits(ctx->cfg.pix_tile_w, ctx->cfg.pix_tile_h)) {
         *tw = ctx->cfg.pix_tile_w;
@@ -297,9 +303,9 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
         // The largest ladder tile of 128 to max(128, 512 zoom^2) output
         // pixels (about 512 input pixels: larger tiles leave too few
         // workgroups to balance) whose bitset takes at most 16 KB; else one
-        // with up to 32 KB (N = 2048: 16x8 beats 8x8 by 19%); else a smaller
-        // tile with at most 16 KB; else any that fits (measured on MI355X,
-        // 1080p zoom-1 planes, N = 16..2048).
+        // with up to 32 KB; else a smaller tile with at most 16 KB; each
+        // leaving room for two workgroups per CU; else any that fits
+        // (measured on MI355X, 1080p zoom-1 planes, N = 16..2048).
         const double cap_px = std::max(128.0, 512.0 * (double)d->zoom * (double)d->zoom);
         *tw = 0;
         for (int pass = 0; pass < 4 && *tw == 0; ++pass) {
@@ -307,9 +313,9 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
                 const int px = t[0] * t[1];
                 const size_t bits = (size_t)px * words * 4;
                 const bool in_cap = (double)px <= cap_px;
-                const bool ok = pass == 0   ? in_cap && px >= 128 && bits <= 16384
-                                : pass == 1 ? in_cap && px >= 128 && bits <= 32768
-                                : pass == 2 ? in_cap && bits <= 16384
+                const bool ok = pass == 0   ? in_cap && px >= 128 && bits <= 16384 && fits2(t[0], t[1])
+                                : pass == 1 ? in_cap && px >= 128 && bits <= 32768 && fits2(t[0], t[1])
+                                : pass == 2 ? in_cap && bits <= 16384 && fits2(t[0], t[1])
                                             : true;
                 if (ok && fits(t[0], t[1])) {
                     *tw = t[0];
@@ -502,14 +508,16 @@ hipError_t tile_order_sort(fg_ctx *ctx, int n, const TileOrder &t, hipStream_t s
 // more than one 64-sample step of its x-run.
 struct ListPlan {
     int q = 32, classes = 0, ew = 0;
+    // Lists for every table of at most 2048 samples (beyond, the room of
+    // N * span^2 entries per class grows past 100 MB).  Short lists pay as
+    // well: the splat packs the lists of consecutive grains into its 64-lane
+    // steps (C5 at N = 256, zoom 1: 26.9 ms per frame walking runs).
     size_t bytes(int n) const  // fill counters | (start, length) cells | entries
     {
-        if (classes == 0 || n <= 64 || n > 2048 || !use(n)) return 0;
+        if (classes == 0 || n < 1 || n > 2048) return 0;
         const size_t b = 256 + (size_t)classes * q * q * 8 + fg::splat_lists_entries(q, classes, ew, n) * 2;
         return (b + 255) & ~(size_t)255;
     }
-    bool use(int n) const { return (double)n * run_per_sample > 64.0; }
-    double run_per_sample = 0.0;
 };
 
 ListPlan list_plan(const fg::PixelArgs &a, float eps)
@@ -518,7 +526,6 @@ ListPlan list_plan(const fg::PixelArgs &a, float eps)
     const float slack = 4e-3f + 2e-6f * (float)std::max(a.out_w, a.out_h);
     lp.classes = std::max(1, std::min(32, (int)std::ceil(std::min(a.rm * a.zoom + slack, 0.5f) * 64.0f)));
     lp.ew = (int)std::ceil((2.0 * (double)eps + std::ldexp(1.0, -11)) * lp.q + 0.01);
-    lp.run_per_sample = 2.0 * ((double)a.r_const * a.zoom + slack) + 2.0 * eps;
     return lp;
 }
 

@@ -18,12 +18,14 @@
 //               grain's centre fractions): the samples whose box can hold a
 //               pixel column and row (splat_lists_kernel)
 //   splat       per output tile, every wave streams 64-grain chunks of the
-//               tile's bins on its own; for each grain the lanes walk its
-//               list, or else the one cyclic run of the sample order whose x
-//               fraction can put a pixel column in the grain's candidate box.
-//               A narrow grain with no cell window to check tests the pixel
-//               nearest each sample's target with the exact reference test
-//               in the lane; other grains queue the (grain, sample, pixel)
+//               tile's bins on its own.  The chunk's listed grains (narrow,
+//               no cell window to check) walk their lists concatenated, 64
+//               entries a step whatever the grains they belong to; each lane
+//               tests the pixel nearest its sample's target with the exact
+//               reference test.  Every other grain walks, on its own, the one
+//               cyclic run of the sample order whose x fraction can put a
+//               pixel column in its candidate box: narrow ones test in the
+//               lane as above, the rest queue the (grain, sample, pixel)
 //               candidates whose box meets the tile and drain the queue 64 at
 //               a time through the same test.  Hits go to an LDS bitset of
 //               (pixel, sample) coverage.
@@ -525,11 +527,14 @@ hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, 
     return hipGetLastError();
 }
 
+// Largest sample count whose splat tables hold 8-byte (word, bit) pairs.
+constexpr int kWordBitMaxN = 1024;
+
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {
     const size_t words = ((size_t)n_samples + 31) / 32;
     return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 16) +
-           align16((size_t)n_samples * 8) + align16((size_t)(qbuckets + 1) * 2);
+           align16((size_t)n_samples * (n_samples > kWordBitMaxN ? 2 : 8)) + align16((size_t)(qbuckets + 1) * 2);
 }
 
 
@@ -666,7 +671,9 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     // the direct test reads both with one 16-byte load.  Sample k's bit of
     // tile pixel i is bit k % 32 of bitset word i * words + k / 32; per
     // sorted position the LDS holds that word's byte offset for pixel 0 and
-    // the bit's mask, so a hit's address is one multiply-add.
+    // the bit's mask, so a hit's address is one multiply-add.  Above
+    // kWordBitMaxN samples it holds k alone (2 B instead of 8, so that two
+    // workgroups still share a CU) and the readers derive the pair.
     const int tile_words = a.tile_w * a.tile_h * words;
     const uint32_t w4 = (uint32_t)words * 4u;
     size_t off = 0;
@@ -674,8 +681,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     off += align16((size_t)tile_words * 4);
     float4 *sxo = reinterpret_cast<float4 *>(lds + off);
     off += align16((size_t)N * 16);
+    const bool compact = N > kWordBitMaxN;  // uniform
     uint2 *sw = reinterpret_cast<uint2 *>(lds + off);
-    off += align16((size_t)N * 8);
+    uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
+    off += align16((size_t)N * (compact ? 2 : 8));
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)(Q + 1) * 2);
     if (off > (size_t)a.lds_bytes) {
@@ -687,9 +696,20 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const float2 X = st.xy[k], O = st.o[k];
         sxo[k] = make_float4(X.x, X.y, O.x, O.y);
         const uint32_t ki = st.k[k];
-        sw[k] = make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
+        if (compact)
+            sk[k] = (uint16_t)ki;
+        else
+            sw[k] = make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
+    // the (word byte offset, bit mask) pair of sorted position p
+    auto word_bit = [&](int p) {
+        if (compact) {
+            const uint32_t ki = sk[p];
+            return make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
+        }
+        return sw[p];
+    };
     const float4 ext = *st.ext;
     // the tile's 64-grain chunks go to its waves on demand (the first NW
     // statically), so that a wave that drew sparse chunks takes more
@@ -724,7 +744,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
         const float4 XO = sxo[p];
         const float2 O = make_float2(XO.z, XO.w);
-        const uint2 wm = sw[p];
+        const uint2 wm = word_bit(p);
         const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
         const float yg = ((float)(t.Y0 + ty) + 0.5f) * iz - O.y;
         const float dx = xg - cx, dy = yg - cy;
@@ -911,7 +931,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 const float ocx = from_lane_f(G.x, ba), ocy = from_lane_f(G.y, ba);
                 const float ors2 = from_lane_f(rs2, ba);
                 const float4 XO = sxo[p];
-                const uint2 wm = sw[p];
+                const uint2 wm = word_bit(p);
                 // both reads issue before the test's branches (as in the
                 // per-grain walk)
                 asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
@@ -976,7 +996,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     const int ia = i0 + lane, ib = ia + 64;
                     const int pa = position(up0 + ia), pb = position(up0 + ib);
                     const float4 XOa = sxo[pa], XOb = sxo[pb];
-                    const uint2 wa = sw[pa], wb = sw[pb];
+                    const uint2 wa = word_bit(pa), wb = word_bit(pb);
                     // all the reads issue together, ahead of the tests (an
                     // empty asm that takes the values as operands keeps the
                     // compiler from moving the offset and index reads into

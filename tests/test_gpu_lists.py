@@ -1,10 +1,10 @@
 """The splat's candidate lists at their boundaries.  Lists are kept per
-radius class (rz < (k + 1)/64 output px) and only for narrow grains
-(rz < 0.5) whose x-run exceeds one 64-sample step; everything else walks the
-run.  Constant radii placed just either side of class edges and of the
-narrow limit must give the oracle's planes bit for bit, and adaptive renders
-whose levels straddle the one-step threshold (lists for some levels, runs for
-others) must give each pixel the fixed render at its budget."""
+radius class (rz < (k + 1)/64 output px), only for narrow grains (rz < 0.5)
+and only for tables of at most 2048 samples; everything else walks the
+x-run.  Constant radii placed just either side of class edges and of the
+narrow limit must give the oracle's planes bit for bit (N = 4096: runs
+only), and adaptive renders must give each pixel the fixed render at its
+budget."""
 import numpy as np
 import pytest
 
@@ -17,7 +17,10 @@ pytestmark = pytest.mark.gpu
 SLACK = lambda out_side: 4e-3 + 2e-6 * out_side  # noqa: E731  (the splat kernel's widening of rz)
 
 
-@pytest.mark.parametrize("zoom,n", [(2.0, 256), (2.7, 1024), (1.0, 100)])
+# 1025 and above: past the largest N whose tables hold (word, bit) pairs;
+# 4096: past the largest N with lists
+@pytest.mark.parametrize("zoom,n", [(2.0, 256), (2.7, 1024), (1.0, 100), (1.5, 1025), (1.0, 2048), (1.0, 4096),
+                                    (1.0, 16)])
 def test_class_edges_and_narrow_limit(hip_ctx, oracle, zoom, n):
     rng = np.random.default_rng(int(zoom * 100) + n)
     w, h = 22, 17
@@ -40,9 +43,8 @@ def test_class_edges_and_narrow_limit(hip_ctx, oracle, zoom, n):
         assert bad == 0, case.name
 
 
-def test_adaptive_levels_with_and_without_lists(hip_ctx, oracle):
-    """zoom 1, r = 0.1: a grain's x-run is about 0.24 N samples, so the 128
-    and 256 levels walk runs and the 512 and 1024 levels walk lists."""
+def test_adaptive_levels_with_lists(hip_ctx, oracle):
+    """zoom 1, r = 0.1, levels 128 to 1024, each with its own lists."""
     import torch
     w, h = 40, 28
     u = synth.sinusoid_field(w, h, 77, n_waves=3)
