@@ -186,6 +186,26 @@ __device__ __forceinline__ bool tile_cancelled(const int *flag, int *status)
     return s_cancel != 0;
 }
 
+// Wave-wide inclusive sum.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// v of lane l (a per-lane index, 0 <= l < 64); ba = 4 l, the byte address
+// ds_bpermute takes
+__device__ __forceinline__ int from_lane_i(int v, int ba) { return __builtin_amdgcn_ds_bpermute(ba, v); }
+__device__ __forceinline__ float from_lane_f(float v, int ba)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(ba, __float_as_int(v)));
+}
+
 }  // namespace fg
 
 #endif

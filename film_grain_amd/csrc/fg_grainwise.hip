@@ -280,7 +280,7 @@ hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, 
 }
 
 template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTable st)
+__global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, SampleTable st)
 {
     constexpr int NW = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -439,32 +439,30 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
             gbx = bx;
             gby = by;
             gR = R;
-            for (int it = 0; __any(it < len); ++it) {
-                const int pr = p0 + it;
-                // a list entry, or the cyclic run position (lanes past their
-                // list or run read a valid sample and queue nothing)
-                const int p = a.cell_q > 0 ? (it < len ? (int)a.cell_list[pr] : 0)
-                                           : (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1));
+            // One candidate per lane: sample p (sorted position) against the
+            // grain of lane gl, whose target origin and output radius are
+            // (ebx, eby) and eR; ok false: nothing to test.
+            auto visit = [&](bool ok, int p, int gl, float ebx, float eby, float eR) {
                 const float2 X = sxy[p];
-                const float txp = bx + X.x, typ = by + X.y;
+                const float txp = ebx + X.x, typ = eby + X.y;
                 // bounds() of src/grainwise.rs:126-142 before its clamp; the
                 // tile lies inside the image, so meeting the tile is the clamp
-                const float xl = ceilf((txp - R) - 0.5f), xh = floorf((txp + R) - 0.5f);
-                const float yl = ceilf((typ - R) - 0.5f), yh = floorf((typ + R) - 0.5f);
-                const bool cand = (it < len) & (xl <= xh) & (yl <= yh) & (xl <= fx1) & (xh >= fx0) & (yl <= fy1) &
+                const float xl = ceilf((txp - eR) - 0.5f), xh = floorf((txp + eR) - 0.5f);
+                const float yl = ceilf((typ - eR) - 0.5f), yh = floorf((typ + eR) - 0.5f);
+                const bool cand = ok & (xl <= xh) & (yl <= yh) & (xl <= fx1) & (xh >= fx0) & (yl <= fy1) &
                                   (yh >= fy0);
-                const uint32_t base = (uint32_t)lane | ((uint32_t)p << 6);
+                const uint32_t base = (uint32_t)gl | ((uint32_t)p << 6);
                 const float cxl = fmaxf(xl, fx0), cxh = fminf(xh, fx1);
                 const float cyl = fmaxf(yl, fy0), cyh = fminf(yh, fy1);
                 const int tx = (int)(cxl - fx0), ty = (int)(cyl - fy0);
                 const bool multi = cand & ((cxh > cxl) | (cyh > cyl));
                 if (!__any(multi)) {
                     // one pixel per candidate: the drain's test of it, run
-                    // here on the lane's own grain (same f32 expression)
+                    // here in the lane (same f32 expression)
                     if (cand) {
                         const int k = sk[p];
                         const float dx = (cxl + 0.5f) - txp, dy = (cyl + 0.5f) - typ;
-                        if (dx * dx + dy * dy <= R * R)
+                        if (dx * dx + dy * dy <= eR * eR)
                             atomicOr(&bits[(ty * a.tile_w + tx) * words + (k >> 5)], 1u << (k & 31));
                     }
                 } else {
@@ -475,6 +473,36 @@ __global__ __launch_bounds__(BLOCK) void grainwise_sorted(GrainArgs a, SampleTab
                         const int jy = c ? j / bw : 0, jx = c ? j - jy * bw : 0;
                         push(c, base | ((uint32_t)(tx + jx) << 17) | ((uint32_t)(ty + jy) << 24));
                     }
+                }
+            };
+            if (a.cell_q > 0) {
+                // The lanes' candidate lists, concatenated in lane order, are
+                // walked 64 entries a step (the per-lane walk ran for the
+                // wave's longest list).  Entry e belongs to the first lane g
+                // whose running total incl_g exceeds e, i.e. g = the number
+                // of lanes with incl <= e (a binary search over the
+                // nondecreasing incl), and is its list entry p0 + e - excl.
+                const uint32_t incl = wave_inclusive_sum((uint32_t)len);
+                const int total = __builtin_amdgcn_readlane((int)incl, 63);
+                const int dl = p0 - ((int)incl - len);
+                for (int s0 = 0; s0 < total; s0 += 64) {
+                    const int e = s0 + lane;
+                    int g = 0;
+#pragma unroll
+                    for (int b = 32; b >= 1; b >>= 1) g += from_lane_i((int)incl, (g + b - 1) << 2) <= e ? b : 0;
+                    const bool ok = e < total;  // then g <= 63 and its list holds e
+                    const int ga = g << 2;
+                    const uint32_t pos = (uint32_t)(from_lane_i(dl, ga) + e);
+                    const int p = ok ? (int)a.cell_list[pos] : 0;
+                    visit(ok, p, g, from_lane_f(bx, ga), from_lane_f(by, ga), from_lane_f(R, ga));
+                }
+            } else {
+                for (int it = 0; __any(it < len); ++it) {
+                    // the cyclic run position (lanes past their run read a
+                    // valid sample and queue nothing)
+                    const int pr = p0 + it;
+                    visit(it < len, (int)min(min((uint32_t)pr, (uint32_t)(pr - N)), (uint32_t)(N - 1)), lane, bx, by,
+                          R);
                 }
             }
             // the queue refers to the lanes' current grains

@@ -128,18 +128,6 @@ __device__ __forceinline__ int compact_cells(const unsigned long long m[4], uint
     return n;
 }
 
-// Wave-wide inclusive sum.
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
-
 // Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
 // its 256 cells, the nonempty mask, and the grain count.
 __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
@@ -606,13 +594,6 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-// v of lane l (a per-lane index, 0 <= l < 64); ba = 4 l, the byte address
-// ds_bpermute takes
-__device__ __forceinline__ int from_lane_i(int v, int ba) { return __builtin_amdgcn_ds_bpermute(ba, v); }
-__device__ __forceinline__ float from_lane_f(float v, int ba)
-{
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(ba, __float_as_int(v)));
-}
 
 // One output tile of one plane: the body of the single-plane kernel and of the
 // persistent batch kernel (which therefore give identical planes).  lds is the
