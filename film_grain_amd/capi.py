@@ -36,7 +36,7 @@ EXPORTED = [
     "fg_mlp_budget_device", "fg_mlp_budget_scale_device", "fg_budget_scale_device",
     "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
-    "fg_version",
+    "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
 ALGO_PIXEL, ALGO_GRAIN = 0, 1
@@ -169,6 +169,12 @@ def lib() -> C.CDLL:
         getattr(L, name).restype = C.c_int
     L.fg_version.argtypes = []
     L.fg_version.restype = C.c_char_p
+    L.fg_pixelwise_grains.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp, vp]
+    L.fg_pixelwise_grains.restype = C.c_int
+    L.fg_grainwise_pixel_state.argtypes = [vp, vp, vp, vp, vp]
+    L.fg_grainwise_pixel_state.restype = C.c_int
+    L.fg_max_grains.argtypes = []
+    L.fg_max_grains.restype = C.c_uint64
     _lib = L
     return L
 
@@ -292,6 +298,34 @@ class Context:
         self._check(self._L.fg_log_f32_device(self.handle, C.c_void_p(x.data_ptr()), x.numel(),
                                               C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
 
+    # ---- generator inspection (parity tests) ----
+    def pixelwise_grains(self, desc: PlaneDesc, lam, offsets):
+        """Every grain record of the plane's HBM generation, as (n, 4) f32
+        (cx, cy, r, cell bits), with the extended grid's origin and size."""
+        import numpy as np
+        n = C.c_uint64(0)
+        origin = (C.c_int32 * 2)()
+        size = (C.c_uint32 * 2)()
+        args = (C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()))
+        rc = self._L.fg_pixelwise_grains(self.handle, *args, None, 0, C.byref(n), origin, size)
+        if rc not in (FG_OK, FG_ECAPACITY):
+            self._check(rc)
+        out = np.zeros((max(int(n.value), 1), 4), dtype=np.float32)
+        rc = self._L.fg_pixelwise_grains(self.handle, *args, C.c_void_p(out.ctypes.data), out.shape[0],
+                                         C.byref(n), origin, size)
+        self._check(rc)
+        return out[:int(n.value)], (origin[0], origin[1]), (size[0], size[1])
+
+    def grainwise_pixel_state(self, desc: PlaneDesc, lam):
+        """Per input pixel: the grain count and the Xoshiro state after the count draws."""
+        import numpy as np
+        npx = desc.in_w * desc.in_h
+        q = np.zeros(npx, dtype=np.uint32)
+        st = np.zeros((npx, 4), dtype=np.uint64)
+        self._check(self._L.fg_grainwise_pixel_state(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()),
+                                                     C.c_void_p(q.ctypes.data), C.c_void_p(st.ctypes.data)))
+        return q.reshape(desc.in_h, desc.in_w), st.reshape(desc.in_h, desc.in_w, 4)
+
     def cancel(self, on: bool = True):
         """Raise (or lower) the context's cancel flag: the integrators' tiles
         not yet started return at once and the status reports FG_ECANCELLED."""
@@ -301,8 +335,10 @@ class Context:
         self._check(self._L.fg_stream_status(self.handle, C.c_void_p(_stream_handle(stream))))
 
     def stats_device(self, algo: str, desc: PlaneDesc, lam, offsets, out, stream=None):
+        """Instrumented render.  Pixel-wise: [points, disk tests T, hits];
+        grain-wise: [grains, (grain, sample) pairs, disk tests, hits]."""
         fn = self._L.fg_pixelwise_stats_device if algo == "pixel" else self._L.fg_grainwise_stats_device
-        cnt = (C.c_uint64 * 3)()
+        cnt = (C.c_uint64 * (3 if algo == "pixel" else 4))()
         rc = fn(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
                 C.c_void_p(out.data_ptr()), cnt, C.c_void_p(_stream_handle(stream)))
         self._check(rc)

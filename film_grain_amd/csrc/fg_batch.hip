@@ -182,12 +182,12 @@ hipError_t launch_batch_order(const BatchPlane *planes, int n_planes, const uint
     return hipGetLastError();
 }
 
-__global__ void capacity_check_kernel(const uint32_t *total, uint32_t cap, int *status)
+__global__ void capacity_check_kernel(const unsigned long long *total, uint32_t cap, int *status)
 {
     if (*total > cap) atomicOr(status, kStatusCapacity);
 }
 
-hipError_t launch_capacity_check(const uint32_t *total, uint32_t cap, int *status, hipStream_t s)
+hipError_t launch_capacity_check(const unsigned long long *total, uint32_t cap, int *status, hipStream_t s)
 {
     hipLaunchKernelGGL(capacity_check_kernel, dim3(1), dim3(1), 0, s, total, cap, status);
     return hipGetLastError();

@@ -46,6 +46,7 @@ struct fg_ctx {
     // per-context device scratch: status word, stats counters, offsets range
     int *d_status = nullptr;
     unsigned long long *d_stats = nullptr;
+    unsigned long long *d_sum64 = nullptr;  // a plane's 64-bit grain total (bin_count)
     // cancel flag: coherent host memory that every tile of the integrators
     // reads once (host and device addresses)
     int *h_cancel = nullptr;
@@ -77,6 +78,10 @@ struct fg_ctx {
 };
 
 namespace {
+
+// Most grains one plane (or batch) may hold: grain indices, bin starts and
+// the splat's chunk indices (up to 63 past the last grain) are u32.
+constexpr unsigned long long kMaxGrains = 0xFFFFFF00ull;
 
 int set_error(fg_ctx *ctx, int code, const std::string &msg)
 {
@@ -540,48 +545,135 @@ hipError_t build_lists(const ListPlan &lp, fg::SampleTable &t, unsigned char *at
     return fg::launch_splat_lists(t, lp.q, lp.classes, lp.ew, fill, cs, cl, status, s);
 }
 
-// Per-plane generation into HBM, then the splat-from-HBM kernel.
-int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
-                  hipStream_t s)
+// Where a render's scratch comes from: the context's grow-only buffers
+// (arena == nullptr), or a fixed arena carved front to back (the
+// allocation-free entry point's static device array).
+struct ScratchSource {
+    unsigned char *arena = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+// bytes of scratch for one use: the context buffer b (grown if needed), or
+// the arena's next 256-byte-aligned block (FG_ECAPACITY when it is full).
+int scratch_get(fg_ctx *ctx, ScratchSource &src, DevBuf &b, size_t bytes, void **out, const char *what)
+{
+    if (!src.arena) {
+        const int rc = ensure_bytes(ctx, b, bytes, what);
+        *out = b.p;
+        return rc;
+    }
+    const size_t off = (src.used + 255) & ~(size_t)255;
+    if (off > src.cap || bytes > src.cap - off)
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": the plane needs more scratch than the allocation-free entry "
+                                             "point's " + std::to_string(src.cap >> 20) + " MiB");
+    *out = src.arena + off;
+    src.used = off + bytes;
+    return FG_OK;
+}
+
+// The per-plane generation of the HBM path: Poisson thresholds, per-bin
+// counts and nonempty masks, their scan, and every cell's grains written
+// contiguously per bin.
+struct HbmGen {
+    double *T = nullptr;
+    uint32_t *start = nullptr;
+    unsigned long long *mask = nullptr;
+    float4 *grains = nullptr;
+    unsigned long long total = 0;
+};
+
+int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::PixGeom &g, int dist_kind,
+                 hipStream_t s, HbmGen *out)
 {
     const char *what = "Pixel renderer";
-    int rc = scratch_acquire(ctx, what);
-    if (rc != FG_OK) return rc;
-    const int q = sample_buckets((uint32_t)a.n_samples);
-    if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes(a.n_samples, q), what)) != FG_OK) return rc;
-    fg::SampleTable st{};
-    splat_bounds(a, g, range, &st.eps, &st.safe_r);
+    int rc;
     const long long nbins = (long long)g.nbx * g.nby;
     const size_t npx = (size_t)a.in_w * a.in_h;
-    if ((rc = ensure_bytes(ctx, ctx->thresholds, npx * sizeof(double), what)) != FG_OK) return rc;
-    if ((rc = ensure_bytes(ctx, ctx->bin_count, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
-    if ((rc = ensure_bytes(ctx, ctx->bin_start, (size_t)(nbins + 1) * 4, what)) != FG_OK) return rc;
-    if ((rc = ensure_bytes(ctx, ctx->bin_mask, (size_t)nbins * 32, what)) != FG_OK) return rc;
+    void *T_p, *cnt_p, *start_p, *mask_p, *temp_p, *grains_p;
+    if ((rc = scratch_get(ctx, src, ctx->thresholds, npx * sizeof(double), &T_p, what)) != FG_OK) return rc;
+    if ((rc = scratch_get(ctx, src, ctx->bin_count, (size_t)(nbins + 1) * 4, &cnt_p, what)) != FG_OK) return rc;
+    if ((rc = scratch_get(ctx, src, ctx->bin_start, (size_t)(nbins + 1) * 4, &start_p, what)) != FG_OK) return rc;
+    if ((rc = scratch_get(ctx, src, ctx->bin_mask, (size_t)nbins * 32, &mask_p, what)) != FG_OK) return rc;
     size_t temp_bytes = 0;
     hipError_t e = fg::scan_bins(nullptr, &temp_bytes, nullptr, nullptr, (int)(nbins + 1), s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    if ((rc = ensure_bytes(ctx, ctx->scan_temp, temp_bytes, what)) != FG_OK) return rc;
-    double *T = static_cast<double *>(ctx->thresholds.p);
-    uint32_t *cnt = static_cast<uint32_t *>(ctx->bin_count.p);
-    uint32_t *start = static_cast<uint32_t *>(ctx->bin_start.p);
-    unsigned long long *mask = static_cast<unsigned long long *>(ctx->bin_mask.p);
-    mark(ctx, 0, s);
-    e = fg::launch_threshold(a, T, s);
-    if (e == hipSuccess) e = hipMemsetAsync(cnt + nbins, 0, 4, s);
+    if ((rc = scratch_get(ctx, src, ctx->scan_temp, temp_bytes, &temp_p, what)) != FG_OK) return rc;
+    double *T = static_cast<double *>(T_p);
+    uint32_t *cnt = static_cast<uint32_t *>(cnt_p);
+    uint32_t *start = static_cast<uint32_t *>(start_p);
+    unsigned long long *mask = static_cast<unsigned long long *>(mask_p);
+    // the plane's grain total, summed in 64 bits beside the u32 bin counts
+    // (cleared by a kernel: the caller may capture this into a graph)
+    a.grain_sum = ctx->d_sum64;
+    e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->d_sum64), 2, 0u, s);
+    if (e == hipSuccess) e = fg::launch_fill_u32(cnt + nbins, 1, 0u, s);
+    if (e == hipSuccess) e = fg::launch_threshold(a, T, s);
     if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, mask, s);
-    temp_bytes = ctx->scan_temp.cap;
-    if (e == hipSuccess) e = fg::scan_bins(ctx->scan_temp.p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
-    uint32_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, start + nbins, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = fg::scan_bins(temp_p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
+    unsigned long long total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, ctx->d_sum64, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    if ((rc = ensure_bytes(ctx, ctx->grains, std::max<size_t>(total, 1) * sizeof(float4), what)) != FG_OK)
+    // bin starts are u32 offsets: a plane of 2^32 - 256 or more grains (a
+    // large cell size or a bright plane) is refused before any kernel reads
+    // them (the splat's chunk indices run up to 63 past the last grain)
+    if (total > kMaxGrains)
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": the plane has " + std::to_string(total) + " grains, more than the " +
+                             std::to_string(kMaxGrains) + " one render addresses; use a smaller cell size or "
+                             "split the plane");
+    if ((rc = scratch_get(ctx, src, ctx->grains, std::max<size_t>((size_t)total, 1) * sizeof(float4), &grains_p,
+                          what)) != FG_OK)
         return rc;
-    float4 *grains = static_cast<float4 *>(ctx->grains.p);
-    TileOrder order;
-    if ((rc = tile_order_prepare(ctx, a.n_tiles, &order, s, what)) != FG_OK) return rc;
+    float4 *grains = static_cast<float4 *>(grains_p);
     e = fg::launch_bin_grains(a, g, dist_kind, T, start, mask, grains, s);
-    if (e == hipSuccess) e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, ctx->samples.p, &st, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    out->T = T;
+    out->start = start;
+    out->mask = mask;
+    out->grains = grains;
+    out->total = total;
+    return FG_OK;
+}
+
+// Per-plane generation into HBM, then the splat-from-HBM kernel.  With an
+// arena, every scratch buffer comes from it and nothing is allocated.
+int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
+                  hipStream_t s, ScratchSource src = ScratchSource())
+{
+    const char *what = "Pixel renderer";
+    int rc = FG_OK;
+    // the context's buffers are shared by its calls: wait for the previous
+    // user (an arena's callers are told that overlapping calls race)
+    if (!src.arena && (rc = scratch_acquire(ctx, what)) != FG_OK) return rc;
+    const int q = sample_buckets((uint32_t)a.n_samples);
+    void *samples_p;
+    if ((rc = scratch_get(ctx, src, ctx->samples, fg::sample_table_bytes(a.n_samples, q), &samples_p, what)) != FG_OK)
+        return rc;
+    fg::SampleTable st{};
+    splat_bounds(a, g, range, &st.eps, &st.safe_r);
+    mark(ctx, 0, s);
+    HbmGen gen;
+    if ((rc = hbm_generate(ctx, src, a, g, dist_kind, s, &gen)) != FG_OK) return rc;
+    uint32_t *start = gen.start;
+    const float4 *grains = gen.grains;
+    // heaviest-first tile order: weights and ids (in, out), sort workspace
+    TileOrder order;
+    void *tiles_p, *tile_temp_p;
+    size_t sort_temp = 0;
+    hipError_t e = fg::sort_tiles(nullptr, &sort_temp, nullptr, nullptr, nullptr, nullptr, a.n_tiles, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = scratch_get(ctx, src, ctx->tiles, (size_t)a.n_tiles * 16, &tiles_p, what)) != FG_OK) return rc;
+    if ((rc = scratch_get(ctx, src, ctx->tile_temp, sort_temp, &tile_temp_p, what)) != FG_OK) return rc;
+    {
+        unsigned char *p = static_cast<unsigned char *>(tiles_p);
+        order.w = reinterpret_cast<uint32_t *>(p);
+        order.w_sorted = reinterpret_cast<uint32_t *>(p + (size_t)a.n_tiles * 4);
+        order.ids = reinterpret_cast<int *>(p + (size_t)a.n_tiles * 8);
+        order.order = reinterpret_cast<int *>(p + (size_t)a.n_tiles * 12);
+    }
+    e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, samples_p, &st, s);
     // candidate lists for the direct test, one set per sample table
     const ListPlan lp = list_plan(a, st.eps);
     std::vector<int> &ln = ctx->level_n;
@@ -592,13 +684,15 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     ln.push_back(a.n_samples);
     size_t lists_total = 0;
     for (int l : ln) lists_total += lp.bytes(l);
+    unsigned char *lists_at = nullptr;
     if (lists_total > 0 && lists_total < ((size_t)1 << 32)) {
-        if ((rc = ensure_bytes(ctx, ctx->cells, lists_total, what)) != FG_OK) return rc;
+        void *cells_p;
+        if ((rc = scratch_get(ctx, src, ctx->cells, lists_total, &cells_p, what)) != FG_OK) return rc;
+        lists_at = static_cast<unsigned char *>(cells_p);
         a.cl_q = lp.q;
         a.cl_classes = lp.classes;
         a.cl_inv_dz = (float)(2 * lp.q);
     }
-    unsigned char *lists_at = static_cast<unsigned char *>(ctx->cells.p);
     auto build_lists_for = [&](fg::SampleTable &t) -> hipError_t {
         const size_t b = lp.bytes(t.n);
         if (a.cl_q == 0 || b == 0) return hipSuccess;
@@ -608,8 +702,9 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     };
     if (e == hipSuccess && !a.budget) {
         // the splat kernel reads its one table from device memory
-        if ((rc = ensure_bytes(ctx, ctx->levels, 256, what)) != FG_OK) return rc;
-        fg::SampleTable *d_table = static_cast<fg::SampleTable *>(ctx->levels.p);
+        void *levels_p;
+        if ((rc = scratch_get(ctx, src, ctx->levels, 256, &levels_p, what)) != FG_OK) return rc;
+        fg::SampleTable *d_table = static_cast<fg::SampleTable *>(levels_p);
         e = build_lists_for(st);
         if (e == hipSuccess) e = fg::launch_store_table(st, d_table, s);
         a.level_tables = d_table;
@@ -621,8 +716,10 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
         size_t bytes = 0;
         for (int l : ln) bytes += (fg::sample_table_bytes(l, sample_buckets((uint32_t)l)) + 255) & ~(size_t)255;
         const size_t head = 256 + (((size_t)nl * sizeof(fg::SampleTable) + 255) & ~(size_t)255);
-        if ((rc = ensure_bytes(ctx, ctx->levels, head + bytes + (size_t)a.n_tiles, what)) != FG_OK) return rc;
-        unsigned char *base = static_cast<unsigned char *>(ctx->levels.p);
+        void *levels_p;
+        if ((rc = scratch_get(ctx, src, ctx->levels, head + bytes + (size_t)a.n_tiles, &levels_p, what)) != FG_OK)
+            return rc;
+        unsigned char *base = static_cast<unsigned char *>(levels_p);
         int *d_level_n = reinterpret_cast<int *>(base);
         fg::SampleTable *d_tables = reinterpret_cast<fg::SampleTable *>(base + 256);
         unsigned char *p = base + head;
@@ -651,7 +748,8 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
         if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
     if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a, g, start, order.w, order.ids, s);
-    if (e == hipSuccess) e = tile_order_sort(ctx, a.n_tiles, order, s);
+    if (e == hipSuccess)
+        e = fg::sort_tiles(tile_temp_p, &sort_temp, order.w, order.w_sorted, order.ids, order.order, a.n_tiles, s);
     a.tile_list = order.order;
     mark(ctx, 1, s);
     mark(ctx, 2, s);
@@ -659,7 +757,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     if (e == hipSuccess) e = fg::launch_splat_hbm(a, g, st, start, grains, block, s);
     mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    return scratch_release(ctx, s, what);
+    return src.arena ? FG_OK : scratch_release(ctx, s, what);
 }
 
 // Launch arguments of one pixel-wise plane (all but the launch order and the
@@ -704,6 +802,7 @@ int pixel_args(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const
     a.r_const = rc;
     a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
+    a.grain_sum = nullptr;
     a.grain_total = nullptr;
     a.grain_cap = 0;
     *out = a;
@@ -748,6 +847,8 @@ int do_pixelwise_budget(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lamb
         return set_error(ctx, FG_EINVAL, "sample count too large for one pixel-wise tile");
     if (variant == 3) return run_pixel_hbm(ctx, a, geom, range, (int)d->dist_kind, s);
     mark(ctx, 0, s);
+    e = fg::launch_lambda_check(d_lambda, (long long)d->in_w * d->in_h, a.status, s);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     e = fg::launch_pixelwise(a, (int)d->dist_kind, variant == 1 ? 1 : 0, stats, s);
@@ -901,6 +1002,8 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     hipError_t e = fg::launch_offsets_range(a.offs, a.n_samples, ctx->d_range, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     mark(ctx, 0, s);
+    if (stats) e = fg::launch_grain_count(a, (int)d->dist_kind, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     e = fg::launch_grainwise(a, (int)d->dist_kind, stats, s);
@@ -922,6 +1025,13 @@ int status_error(fg_ctx *ctx, int status, const char *what)
         return set_error(ctx, FG_EINVAL,
                          std::string(what) + ": the cell region of one tile exceeds 65535 cells per side; "
                                              "use a larger cell size");
+    if (status & fg::kStatusLambda)
+        return set_error(ctx, FG_EINVAL,
+                         std::string(what) + ": lambda values must be finite and at most 1e6 (MAX_LAMBDA); "
+                                             "NaN, +inf and larger values were skipped");
+    if (status & fg::kStatusOverflow)
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": a plane's grain total reached 2^32; use a smaller cell size");
     if (status & fg::kStatusCapacity)
         return set_error(ctx, FG_ECAPACITY,
                          std::string(what) + ": planned capacity exceeded (a batch's grains, or the splat's "
@@ -1009,7 +1119,7 @@ int render_host(fg_ctx *ctx, const fg_plane_desc *d, const float *lambda, const 
 }
 
 int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const float *d_offs,
-                 float *d_out, uint64_t counters[3], void *stream, DeviceFn fn, const char *what)
+                 float *d_out, uint64_t *counters, int n_counters, void *stream, DeviceFn fn, const char *what)
 {
     int rc = validate(ctx, d);
     if (rc != FG_OK) return rc;
@@ -1025,7 +1135,7 @@ int stats_device(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
     e = hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     rc = read_status(ctx, s, what);
-    for (int i = 0; i < 3; ++i) counters[i] = h[i];
+    for (int i = 0; i < n_counters; ++i) counters[i] = h[i];
     return rc;
 }
 
@@ -1109,6 +1219,7 @@ struct fg_batch {
     float4 *ranges = nullptr;
     int *status = nullptr;
     uint32_t *cnt = nullptr, *start = nullptr, *partials = nullptr, *weights = nullptr, *counter = nullptr;
+    unsigned long long *total64 = nullptr;  // the batch's grain total in 64 bits
     unsigned long long *mask = nullptr;
     int2 *items = nullptr;
     float4 *grains = nullptr;
@@ -1150,6 +1261,7 @@ void batch_free(fg_batch *b)
 hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
 {
     hipError_t e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->status), 1, 0u, s);
+    if (e == hipSuccess) e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->total64), 2, 0u, s);
     for (int p = 0; p < b->n && e == hipSuccess; ++p) {
         const fg::BatchPlane &bp = b->planes[p];
         e = fg::launch_threshold(bp.a, b->T[p], s);
@@ -1164,7 +1276,7 @@ hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
 hipError_t batch_enqueue(fg_batch *b, hipStream_t s)
 {
     hipError_t e = batch_enqueue_counts(b, s);
-    if (e == hipSuccess) e = fg::launch_capacity_check(b->start + b->total_bins, b->cap, b->status, s);
+    if (e == hipSuccess) e = fg::launch_capacity_check(b->total64, b->cap, b->status, s);
     for (int p = 0; p < b->n && e == hipSuccess; ++p) {
         const fg::BatchPlane &bp = b->planes[p];
         e = fg::launch_bin_grains(bp.a, bp.geo, b->kind[p], b->T[p], bp.bin_start, b->mask + 4 * b->bin_off[p],
@@ -1275,21 +1387,27 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     if ((rc = batch_alloc(b, (void **)&b->weights, (size_t)b->n_items * 4, what)) != FG_OK) return rc;
     if ((rc = batch_alloc(b, (void **)&b->items, (size_t)b->n_items * sizeof(int2), what)) != FG_OK) return rc;
     if ((rc = batch_alloc(b, (void **)&b->counter, 4, what)) != FG_OK) return rc;
+    if ((rc = batch_alloc(b, (void **)&b->total64, 16, what)) != FG_OK) return rc;
+    for (int p = 0; p < n; ++p) b->planes[p].a.grain_sum = b->total64;
     if ((rc = batch_alloc(b, (void **)&b->d_planes, sizeof(fg::BatchPlane) * n, what)) != FG_OK) return rc;
 
     // counting pass over the current inputs sizes the grain array
     e = batch_enqueue_counts(b, s);
-    uint32_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, b->start + nb, 4, hipMemcpyDeviceToHost, s);
+    unsigned long long total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, b->total64, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
+    if (total > kMaxGrains)
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": the batch has " + std::to_string(total) + " grains, more than the " +
+                             std::to_string(kMaxGrains) + " one batch addresses; split it");
     const double want = (double)total * 1.25 + 65536.0;
-    b->cap = (uint32_t)std::min(want, 4294967295.0);
+    b->cap = (uint32_t)std::min(want, (double)kMaxGrains);
     if ((rc = batch_alloc(b, (void **)&b->grains, (size_t)b->cap * sizeof(float4), what)) != FG_OK) return rc;
     for (int p = 0; p < n; ++p) {
         fg::BatchPlane &bp = b->planes[p];
         bp.a.lds_bytes = b->lds;
-        bp.a.grain_total = b->start + nb;
+        bp.a.grain_total = b->total64;
         bp.a.grain_cap = b->cap;
         bp.bin_start = b->start + b->bin_off[p];
         bp.grains = b->grains;
@@ -1339,6 +1457,7 @@ int fg_ctx_create(int device_ordinal, fg_ctx **out)
     }
     if (e == hipSuccess) e = hipMalloc(&ctx->d_status, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_sum64, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_range, sizeof(float4));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, sizeof(int));
@@ -1364,6 +1483,7 @@ void fg_ctx_destroy(fg_ctx *ctx)
     free_buffers(ctx);
     (void)hipFree(ctx->d_status);
     (void)hipFree(ctx->d_stats);
+    (void)hipFree(ctx->d_sum64);
     (void)hipFree(ctx->d_range);
     if (ctx->h_cancel) (void)hipHostFree(ctx->h_cancel);
     if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
@@ -1450,15 +1570,15 @@ int fg_pixelwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const floa
                               const float *d_offsets_input_xy, float *d_out, uint64_t counters[3],
                               void *hip_stream)
 {
-    return stats_device(ctx, desc, d_lambda, d_offsets_input_xy, d_out, counters, hip_stream,
+    return stats_device(ctx, desc, d_lambda, d_offsets_input_xy, d_out, counters, 3, hip_stream,
                         do_pixelwise, "Pixel renderer");
 }
 
 int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
-                              const float *d_offsets_xy, float *d_out, uint64_t counters[3],
+                              const float *d_offsets_xy, float *d_out, uint64_t counters[4],
                               void *hip_stream)
 {
-    return stats_device(ctx, desc, d_lambda, d_offsets_xy, d_out, counters, hip_stream,
+    return stats_device(ctx, desc, d_lambda, d_offsets_xy, d_out, counters, 4, hip_stream,
                         do_grainwise, "Grain renderer");
 }
 
@@ -1806,6 +1926,95 @@ int fg_mlp_adam_device(fg_ctx *ctx, float *d_params, const float *d_grad, float 
     const hipError_t e = fg::launch_mlp_adam(d_params, d_grad, d_m, d_v, lr, beta1, beta2, eps, bc1, bc2,
                                              reinterpret_cast<hipStream_t>(hip_stream));
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Adam step");
+}
+
+uint64_t fg_max_grains(void) { return kMaxGrains; }
+
+int fg_pixelwise_grains(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                        const float *d_offsets_input_xy, float *grains_xyrc, uint64_t capacity, uint64_t *n_grains,
+                        int32_t origin[2], uint32_t size[2])
+{
+    const char *what = "Pixel renderer";
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offsets_input_xy || !n_grains || !origin || !size || (capacity && !grains_xyrc))
+        return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    hipStream_t s = ctx->stream;
+    float4 range{};
+    e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offsets_input_xy), (int)desc->n_samples,
+                                 ctx->d_range, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&range, ctx->d_range, sizeof range, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    fg::PixGeom geom{};
+    int tw, th, lds;
+    if (!plane_geometry(desc, range, &geom) || !hbm_config(ctx, desc, &tw, &th, &lds, 0))
+        return set_error(ctx, FG_EINVAL, "the plane's cell grid does not fit the HBM generation");
+    fg::PixelArgs a{};
+    if ((rc = pixel_args(ctx, desc, d_lambda, d_offsets_input_xy, nullptr, tw, th, lds, ctx->d_range, ctx->d_status,
+                         &a)) != FG_OK)
+        return rc;
+    if ((rc = scratch_acquire(ctx, what)) != FG_OK) return rc;
+    ScratchSource src;
+    HbmGen gen;
+    if ((rc = hbm_generate(ctx, src, a, geom, (int)desc->dist_kind, s, &gen)) != FG_OK) return rc;
+    *n_grains = gen.total;
+    origin[0] = geom.Ix0;
+    origin[1] = geom.Jy0;
+    size[0] = (uint32_t)geom.W;
+    size[1] = (uint32_t)geom.H;
+    if (gen.total <= capacity && gen.total > 0)
+        e = hipMemcpyAsync(grains_xyrc, gen.grains, (size_t)gen.total * sizeof(float4), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if ((rc = scratch_release(ctx, s, what)) != FG_OK) return rc;
+    rc = read_status(ctx, s, what);
+    if (rc == FG_OK && gen.total > capacity)
+        return set_error(ctx, FG_ECAPACITY, "grain buffer holds fewer records than the plane has");
+    return rc;
+}
+
+int fg_grainwise_pixel_state(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda, uint32_t *q_out,
+                             uint64_t *state_out)
+{
+    const char *what = "Grain renderer";
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !q_out || !state_out) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    hipStream_t s = ctx->stream;
+    if ((rc = scratch_acquire(ctx, what)) != FG_OK) return rc;
+    const size_t npx = (size_t)desc->in_w * desc->in_h;
+    if ((rc = ensure_bytes(ctx, ctx->pix_state, npx * 36 + 32, what)) != FG_OK) return rc;
+    unsigned char *ps = static_cast<unsigned char *>(ctx->pix_state.p);
+    fg::GrainArgs a{};
+    a.lambda = d_lambda;
+    a.status = ctx->d_status;
+    a.seed = desc->seed;
+    a.in_w = (int)desc->in_w;
+    a.in_h = (int)desc->in_h;
+    a.pix_s01 = reinterpret_cast<ulonglong2 *>(ps);
+    a.pix_s23 = reinterpret_cast<ulonglong2 *>(ps + npx * 16);
+    a.pix_q = reinterpret_cast<uint32_t *>(ps + npx * 32);
+    std::vector<uint64_t> s01(2 * npx), s23(2 * npx);
+    e = fg::launch_pixel_state(a, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(q_out, a.pix_q, npx * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(s01.data(), a.pix_s01, npx * 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(s23.data(), a.pix_s23, npx * 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    for (size_t p = 0; p < npx; ++p) {
+        state_out[4 * p] = s01[2 * p];
+        state_out[4 * p + 1] = s01[2 * p + 1];
+        state_out[4 * p + 2] = s23[2 * p];
+        state_out[4 * p + 3] = s23[2 * p + 1];
+    }
+    if ((rc = scratch_release(ctx, s, what)) != FG_OK) return rc;
+    return read_status(ctx, s, what);
 }
 
 const char *fg_version(void) { return "film_grain_amd 0.1.0 (gfx950)"; }

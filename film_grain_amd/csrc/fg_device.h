@@ -58,10 +58,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
 // Poisson threshold of one lambda value for the pixel-wise cells
 // (src/pixelwise.rs:69-82): 2.0 when the cell is skipped (lambda <= 0 or
 // lambda*delta*delta <= 0), exp(-expected) for the Knuth branch (expected < 12),
-// and -expected for the rejection branch.
+// and -expected for the rejection branch.  Values outside the accepted range
+// (NaN, above kMaxLambda) are skipped too; lambda_invalid flags them.
+__device__ __forceinline__ bool lambda_invalid(float lam) { return !(lam <= kMaxLambda); }
 __device__ __forceinline__ double cell_threshold(float lam, float delta)
 {
-    if (!(lam > 0.0f)) return 2.0;
+    if (!(lam > 0.0f) || !(lam <= kMaxLambda)) return 2.0;
     const float expected = lam * delta * delta;
     if (!(expected > 0.0f)) return 2.0;
     if (expected < 12.0f) return exp(-(double)expected);

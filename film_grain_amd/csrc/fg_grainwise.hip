@@ -83,7 +83,11 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
     const bool tester = tid < KL * GS;
 
     const float uscale1 = 1.0f;  // Uniform::new(0.0f32, 1.0) keeps scale 1
-    unsigned long long st_grains = 0, st_pairs = 0, st_hits = 0;
+    // instrumented twin: disk tests and hits of the tile's own pixels (each
+    // output pixel lies in one tile, so the sums over tiles are the
+    // reference's); grains and pairs are counted once per grain by
+    // grain_count_kernel, since tiles regenerate their halo's grains
+    unsigned long long st_tests = 0, st_hits = 0;
     __syncthreads();
 
     for (int pb = 0; pb < npx; pb += kBlock) {
@@ -95,7 +99,8 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
         if (pix < npx) {
             const int x = xi0 + pix % rwi, y = yi0 + pix / rwi;
             const float lam = a.lambda[(size_t)y * a.in_w + x];
-            if (lam > 0.0f) {
+            if (lambda_invalid(lam)) atomicOr(a.status, kStatusLambda);
+            if (lam > 0.0f && lam <= kMaxLambda) {
                 rng.seed(mix3(a.seed, kPixelStream, x, y));
                 q = poisson_u32(rng, (double)lam);
             }
@@ -123,11 +128,9 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
                     const int nb = (int)s_slot_n[b];
                     for (int g = gs; g < nb; g += GS) {
                         const float4 G = slots[b * gpt + g];
-                        if (STATS && kl == 0) ++st_grains;
                         for (int k = kl; k < a.n_samples; k += KL) {
                             const float2 o = a.offs[k];
                             const float txp = G.x + o.x, typ = G.y + o.y;
-                            if (STATS) ++st_pairs;
                             int xmn, xmx, ymn, ymx;
                             if (!ref_bounds(txp, G.z, a.out_w, xmn, xmx)) continue;
                             if (!ref_bounds(typ, G.z, a.out_h, ymn, ymx)) continue;
@@ -143,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
                                 if (dy2 > G.w) continue;
                                 for (int ox = xmn; ox <= xmx; ++ox) {
                                     const float dx = ((float)ox + 0.5f) - txp;
+                                    if (STATS) ++st_tests;
                                     if (dx * dx + dy2 <= G.w) {
                                         atomicOr(&bits[((oy - Y0) * a.tile_w + (ox - X0)) * words + wk], mask);
                                         if (STATS) ++st_hits;
@@ -167,9 +171,8 @@ __global__ __launch_bounds__(kBlock) void grainwise_tiles(GrainArgs a)
         a.out[(size_t)(Y0 + oy) * a.out_w + (X0 + ox)] = (float)count * a.inv_samples;
     }
     if (STATS) {
-        atomicAdd(&a.stats[0], st_grains);
-        atomicAdd(&a.stats[1], st_pairs);
-        atomicAdd(&a.stats[2], st_hits);
+        atomicAdd(&a.stats[2], st_tests);
+        atomicAdd(&a.stats[3], st_hits);
     }
 }
 
@@ -204,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void pixel_state_kernel(GrainArgs a)
         Xoshiro rng;
         rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
         uint32_t q = 0;
-        if (lam > 0.0f) {
+        // values outside the accepted range are skipped and flagged (once per wave)
+        if (__ballot(lambda_invalid(lam)) != 0ull && (threadIdx.x & 63) == 0) atomicOr(a.status, kStatusLambda);
+        if (lam > 0.0f && lam <= kMaxLambda) {
             rng.seed(mix3(a.seed, kPixelStream, x, y));
             q = poisson_u32(rng, (double)lam);
         }
@@ -212,6 +217,48 @@ __global__ __launch_bounds__(kBlock) void pixel_state_kernel(GrainArgs a)
         a.pix_s01[p] = make_ulonglong2(rng.s0, rng.s1);
         a.pix_s23[p] = make_ulonglong2(rng.s2, rng.s3);
     }
+}
+
+// Instrumented twin, per input pixel: the grains the reference draws with a
+// positive output radius (src/grainwise.rs:46-66), each of which it pairs with
+// every sample.  stats[0] += grains, stats[1] += grains * N.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void grain_count_kernel(GrainArgs a)
+{
+    const long long n = (long long)a.in_w * a.in_h;
+    unsigned long long cnt = 0;
+    for (long long p = (long long)blockIdx.x * kBlock + threadIdx.x; p < n; p += (long long)gridDim.x * kBlock) {
+        const int y = (int)(p / a.in_w), x = (int)(p - (long long)y * a.in_w);
+        const float lam = a.lambda[p];
+        if (!(lam > 0.0f && lam <= kMaxLambda)) continue;
+        Xoshiro rng;
+        rng.seed(mix3(a.seed, kPixelStream, x, y));
+        const uint32_t q = poisson_u32(rng, (double)lam);
+        for (uint32_t g = 0; g < q; ++g) {
+            (void)uniform_f32(rng, 1.0f);
+            (void)uniform_f32(rng, 1.0f);
+            float rad = KIND == 0 ? a.r_const : lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
+            if (rad > a.rm) rad = a.rm;
+            if (rad <= 0.0f || rad * a.zoom <= 0.0f) continue;
+            ++cnt;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0 && cnt) {
+        atomicAdd(&a.stats[0], cnt);
+        atomicAdd(&a.stats[1], cnt * (unsigned long long)a.n_samples);
+    }
+}
+
+hipError_t launch_grain_count(const GrainArgs &a, int dist_kind, hipStream_t s)
+{
+    const long long n = (long long)a.in_w * a.in_h;
+    const unsigned blocks = (unsigned)std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, 8192));
+    if (dist_kind == 0)
+        hipLaunchKernelGGL(grain_count_kernel<0>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(grain_count_kernel<1>, dim3(blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_pixel_state(const GrainArgs &a, hipStream_t s)

@@ -17,6 +17,14 @@ constexpr int kStatusNeighbourhood = 1;  // one output pixel's cell window excee
 constexpr int kStatusCancelled = 2;
 constexpr int kStatusRegion = 4;         // a tile's cell region exceeds 65535 cells per side
 constexpr int kStatusCapacity = 8;       // a batch's grains exceed the planned capacity
+constexpr int kStatusLambda = 16;        // a lambda value is NaN, +inf or above kMaxLambda
+constexpr int kStatusOverflow = 32;      // a plane's grain total reaches 2^32
+
+// Largest lambda the integrators accept: lambda_plane caps every value at
+// MAX_LAMBDA (src/model.rs:11, 262).  Values above it, +inf and NaN (on which
+// Poisson::new fails in the reference) are skipped by the kernels, which never
+// run a Poisson draw on them, and reported as kStatusLambda.
+constexpr float kMaxLambda = 1.0e6f;
 
 // Pixel-wise integrator launch (src/pixelwise.rs:11-106).
 struct PixelArgs {
@@ -37,10 +45,14 @@ struct PixelArgs {
     int offs_in_lds;            // splat: stage the offsets table in LDS
     int div_fast;               // 1: floor((a)/delta) via the checked reciprocal sequence
     float inv_zoom, zoom, rm, delta, rdelta, uscale, r_const, r2_const, inv_samples;
+    // the plane's (or batch's) grain total in 64 bits, which bin_count adds
+    // each bin's sum to (nullptr: not summed); the per-bin counts and their
+    // scan are u32, valid while this total stays below 2^32
+    unsigned long long *grain_sum;
     // batches: the grain total (device) and the grain array's capacity; when
     // the total exceeds it the grain and splat kernels do nothing (the
     // capacity check reports it).  grain_total == nullptr: unchecked.
-    const uint32_t *grain_total;
+    const unsigned long long *grain_total;
     uint32_t grain_cap;
     // adaptive sampling (HBM splat only): per output pixel its sample count
     // n in [1, n_samples] (the first n offsets, as a render with N = n uses),
@@ -67,7 +79,7 @@ struct GrainArgs {
     float *out;
     int *status;
     const int *cancel;   // optional: the context's host-visible cancel flag
-    unsigned long long *stats;  // optional counters: grains, (grain, sample) pairs, hits
+    unsigned long long *stats;  // optional counters: grains, (grain, sample) pairs, disk tests, hits
     const float4 *off_range;    // device: (min x, max x, min y, max y) of offs
     uint64_t seed;
     double log_mu, log_sigma;
@@ -96,6 +108,9 @@ hipError_t launch_offsets_range(const float2 *offs, int n, float4 *range, hipStr
 hipError_t launch_pixelwise(const PixelArgs &a, int dist_kind, int variant, bool with_stats,
                             hipStream_t s);
 hipError_t launch_grainwise(const GrainArgs &a, int dist_kind, bool with_stats, hipStream_t s);
+// Instrumented twin: stats[0] += grains drawn with a positive output radius,
+// stats[1] += those grains * N (the reference's (grain, sample) pairs).
+hipError_t launch_grain_count(const GrainArgs &a, int dist_kind, hipStream_t s);
 
 // Host check that q0 = a*rd, r = fma(-q0, d, a), q = fma(r, rd, q0) equals the
 // IEEE quotient a/d for every a of one binade (hence, by scaling, for every a

@@ -60,6 +60,7 @@ hipError_t launch_store_table(const SampleTable &t, SampleTable *dst, hipStream_
 hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
                               uint16_t *list, int *status, hipStream_t s);
 
+// Also sets kStatusLambda in *a.status for lambda values outside the accepted range.
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s);
 // bin_mask: four 64-bit words per bin, the nonempty cells (bin_grains reads them).
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
@@ -132,8 +133,12 @@ hipError_t launch_batch_weights(const BatchPlane *planes, int n_planes, uint32_t
 // scratch holds 2 * 64 u32.
 hipError_t launch_batch_order(const BatchPlane *planes, int n_planes, const uint32_t *w, int n_items, int2 *items,
                               uint32_t *scratch, hipStream_t s);
-// Sets kStatusCapacity in *status when the batch's grain total exceeds cap.
-hipError_t launch_capacity_check(const uint32_t *total, uint32_t cap, int *status, hipStream_t s);
+// Sets kStatusCapacity in *status when the batch's grain total (64-bit, as
+// bin_count sums it) exceeds cap.
+hipError_t launch_capacity_check(const unsigned long long *total, uint32_t cap, int *status, hipStream_t s);
+// Sets kStatusLambda in *status when a lambda value is NaN, +inf or above
+// kMaxLambda (for the formulations whose kernels read lambda directly).
+hipError_t launch_lambda_check(const float *lambda, long long n, int *status, hipStream_t s);
 
 // LDS bytes the splat kernel needs beyond its static queues for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);

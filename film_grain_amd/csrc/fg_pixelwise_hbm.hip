@@ -32,17 +32,27 @@
 // A grain record is (cx, cy, r, cell) with cell = (i - Ix0) | (j - Jy0) << 16.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "fg_device.h"
 #include "fg_pixelwise_hbm.h"
 #include "fg_region.h"
 
 namespace fg {
 
+// Also flags lambda values outside the accepted range (kStatusLambda), once
+// per wave.
 __global__ __launch_bounds__(kBlock) void threshold_kernel(const float *lambda, long long n, float delta,
-                                                           double *T)
+                                                           double *T, int *status)
 {
     const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (p < n) T[p] = cell_threshold(lambda[p], delta);
+    bool bad = false;
+    if (p < n) {
+        const float lam = lambda[p];
+        bad = lambda_invalid(lam);
+        T[p] = cell_threshold(lam, delta);
+    }
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(status, kStatusLambda);
 }
 
 // Cells of a bin, one wave per bin: lane l holds the four cells of column
@@ -154,7 +164,8 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
         m[q] = __ballot(ne);
     }
     const int n = compact_cells(m, s_list[wid]);
-    uint32_t total = 0;
+    // summed in 64 bits: one cell's count alone can reach 2^32 - 1
+    unsigned long long total = 0;
     for (int base = 0; base < n; base += 64) {
         uint32_t c = 0;
         if (base + lane < n) {
@@ -166,9 +177,19 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
             Xoshiro rng;
             c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
         }
-        total += __shfl(wave_inclusive_sum(c), 63, 64);
+        unsigned long long x = c;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        total += x;
     }
-    if (lane == 0) bin_count[bin] = total;
+    if (lane == 0) {
+        // a bin of 2^32 or more grains saturates; the plane's 64-bit sum then
+        // reaches 2^32 as well, and the host refuses the plane before any
+        // kernel uses the (wrapped) u32 scan
+        bin_count[bin] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
+        if (a.grain_sum && total) atomicAdd(a.grain_sum, total);
+        if (total > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
+    }
     if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
 }
 
@@ -1087,11 +1108,27 @@ hipError_t launch_splat_batch(const BatchPlane *planes, const int2 *items, int n
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(kBlock) void lambda_check_kernel(const float *lambda, long long n, int *status)
+{
+    bool bad = false;
+    for (long long p = (long long)blockIdx.x * kBlock + threadIdx.x; p < n; p += (long long)gridDim.x * kBlock)
+        bad = bad || lambda_invalid(lambda[p]);
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(status, kStatusLambda);
+}
+
+hipError_t launch_lambda_check(const float *lambda, long long n, int *status, hipStream_t s)
+{
+    const long long blocks = std::min<long long>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(lambda_check_kernel, dim3((unsigned)std::max<long long>(blocks, 1)), dim3(kBlock), 0, s,
+                       lambda, n, status);
+    return hipGetLastError();
+}
+
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s)
 {
     const long long n = (long long)a.in_w * a.in_h;
     hipLaunchKernelGGL(threshold_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       a.lambda, n, a.delta, T);
+                       a.lambda, n, a.delta, T, a.status);
     return hipGetLastError();
 }
 

@@ -18,7 +18,14 @@
  * tests in f32 without contraction.
  *
  * Data contract (src/wgpu/mod.rs:17-38, 353-358, 490-495):
- *   lambda   in_h x in_w f32, row-major: the lambda_plane of one colour plane
+ *   lambda   in_h x in_w f32, row-major: the lambda_plane of one colour plane.
+ *            Accepted values are those lambda_plane produces, at most
+ *            MAX_LAMBDA = 1e6 (src/model.rs:11, 262).  Values <= 0 (and -inf)
+ *            are skipped, as the reference skips them (src/pixelwise.rs:73,
+ *            src/grainwise.rs:41).  NaN (on which the reference's Poisson::new
+ *            fails), +inf and values above 1e6 are skipped too, and the call
+ *            reports FG_EINVAL (at once for the host-memory entry points, from
+ *            fg_stream_status for the device ones); no kernel draws on them.
  *   offsets  n_samples (x, y) f32 pairs: Derived::offsets_input (input pixels)
  *            for the pixel-wise path, Derived::offsets (output pixels) for the
  *            grain-wise path
@@ -142,10 +149,13 @@ int fg_pixelwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const floa
                               const float *d_offsets_input_xy, float *d_out, uint64_t counters[3],
                               void *hip_stream);
 
-/* Instrumented grain-wise render: counters[0] = grains, counters[1] = (grain,
- * sample) pairs tested, counters[2] = bits set.  No reference counterpart. */
+/* Instrumented grain-wise render, counting the reference's own work
+ * (src/grainwise.rs:46-104): counters[0] = grains drawn with a positive
+ * output radius, counters[1] = (grain, sample) pairs = grains * N,
+ * counters[2] = disk tests (pixels of a pair's bounds() box in rows with
+ * dy^2 <= R^2), counters[3] = tests that pass.  No reference counterpart. */
 int fg_grainwise_stats_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
-                              const float *d_offsets_xy, float *d_out, uint64_t counters[3],
+                              const float *d_offsets_xy, float *d_out, uint64_t counters[4],
                               void *hip_stream);
 
 /* Launch shape used by the next renders on this context; 0 keeps the
@@ -190,7 +200,7 @@ int fg_get_timings(fg_ctx *ctx, float ms[3]);
  * nothing and makes fg_batch_status return FG_ECAPACITY (create the batch
  * again).  Each plane's result equals fg_render_pixelwise_device's bit for
  * bit.  A batch must not run concurrently with itself. */
-#define FG_ECAPACITY 7
+#define FG_ECAPACITY 7 /* also: a plane (or batch) with more than fg_max_grains() grains */
 typedef struct fg_batch fg_batch;
 int fg_batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n_planes, const float *const *d_lambdas,
                     const float *const *d_offsets_input_xy, float *const *d_outs, int capture_graph,
@@ -296,6 +306,38 @@ int fg_color_merge_device(fg_ctx *ctx, int color_mode, const float *d_p0, const 
 /* d_out[i] = logf(d_in[i]) exactly as the C library computes it, for inputs in
  * [1e-6, 1] (other inputs get a result within one unit in the last place). */
 int fg_log_f32_device(fg_ctx *ctx, const float *d_in, int64_t n, float *d_out, void *hip_stream);
+
+/* ---- Generator inspection (parity tests) --------------------------------
+ * No reference counterpart: they expose the device generators' state so that
+ * tests can compare it cell by cell with the reference's CPU path.
+ *
+ * fg_pixelwise_grains runs the pixel-wise HBM path's generation for one plane
+ * (per cell: cell_rng, the clamped lambda lookup, the Poisson count, then each
+ * grain's centre and radius, src/pixelwise.rs:66-100) and copies every grain
+ * record to host memory as 4 floats (cx, cy, r, cell), where cell holds the
+ * bits (i - origin[0]) | (j - origin[1]) << 16 of the grain's cell (i, j);
+ * each cell's grains are contiguous and in draw order.  origin and size get
+ * the plane's extended cell grid (first cell, cells per row and column);
+ * *n_grains the plane's grain count.  When that exceeds capacity nothing is
+ * copied and FG_ECAPACITY is returned.  Synchronous.
+ *
+ * fg_grainwise_pixel_state runs the grain-wise path's Poisson pass for one
+ * plane (pixel_rng and Poisson(lambda), src/grainwise.rs:37-45) and copies,
+ * per input pixel, the count to q_out[in_w * in_h] and the Xoshiro256++
+ * state after the count draws (from which the grain draws continue) to
+ * state_out[4 * in_w * in_h].  Synchronous. */
+int fg_pixelwise_grains(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                        const float *d_offsets_input_xy, float *grains_xyrc, uint64_t capacity,
+                        uint64_t *n_grains, int32_t origin[2], uint32_t size[2]);
+int fg_grainwise_pixel_state(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda, uint32_t *q_out,
+                             uint64_t *state_out);
+
+/* Most grains one pixel-wise plane (or one batch) may hold: grain indices,
+ * bin offsets and chunk indices are 32-bit.  The count pass sums the grains in
+ * 64 bits, and a plane or batch above this limit is refused with
+ * FG_ECAPACITY before any grain is written.  No reference counterpart (the CPU
+ * path never stores grains). */
+uint64_t fg_max_grains(void);
 
 /* Library version string. */
 const char *fg_version(void);

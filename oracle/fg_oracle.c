@@ -15,21 +15,27 @@
  *   xoshiro256++            rand_xoshiro reference outputs for state [1,2,3,4].
  *   seed_from_u64           equals the splitmix64 sequence (closed form).
  *   UniformFloat<f32>       exhaustive check over all 2^23 mantissas (range, scale).
- *   Standard/Open01 f64     closed forms; range checks.
+ *   Standard/Open01 f64,    driven with chosen next_u64 values: monotone in the
+ *   UniformFloat sample     source, on the 2^-53 (2^-23 * scale) lattice, odd
+ *                           multiples of 2^-53 for Open01, and independent of the
+ *                           low bits each one discards (tests/test_oracle_bits.py).
  *   log_gamma               against libm lgamma (a wrong coefficient fails).
  *   Poisson                 mean/variance and chi-square against the pmf for both
  *                           branches (Knuth < 12, Cauchy rejection >= 12).
  *   ziggurat tables         recurrence invariants and the decimal literals of the
  *                           published table; StandardNormal moments and KS test.
  *   ChaCha                  RFC 7539 A.1 keystream vectors (20 rounds).
+ *   PCG32 step and output   the published pcg32 demo vector (seed 42, stream
+ *                           54); rand_core's increment constant is unpinned.
  *   inverse normal cdf      published quantiles; Phi(Phi^-1(p)) = p.
  *   pixel-wise / grain-wise brute force on tiny images (every grain enumerated
  *                           without the window), coverage invariant E[out] = u on
  *                           flat fields, grain-wise/pixel-wise agreement of means.
  * Parity unpinned (no import of the Rust crates is possible here): the exact
  * decimal literals of rand_distr's ZIG_NORM tables beyond the entries checked,
- * rand_chacha's PCG32 seed fill, rand_distr's operation order inside log_gamma
- * and the rejection loop, and statrs' erfc_inv.
+ * rand_core's PCG32 increment constant in the StdRng seed fill, rand_distr's
+ * operation order inside log_gamma and the rejection loop, and statrs'
+ * erfc_inv.
  */
 #include "fg_oracle.h"
 
@@ -360,14 +366,24 @@ typedef struct {
     int index;
 } stdrng_t;
 
+/* PCG32 (O'Neill's XSH-RR output of a 64-bit LCG) as rand_core 0.6.4's
+ * seed_from_u64 runs it: the state is advanced first, then the new state is
+ * output.  The multiplier and the output function are pinned by the published
+ * pcg32 demo vector; the increment 11634580027462260723 is rand_core's
+ * constant and is parity unpinned (no published vector uses it). */
+static inline uint32_t pcg32_next(uint64_t *state, uint64_t inc)
+{
+    const uint64_t s = *state * 6364136223846793005ULL + inc;
+    *state = s;
+    const uint32_t xorshifted = (uint32_t)(((s >> 18) ^ s) >> 27);
+    const uint32_t rot = (uint32_t)(s >> 59);
+    return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
+uint32_t fgo_pcg32_next(uint64_t *state, uint64_t inc) { return pcg32_next(state, inc); }
+
 static void stdrng_seed_from_u64(stdrng_t *r, uint64_t state)
 {
-    for (int w = 0; w < 8; ++w) {
-        state = state * 6364136223846793005ULL + 11634580027462260723ULL;
-        const uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
-        const uint32_t rot = (uint32_t)(state >> 59);
-        r->key[w] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
-    }
+    for (int w = 0; w < 8; ++w) r->key[w] = pcg32_next(&state, 11634580027462260723ULL);
     r->counter = 0;
     r->index = 64;
 }
@@ -771,6 +787,23 @@ int fgo_cell_dump(const fgo_plane *p, const float *lambda, int32_t i, int32_t j,
     return (int)q;
 }
 
+/* The Poisson count of cell (i, j) alone (src/pixelwise.rs:66-82: cell_rng,
+ * the clamped lambda lookup, Poisson::new(lambda * delta * delta), `as u32`). */
+uint32_t fgo_cell_count(const fgo_plane *p, const float *lambda, int32_t i, int32_t j)
+{
+    const float delta = p->delta;
+    uint64_t s[4];
+    xo_seed(mix3(p->seed, FGO_CELL_STREAM, (int64_t)i, (int64_t)j), s);
+    const float sample_x = (float)i * delta, sample_y = (float)j * delta;
+    const float lambda_cell = lambda_get_clamped(p, lambda, sat_i64f(floorf(sample_x)), sat_i64f(floorf(sample_y)));
+    if (lambda_cell <= 0.0f) return 0;
+    const float expected = lambda_cell * delta * delta;
+    if (expected <= 0.0f) return 0;
+    poisson_t po;
+    poisson_new(&po, (double)expected);
+    return sat_u32(poisson_sample_xo(&po, s));
+}
+
 /* ------------------------------------------------------------------------- */
 /* Grain-wise integrator -- src/grainwise.rs                                   */
 /* ------------------------------------------------------------------------- */
@@ -793,6 +826,16 @@ static inline int bounds(float center, float radius, int32_t limit, int32_t *lo,
 
 int fgo_render_grainwise(const fgo_plane *p, const float *lambda, const float *offsets_xy,
                          float *out, uint64_t *bitsets_out, int threads, uint64_t *pairs_out)
+{
+    fgo_grain_counters c;
+    const int rc = fgo_render_grainwise_counted(p, lambda, offsets_xy, out, bitsets_out, threads, &c);
+    if (rc == 0 && pairs_out) *pairs_out = c.pairs;
+    return rc;
+}
+
+int fgo_render_grainwise_counted(const fgo_plane *p, const float *lambda, const float *offsets_xy,
+                                 float *out, uint64_t *bitsets_out, int threads,
+                                 fgo_grain_counters *counters)
 {
     if (!p || !lambda || !out) return -1;
     if (p->n_samples > 0 && !offsets_xy) return -1;
@@ -817,10 +860,10 @@ int fgo_render_grainwise(const fgo_plane *p, const float *lambda, const float *o
     const float zoom = p->zoom;
     const uint32_t nsamp = p->n_samples > 1 ? p->n_samples : 1;
     const float inv_samples = 1.0f / (float)nsamp;
-    uint64_t pairs = 0;
+    uint64_t grains = 0, pairs = 0, tests = 0, hits = 0;
 #ifdef _OPENMP
     if (threads <= 0) threads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(+ : pairs)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(+ : grains, pairs, tests, hits)
 #endif
     for (int64_t y = 0; y < (int64_t)p->in_h; ++y) {
         for (uint32_t x = 0; x < p->in_w; ++x) {
@@ -840,6 +883,7 @@ int fgo_render_grainwise(const fgo_plane *p, const float *lambda, const float *o
                 const float radius_out = radius * zoom;
                 if (radius_out <= 0.0f) continue;
                 const float radius_sq = radius_out * radius_out;
+                grains++;
                 for (uint32_t k = 0; k < p->n_samples; ++k) {
                     pairs++;
                     const float tx = (cx * zoom) + offsets_xy[2 * k];
@@ -857,9 +901,11 @@ int fgo_render_grainwise(const fgo_plane *p, const float *lambda, const float *o
                         for (int32_t ox = xmn; ox <= xmx; ++ox) {
                             const float center_x = (float)ox + 0.5f;
                             const float dx = center_x - tx;
+                            tests++;
                             if (dx * dx + dy_sq <= radius_sq) {
                                 const uint64_t idx = (uint64_t)oy * p->out_w + (uint64_t)ox;
                                 __atomic_fetch_or(&bits[idx * lanes + lane_idx], mask, __ATOMIC_RELAXED);
+                                hits++;
                             }
                         }
                     }
@@ -873,7 +919,12 @@ int fgo_render_grainwise(const fgo_plane *p, const float *lambda, const float *o
         out[px] = (float)count * inv_samples;
     }
     if (own) free(bits);
-    if (pairs_out) *pairs_out = pairs;
+    if (counters) {
+        counters->grains = grains;
+        counters->pairs = pairs;
+        counters->tests = tests;
+        counters->hits = hits;
+    }
     return 0;
 }
 

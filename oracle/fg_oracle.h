@@ -60,6 +60,7 @@ double fgo_lognormal_sample(uint64_t s[4], double mu, double sigma);
 void fgo_zig_tables(double x_out[257], double f_out[257]);
 void fgo_chacha_block(const uint32_t key[8], uint64_t counter, uint64_t stream,
                       int rounds, uint32_t out[16]);
+uint32_t fgo_pcg32_next(uint64_t *state, uint64_t inc);
 void fgo_stdrng_u64(uint64_t seed, uint32_t n, uint64_t *out);
 int fgo_make_offsets(uint64_t seed, uint32_t n, float sigma_px, float *out_xy);
 double fgo_inverse_normal_cdf(double p);
@@ -127,12 +128,26 @@ int fgo_render_grainwise(const fgo_plane *p, const float *lambda,
                          const float *offsets_xy, float *out, uint64_t *bitsets_out,
                          int threads, uint64_t *pairs_out);
 
+/* The same with the work counters: grains drawn with a positive output
+ * radius, (grain, sample) pairs (grains * N), disk tests (pixels of each
+ * pair's bounds() box in rows whose dy^2 <= r^2, src/grainwise.rs:82-100) and
+ * the tests that pass. */
+typedef struct {
+    uint64_t grains, pairs, tests, hits;
+} fgo_grain_counters;
+int fgo_render_grainwise_counted(const fgo_plane *p, const float *lambda,
+                                 const float *offsets_xy, float *out, uint64_t *bitsets_out,
+                                 int threads, fgo_grain_counters *counters);
+
 /* Per-cell dump of the pixel-wise generator: hash, seeded xoshiro state,
  * Poisson count and up to max_grains grains (cx, cy, r) for cell (i, j).
  * Returns q (the full count) or -1 on error. */
 int fgo_cell_dump(const fgo_plane *p, const float *lambda, int32_t i, int32_t j,
                   uint64_t *hash, uint64_t state[4], float *grains_xyr,
                   int max_grains);
+
+/* Poisson count of cell (i, j) only (no grain draws). */
+uint32_t fgo_cell_count(const fgo_plane *p, const float *lambda, int32_t i, int32_t j);
 
 /* Per-input-pixel dump of the grain-wise generator. */
 int fgo_pixel_dump(const fgo_plane *p, const float *lambda, int32_t x, int32_t y,

@@ -81,6 +81,10 @@ class FgoCounters(C.Structure):
     ]
 
 
+class FgoGrainCounters(C.Structure):
+    _fields_ = [("grains", C.c_uint64), ("pairs", C.c_uint64), ("tests", C.c_uint64), ("hits", C.c_uint64)]
+
+
 _lib = None
 _U64P = C.POINTER(C.c_uint64)
 _F32P = C.POINTER(C.c_float)
@@ -129,6 +133,8 @@ def lib():
     L.fgo_lognormal_sample.restype = C.c_double
     L.fgo_zig_tables.argtypes = [_F64P, _F64P]
     L.fgo_chacha_block.argtypes = [_U32P, C.c_uint64, C.c_uint64, C.c_int, _U32P]
+    L.fgo_pcg32_next.argtypes = [_U64P, C.c_uint64]
+    L.fgo_pcg32_next.restype = C.c_uint32
     L.fgo_stdrng_u64.argtypes = [C.c_uint64, C.c_uint32, _U64P]
     L.fgo_make_offsets.argtypes = [C.c_uint64, C.c_uint32, C.c_float, _F32P]
     L.fgo_make_offsets.restype = C.c_int
@@ -150,9 +156,14 @@ def lib():
     L.fgo_render_grainwise.argtypes = [C.POINTER(FgoPlane), _F32P, _F32P, _F32P, _U64P, C.c_int,
                                        _U64P]
     L.fgo_render_grainwise.restype = C.c_int
+    L.fgo_render_grainwise_counted.argtypes = [C.POINTER(FgoPlane), _F32P, _F32P, _F32P, _U64P, C.c_int,
+                                               C.POINTER(FgoGrainCounters)]
+    L.fgo_render_grainwise_counted.restype = C.c_int
     L.fgo_cell_dump.argtypes = [C.POINTER(FgoPlane), _F32P, C.c_int32, C.c_int32, _U64P, _U64P,
                                 _F32P, C.c_int]
     L.fgo_cell_dump.restype = C.c_int
+    L.fgo_cell_count.argtypes = [C.POINTER(FgoPlane), _F32P, C.c_int32, C.c_int32]
+    L.fgo_cell_count.restype = C.c_uint32
     L.fgo_pixel_dump.argtypes = [C.POINTER(FgoPlane), _F32P, C.c_int32, C.c_int32, _U64P, _U64P,
                                  _F32P, C.c_int]
     L.fgo_pixel_dump.restype = C.c_int
@@ -323,6 +334,19 @@ def render_grainwise(p: FgoPlane, lam: np.ndarray, offsets: np.ndarray, threads:
     return out
 
 
+def render_grainwise_counted(p: FgoPlane, lam: np.ndarray, offsets: np.ndarray, threads: int = 0):
+    """Grain-wise render and its counters (grains, pairs, disk tests, hits)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float32)
+    offs = np.ascontiguousarray(offsets, dtype=np.float32).reshape(-1, 2)
+    out = np.zeros((p.out_h, p.out_w), dtype=np.float32)
+    c = FgoGrainCounters()
+    rc = lib().fgo_render_grainwise_counted(C.byref(p), _f32p(lam), _f32p(offs), _f32p(out), None, threads,
+                                            C.byref(c))
+    if rc != 0:
+        raise ValueError("oracle render_grainwise failed")
+    return out, dict(grains=c.grains, pairs=c.pairs, tests=c.tests, hits=c.hits)
+
+
 def cell_dump(p: FgoPlane, lam: np.ndarray, i: int, j: int, max_grains: int = 64):
     lam = np.ascontiguousarray(lam, dtype=np.float32)
     h = C.c_uint64(0)
@@ -331,6 +355,11 @@ def cell_dump(p: FgoPlane, lam: np.ndarray, i: int, j: int, max_grains: int = 64
     q = lib().fgo_cell_dump(C.byref(p), _f32p(lam), i, j, C.byref(h), _u64p(st), _f32p(g),
                             max_grains)
     return h.value, st, q, g[:min(q, max_grains)]
+
+
+def cell_count(p: FgoPlane, lam: np.ndarray, i: int, j: int) -> int:
+    lam = np.ascontiguousarray(lam, dtype=np.float32)
+    return int(lib().fgo_cell_count(C.byref(p), _f32p(lam), i, j))
 
 
 def pixel_dump(p: FgoPlane, lam: np.ndarray, x: int, y: int, max_grains: int = 2048):
