@@ -36,7 +36,8 @@ EXPORTED = [
     "fg_mlp_budget_device", "fg_mlp_budget_scale_device", "fg_budget_scale_device",
     "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
-    "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_version",
+    "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_render_pixelwise_device_noalloc",
+    "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
 ALGO_PIXEL, ALGO_GRAIN = 0, 1
@@ -46,6 +47,7 @@ LOSS_SQUARED = 0
 LOSS_NLL = 1
 FG_ECAPACITY = 7
 ERROR_NAMES[FG_ECAPACITY] = "FG_ECAPACITY"
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 
 class PlaneDesc(C.Structure):
@@ -173,6 +175,8 @@ def lib() -> C.CDLL:
     L.fg_pixelwise_grains.restype = C.c_int
     L.fg_grainwise_pixel_state.argtypes = [vp, vp, vp, vp, vp]
     L.fg_grainwise_pixel_state.restype = C.c_int
+    L.fg_render_pixelwise_device_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.fg_render_pixelwise_device_noalloc.restype = C.c_int
     L.fg_max_grains.argtypes = []
     L.fg_max_grains.restype = C.c_uint64
     _lib = L
@@ -238,6 +242,14 @@ class Context:
         fn = self._L.fg_render_pixelwise_device if algo == "pixel" else self._L.fg_render_grainwise_device
         rc = fn(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
                 C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
+        self._check(rc)
+
+    def render_device_noalloc(self, desc: PlaneDesc, lam, offsets, out, stream=None):
+        """Pixel-wise render like render_device, with every scratch buffer in
+        the library's static device array (fg_render_pixelwise_device_noalloc)."""
+        rc = self._L.fg_render_pixelwise_device_noalloc(
+            self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
+            C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
         self._check(rc)
 
     def render_adaptive_device(self, desc: PlaneDesc, lam, offsets, budget, out, stream=None):

@@ -47,6 +47,9 @@ struct fg_ctx {
     int *d_status = nullptr;
     unsigned long long *d_stats = nullptr;
     unsigned long long *d_sum64 = nullptr;  // a plane's 64-bit grain total (bin_count)
+    // device address of this device's copy of the allocation-free entry
+    // point's static scratch array (hipGetSymbolAddress at creation)
+    unsigned char *noalloc_scratch = nullptr;
     // cancel flag: coherent host memory that every tile of the integrators
     // reads once (host and device addresses)
     int *h_cancel = nullptr;
@@ -76,6 +79,13 @@ struct fg_ctx {
     fg_launch_config cfg{};
     std::map<uint32_t, bool> div_checked;  // cell size -> reciprocal division verified
 };
+
+// Scratch of fg_render_pixelwise_device_noalloc: one copy per device, held
+// from the library's load until the process ends.  In host code this name is
+// a host-side stand-in, never a device address: host code uses the address
+// hipGetSymbolAddress returns for the current device (fg_ctx::noalloc_scratch).
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -676,14 +686,17 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     e = fg::launch_sample_table(a.offs, a.n_samples, a.zoom, q, samples_p, &st, s);
     // candidate lists for the direct test, one set per sample table
     const ListPlan lp = list_plan(a, st.eps);
+    // (a fixed-N render touches no host container: the allocation-free
+    // entry point takes this path)
     std::vector<int> &ln = ctx->level_n;
-    ln.clear();
+    size_t lists_total = lp.bytes(a.n_samples);
     if (a.budget) {
+        ln.clear();
         for (int l = 16; l < a.n_samples; l *= 2) ln.push_back(l);
+        ln.push_back(a.n_samples);
+        lists_total = 0;
+        for (int l : ln) lists_total += lp.bytes(l);
     }
-    ln.push_back(a.n_samples);
-    size_t lists_total = 0;
-    for (int l : ln) lists_total += lp.bytes(l);
     unsigned char *lists_at = nullptr;
     if (lists_total > 0 && lists_total < ((size_t)1 << 32)) {
         void *cells_p;
@@ -1458,6 +1471,9 @@ int fg_ctx_create(int device_ordinal, fg_ctx **out)
     if (e == hipSuccess) e = hipMalloc(&ctx->d_status, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_sum64, 2 * sizeof(unsigned long long));
+    // this device's copy of the allocation-free entry point's scratch (the
+    // device is current)
+    if (e == hipSuccess) e = hipGetSymbolAddress(reinterpret_cast<void **>(&ctx->noalloc_scratch), HIP_SYMBOL(scratch));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_range, sizeof(float4));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(ctx->d_status, 0, sizeof(int));
@@ -1528,6 +1544,40 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     return do_pixelwise(ctx, desc, d_lambda, d_offsets_input_xy, d_out,
                         reinterpret_cast<hipStream_t>(hip_stream), false);
+}
+
+int fg_render_pixelwise_device_noalloc(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                                       const float *d_offsets_input_xy, float *d_out, void *hip_stream)
+{
+    const char *what = "Pixel renderer (allocation-free)";
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (!d_lambda || !d_offsets_input_xy || !d_out) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (desc->n_samples > kHbmMaxSamples)
+        return set_error(ctx, FG_EINVAL, std::string(what) + ": n_samples must be <= 2048");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    if (!ctx->noalloc_scratch) return set_error(ctx, FG_EHIP, std::string(what) + ": no scratch address");
+    hipStream_t s = reinterpret_cast<hipStream_t>(hip_stream);
+    float4 range{};
+    e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offsets_input_xy), (int)desc->n_samples,
+                                 ctx->d_range, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&range, ctx->d_range, sizeof range, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    fg::PixGeom geom{};
+    int tw, th, lds;
+    if (!plane_geometry(desc, range, &geom) || !hbm_config(ctx, desc, &tw, &th, &lds, 4LL * ctx->num_cus))
+        return set_error(ctx, FG_EINVAL, std::string(what) + ": the plane's cell grid does not fit the HBM path");
+    fg::PixelArgs a{};
+    if ((rc = pixel_args(ctx, desc, d_lambda, d_offsets_input_xy, d_out, tw, th, lds, ctx->d_range, ctx->d_status,
+                         &a)) != FG_OK)
+        return rc;
+    ScratchSource src;
+    src.arena = ctx->noalloc_scratch;
+    src.cap = kNoAllocScratchBytes;
+    return run_pixel_hbm(ctx, a, geom, range, (int)desc->dist_kind, s, src);
 }
 
 int fg_render_pixelwise_adaptive_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,

@@ -110,6 +110,25 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
 int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
                                const float *d_offsets_xy, float *d_out, void *hip_stream);
 
+/* Allocation-free variant of fg_render_pixelwise_device, with the same
+ * arguments, result and asynchrony.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating library routine: every scratch buffer of the pixel-wise HBM
+ * path (thresholds, bins, grains, sample table, candidate lists, tile order)
+ * comes from one static device array of FG_NOALLOC_SCRATCH_BYTES (256 MiB)
+ * per device, held from the library's load until the process ends.  A plane
+ * whose scratch would exceed it (about 15 million grains) is refused with
+ * FG_ECAPACITY; so are n_samples > 2048.  Every call on a device shares that
+ * device's array, across contexts: calls that overlap on one device race
+ * (serialise them, e.g. on one stream with no other caller).  The array
+ * keeps the last call's contents; the path clears what must start at zero.
+ * Like the other device entry points it synchronises once with the stream
+ * after its counting pass. */
+#define FG_NOALLOC_SCRATCH_BYTES (256u << 20)
+int fg_render_pixelwise_device_noalloc(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
+                                       const float *d_offsets_input_xy, float *d_out, void *hip_stream);
+
 /* Adaptive sampling: pixel-wise render where output pixel i uses only the
  * first d_budget[i] of the n_samples offsets (u16 per output pixel, clamped to
  * [1, n_samples]) and is (covered samples) * (1 / d_budget[i]): for every
