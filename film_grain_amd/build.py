@@ -43,7 +43,7 @@ if SANITIZE:
 LINK_SANITIZE = ["-Xarch_host", f"-fsanitize={SANITIZE}", "-shared-libasan"] if SANITIZE else []
 
 HIP_SOURCES = ["fg_pixelwise.hip", "fg_pixelwise_hbm.hip", "fg_grainwise.hip", "fg_batch.hip", "fg_mlp.hip",
-               "fg_frame.hip", "fg_capi.hip"]
+               "fg_frame.hip", "fg_probe.hip", "fg_capi.hip"]
 # The exact-logf correction table (fg_logf.h) is generated from this
 # machine's C library at every build and checked over its whole domain.
 LOGF_GEN = "gen_logf_table.hip"
@@ -76,9 +76,29 @@ def logf_table() -> Path:
     return header
 
 
+EXP_GEN = "gen_exp_table.cpp"
+EXP_TABLE = "fg_exp_neg_table.h"
+GXX = os.environ.get("CXX", "g++")
+
+
+def exp_table() -> Path:
+    """Builds and runs the exp(-x) table generator (host C++ only); returns
+    the header."""
+    gen = BUILD / "gen_exp_table"
+    _run([GXX, "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-pthread", f"-I{CSRC}",
+          str(CSRC / EXP_GEN), "-o", str(gen), "-lm"])
+    header = BUILD / EXP_TABLE
+    r = subprocess.run([str(gen), str(header)], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("exp table generation failed")
+    return header
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     logf_table()
+    exp_table()
     jobs = []
     objs = []
     for src in HIP_SOURCES:
@@ -88,7 +108,7 @@ def build(verbose: bool = False) -> Path:
     for src in HOST_SOURCES:
         obj = BUILD / ("host_" + src + ".o")
         objs.append(obj)
-        jobs.append([HIPCC, *COMMON, "-c", str(CSRC / "host" / src), "-o", str(obj)])
+        jobs.append([HIPCC, *COMMON, f"-I{BUILD}", "-c", str(CSRC / "host" / src), "-o", str(obj)])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -37,6 +37,7 @@ EXPORTED = [
     "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
     "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_render_pixelwise_device_noalloc",
+    "fg_math_device",
     "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
@@ -48,6 +49,7 @@ LOSS_NLL = 1
 FG_ECAPACITY = 7
 ERROR_NAMES[FG_ECAPACITY] = "FG_ECAPACITY"
 NOALLOC_SCRATCH_BYTES = 256 << 20
+MATH_EXP_NEG, MATH_EXP, MATH_LOG, MATH_TAN_PI, MATH_LOG_GAMMA, MATH_SQRT, MATH_EXP_F32 = range(7)
 
 
 class PlaneDesc(C.Structure):
@@ -177,6 +179,8 @@ def lib() -> C.CDLL:
     L.fg_grainwise_pixel_state.restype = C.c_int
     L.fg_render_pixelwise_device_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
     L.fg_render_pixelwise_device_noalloc.restype = C.c_int
+    L.fg_math_device.argtypes = [vp, C.c_int, vp, C.c_int64, vp, vp]
+    L.fg_math_device.restype = C.c_int
     L.fg_max_grains.argtypes = []
     L.fg_max_grains.restype = C.c_uint64
     _lib = L
@@ -337,6 +341,11 @@ class Context:
         self._check(self._L.fg_grainwise_pixel_state(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()),
                                                      C.c_void_p(q.ctypes.data), C.c_void_p(st.ctypes.data)))
         return q.reshape(desc.in_h, desc.in_w), st.reshape(desc.in_h, desc.in_w, 4)
+
+    def math_device(self, fn: int, x, out, stream=None):
+        """The integrators' f64 math on the device (FG_MATH_* codes), element-wise."""
+        self._check(self._L.fg_math_device(self.handle, fn, C.c_void_p(x.data_ptr()), x.numel(),
+                                           C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream))))
 
     def cancel(self, on: bool = True):
         """Raise (or lower) the context's cancel flag: the integrators' tiles

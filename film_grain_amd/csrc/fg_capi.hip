@@ -1916,6 +1916,17 @@ int fg_color_merge_device(fg_ctx *ctx, int color_mode, const float *d_p0, const 
     return e == hipSuccess ? FG_OK : hip_error(ctx, e, "Colour merge");
 }
 
+int fg_math_device(fg_ctx *ctx, int fn, const double *d_in, int64_t n, double *d_out, void *hip_stream)
+{
+    if (!ctx) return FG_EINVAL;
+    if (fn < 0 || fn > fg::kMathExpF32) return set_error(ctx, FG_EINVAL, "unknown math function");
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return set_error(ctx, FG_EINVAL, "null buffer");
+    if (n == 0) return FG_OK;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = fg::launch_math_probe(fn, d_in, n, d_out, reinterpret_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? FG_OK : hip_error(ctx, e, "math");
+}
+
 int fg_log_f32_device(fg_ctx *ctx, const float *d_in, int64_t n, float *d_out, void *hip_stream)
 {
     if (!ctx) return FG_EINVAL;

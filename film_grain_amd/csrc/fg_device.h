@@ -66,7 +66,7 @@ __device__ __forceinline__ double cell_threshold(float lam, float delta)
     if (!(lam > 0.0f) || !(lam <= kMaxLambda)) return 2.0;
     const float expected = lam * delta * delta;
     if (!(expected > 0.0f)) return 2.0;
-    if (expected < 12.0f) return exp(-(double)expected);
+    if (expected < 12.0f) return exp_neg_glibc(expected);  // glibc's exp(-expected), exactly
     return -(double)expected;
 }
 

@@ -11,6 +11,11 @@ namespace fg {
 constexpr int kColorLuma = 0;  // render Y; carry BT.709 Cb, Cr at the input size
 constexpr int kColorRgb = 1;   // render R, G, B
 
+// The integrators' f64 math on the device, element-wise (fn: the kMath*
+// codes, equal to FG_MATH_* of the public header).
+constexpr int kMathExpNeg = 0, kMathExp = 1, kMathLog = 2, kMathTanPi = 3, kMathLogGamma = 4, kMathSqrt = 5,
+              kMathExpF32 = 6;
+hipError_t launch_math_probe(int fn, const double *in, long long n, double *out, hipStream_t s);
 // out[i] = the C library's logf(in[i]) for in[i] in [1e-6, 1].
 hipError_t launch_log_f32(const float *in, long long n, float *out, hipStream_t s);
 // normalize_plane then lambda_plane of n values; norm may be null; max_bits

@@ -10,7 +10,10 @@
 //   * rand_distr 0.4.3 Poisson (Knuth below 12, Cauchy rejection above) with its
 //     Lanczos log_gamma, StandardNormal ziggurat (256 layers), LogNormal
 // All arithmetic keeps the reference's operation order and is compiled with
-// -ffp-contract=off: Rust never fuses a*b+c.
+// -ffp-contract=off: Rust never fuses a*b+c.  The reference's exponentials are
+// glibc's exp; on the device they are fg_exp_dd (fg_exp.h), which equals glibc
+// but for about 2 inputs in 10^4, corrected by a table on the Knuth
+// threshold's domain (fg_exp_glibc.h).  Host builds call glibc itself.
 #ifndef FG_RNG_H
 #define FG_RNG_H
 
@@ -26,6 +29,15 @@
 #endif
 
 #include "fg_zig_tables.h"
+#if defined(__HIPCC__)
+#include "fg_exp_glibc.h"
+#endif
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FG_EXP(x) fg_exp_dd(x)
+#else
+#define FG_EXP(x) exp(x)
+#endif
 
 namespace fg {
 
@@ -179,7 +191,7 @@ FG_HD double poisson_rejection(Xoshiro &rng, double lambda)
         }
         result = floor(result);
         const double check = 0.9 * (1.0 + comp * comp) *
-                             exp(result * log_lambda - log_gamma(1.0 + result) - magic);
+                             FG_EXP(result * log_lambda - log_gamma(1.0 + result) - magic);
         if (standard_f64(rng.next()) <= check) break;
     }
     return result;
@@ -191,7 +203,11 @@ FG_HD double poisson_rejection(Xoshiro &rng, double lambda)
 FG_HD uint32_t poisson_u32(Xoshiro &rng, double lambda)
 {
     if (lambda < 12.0) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double e = exp_neg_glibc((float)lambda);  // lambda is an f32 value
+#else
         const double e = exp(-lambda);
+#endif
         double p = 1.0;
         uint32_t n = 0;
         while (p > e) {
@@ -227,7 +243,7 @@ FG_HD double standard_normal(NextU64 &&next, const double *zx, const double *zf)
             }
             return u < 0.0 ? xx - FG_ZIG_NORM_R : FG_ZIG_NORM_R - xx;
         }
-        if (zf[i + 1] + (zf[i] - zf[i + 1]) * standard_f64(next()) < exp(-x * x / 2.0)) return x;
+        if (zf[i + 1] + (zf[i] - zf[i + 1]) * standard_f64(next()) < FG_EXP(-x * x / 2.0)) return x;
     }
 }
 
@@ -235,7 +251,7 @@ FG_HD double standard_normal(NextU64 &&next, const double *zx, const double *zf)
 FG_HD float lognormal_f32(Xoshiro &rng, double mu, double sigma, const double *zx, const double *zf)
 {
     const double z = standard_normal([&rng]() { return rng.next(); }, zx, zf);
-    return (float)exp(mu + sigma * z);
+    return (float)FG_EXP(mu + sigma * z);
 }
 
 // Rust float->int `as` casts saturate and send NaN to 0.

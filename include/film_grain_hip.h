@@ -358,6 +358,23 @@ int fg_grainwise_pixel_state(fg_ctx *ctx, const fg_plane_desc *desc, const float
  * path never stores grains). */
 uint64_t fg_max_grains(void);
 
+/* The f64 math the integrators evaluate on the device, element-wise on n
+ * doubles (device memory, asynchronous), for comparison with the C library
+ * the reference's Rust std and rand_distr call: FG_MATH_EXP_NEG exp(-x) (the
+ * Knuth threshold, src/pixelwise.rs:81 via Poisson::new), FG_MATH_EXP exp(x),
+ * FG_MATH_LOG log(x), FG_MATH_TAN_PI tan(pi * x) (the Cauchy draw of the
+ * rejection sampler), FG_MATH_LOG_GAMMA rand_distr's log_gamma(x),
+ * FG_MATH_SQRT sqrt(x), FG_MATH_EXP_F32 (double)(float)exp(x) (a LogNormal
+ * radius, src/model.rs:142).  No reference counterpart. */
+#define FG_MATH_EXP_NEG 0
+#define FG_MATH_EXP 1
+#define FG_MATH_LOG 2
+#define FG_MATH_TAN_PI 3
+#define FG_MATH_LOG_GAMMA 4
+#define FG_MATH_SQRT 5
+#define FG_MATH_EXP_F32 6
+int fg_math_device(fg_ctx *ctx, int fn, const double *d_in, int64_t n, double *d_out, void *hip_stream);
+
 /* Library version string. */
 const char *fg_version(void);
 

@@ -962,6 +962,118 @@ int fgo_pixel_dump(const fgo_plane *p, const float *lambda, int32_t x, int32_t y
     return (int)q;
 }
 
+/* Digest of one grain record at position k of its cell: splitmix64's
+ * finaliser of the f32 bits of (cx, cy, r) and k, summed per cell mod 2^64
+ * (tests/cellcheck.py computes the same from the device's grain buffer). */
+static inline uint64_t grain_digest(float cx, float cy, float r, uint64_t k)
+{
+    uint64_t v = ((uint64_t)f32_bits(cx) << 32) | (uint64_t)f32_bits(cy);
+    v ^= (uint64_t)f32_bits(r) * 0x9E3779B97F4A7C15ULL;
+    v ^= k * 0xD1B54A32D192ED03ULL;
+    return sm64(v);
+}
+
+int fgo_cell_digests(const fgo_plane *p, const float *lambda, int32_t i0, int32_t j0, uint32_t w, uint32_t h,
+                     uint32_t *q_out, uint64_t *digest_out, int threads)
+{
+    if (!p || !lambda || !q_out || !digest_out) return -1;
+    zig_init();
+    const float delta = p->delta;
+    pix_ctx c;
+    c.p = p;
+    c.lambda = lambda;
+    c.uniform_scale = fgo_uniform_f32_scale(0.0f, delta);
+    c.log_mu = (double)p->log_mu;
+    c.log_sigma = (double)p->log_sigma;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#endif
+    for (int64_t jj = 0; jj < (int64_t)h; ++jj) {
+        for (uint32_t ii = 0; ii < w; ++ii) {
+            const int32_t i = i0 + (int32_t)ii, j = j0 + (int32_t)jj;
+            const uint64_t at = (uint64_t)jj * w + ii;
+            uint64_t s[4];
+            xo_seed(mix3(p->seed, FGO_CELL_STREAM, (int64_t)i, (int64_t)j), s);
+            const float sample_x = (float)i * delta, sample_y = (float)j * delta;
+            const float lambda_cell =
+                lambda_get_clamped(p, lambda, sat_i64f(floorf(sample_x)), sat_i64f(floorf(sample_y)));
+            uint32_t q = 0;
+            uint64_t dg = 0;
+            const float expected = lambda_cell * delta * delta;
+            if (lambda_cell > 0.0f && expected > 0.0f) {
+                poisson_t po;
+                poisson_new(&po, (double)expected);
+                q = sat_u32(poisson_sample_xo(&po, s));
+                for (uint32_t g = 0; g < q; ++g) {
+                    const float cx = sample_x + uni_f32(s, 0.0f, c.uniform_scale);
+                    const float cy = sample_y + uni_f32(s, 0.0f, c.uniform_scale);
+                    float radius = radius_sample(&c, s);
+                    if (radius > p->rm) radius = p->rm;
+                    dg += grain_digest(cx, cy, radius, g);
+                }
+            }
+            q_out[at] = q;
+            digest_out[at] = dg;
+        }
+    }
+    return 0;
+}
+
+int fgo_pixel_counts(const fgo_plane *p, const float *lambda, uint32_t *q_out, uint64_t *state_out, int threads)
+{
+    if (!p || !lambda || !q_out) return -1;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#endif
+    for (int64_t y = 0; y < (int64_t)p->in_h; ++y) {
+        for (uint32_t x = 0; x < p->in_w; ++x) {
+            const uint64_t at = (uint64_t)y * p->in_w + x;
+            const float lambda_val = lambda[at];
+            uint64_t s[4] = {0, 0, 0, 0};
+            uint32_t q = 0;
+            if (lambda_val > 0.0f) {
+                xo_seed(mix3(p->seed, FGO_PIXEL_STREAM, (int64_t)(int32_t)x, (int64_t)y), s);
+                poisson_t po;
+                poisson_new(&po, (double)lambda_val);
+                q = sat_u32(poisson_sample_xo(&po, s));
+            }
+            q_out[at] = q;
+            if (state_out) memcpy(state_out + 4 * at, s, sizeof s);
+        }
+    }
+    return 0;
+}
+
+/* The C library's f64 math (what Rust's f64::exp/ln/tan/sqrt call on Linux,
+ * and through them rand_distr), element-wise, for comparison with the
+ * device: fn as FG_MATH_* (0 exp(-x), 1 exp(x), 2 log(x), 3 tan(pi x),
+ * 4 log_gamma(x), 5 sqrt(x), 6 (double)(float)exp(x)). */
+int fgo_glibc_math(int fn, const double *in, int64_t n, double *out, int threads)
+{
+    if (fn < 0 || fn > 6 || n < 0) return -1;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(threads)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        const double x = in[i];
+        double r = 0.0;
+        switch (fn) {
+        case 0: r = exp(-x); break;
+        case 1: r = exp(x); break;
+        case 2: r = log(x); break;
+        case 3: r = tan(3.14159265358979323846 * x); break;
+        case 4: r = fgo_log_gamma(x); break;
+        case 5: r = sqrt(x); break;
+        case 6: r = (double)(float)exp(x); break;
+        }
+        out[i] = r;
+    }
+    return 0;
+}
+
 int fgo_max_threads(void)
 {
 #ifdef _OPENMP

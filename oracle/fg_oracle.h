@@ -154,6 +154,19 @@ int fgo_pixel_dump(const fgo_plane *p, const float *lambda, int32_t x, int32_t y
                    uint64_t *hash, uint64_t state[4], float *grains_xyr,
                    int max_grains);
 
+/* Every cell of the w x h grid from (i0, j0): the Poisson count and a digest
+ * of its grains (the sum mod 2^64 of splitmix64's finaliser of each grain's
+ * f32 bits and position), row-major.  And every input pixel's grain-wise
+ * Poisson count, with the stream state after the count draws (state_out,
+ * optional, 4 words per pixel; zero where lambda <= 0). */
+int fgo_cell_digests(const fgo_plane *p, const float *lambda, int32_t i0, int32_t j0, uint32_t w, uint32_t h,
+                     uint32_t *q_out, uint64_t *digest_out, int threads);
+int fgo_pixel_counts(const fgo_plane *p, const float *lambda, uint32_t *q_out, uint64_t *state_out, int threads);
+
+/* The C library's f64 math element-wise (fn: 0 exp(-x), 1 exp(x), 2 log(x),
+ * 3 tan(pi x), 4 log_gamma(x), 5 sqrt(x), 6 (double)(float)exp(x)). */
+int fgo_glibc_math(int fn, const double *in, int64_t n, double *out, int threads);
+
 int fgo_max_threads(void);
 
 #ifdef __cplusplus

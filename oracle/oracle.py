@@ -168,6 +168,13 @@ def lib():
                                  _F32P, C.c_int]
     L.fgo_pixel_dump.restype = C.c_int
     L.fgo_max_threads.restype = C.c_int
+    L.fgo_cell_digests.argtypes = [C.POINTER(FgoPlane), _F32P, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
+                                   _U32P, _U64P, C.c_int]
+    L.fgo_cell_digests.restype = C.c_int
+    L.fgo_pixel_counts.argtypes = [C.POINTER(FgoPlane), _F32P, _U32P, _U64P, C.c_int]
+    L.fgo_pixel_counts.restype = C.c_int
+    L.fgo_glibc_math.argtypes = [C.c_int, _F64P, C.c_int64, _F64P, C.c_int]
+    L.fgo_glibc_math.restype = C.c_int
     _lib = L
     return L
 
@@ -370,6 +377,41 @@ def pixel_dump(p: FgoPlane, lam: np.ndarray, x: int, y: int, max_grains: int = 2
     q = lib().fgo_pixel_dump(C.byref(p), _f32p(lam), x, y, C.byref(h), _u64p(st), _f32p(g),
                              max_grains)
     return h.value, st, q, g[:min(q, max_grains)]
+
+
+def cell_digests(p: FgoPlane, lam: np.ndarray, origin, size, threads: int = 0):
+    """Per cell of the size[0] x size[1] grid from origin: Poisson counts and
+    grain digests, as (h, w) arrays."""
+    lam = np.ascontiguousarray(lam, dtype=np.float32)
+    w, h = int(size[0]), int(size[1])
+    q = np.zeros((h, w), dtype=np.uint32)
+    d = np.zeros((h, w), dtype=np.uint64)
+    rc = lib().fgo_cell_digests(C.byref(p), _f32p(lam), int(origin[0]), int(origin[1]), w, h,
+                                q.ctypes.data_as(_U32P), _u64p(d), threads)
+    if rc != 0:
+        raise ValueError("oracle cell_digests failed")
+    return q, d
+
+
+def pixel_counts(p: FgoPlane, lam: np.ndarray, threads: int = 0, states: bool = False):
+    """Every input pixel's grain-wise Poisson count (and the stream state after it)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float32)
+    q = np.zeros((p.in_h, p.in_w), dtype=np.uint32)
+    st = np.zeros((p.in_h, p.in_w, 4), dtype=np.uint64) if states else None
+    rc = lib().fgo_pixel_counts(C.byref(p), _f32p(lam), q.ctypes.data_as(_U32P),
+                                _u64p(st) if states else None, threads)
+    if rc != 0:
+        raise ValueError("oracle pixel_counts failed")
+    return (q, st) if states else q
+
+
+def glibc_math(fn: int, x: np.ndarray, threads: int = 0) -> np.ndarray:
+    """The C library's f64 math element-wise (fn as FG_MATH_*)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    if lib().fgo_glibc_math(fn, x.ctypes.data_as(_F64P), x.size, out.ctypes.data_as(_F64P), threads) != 0:
+        raise ValueError("bad math function")
+    return out
 
 
 def max_threads() -> int:

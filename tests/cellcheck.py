@@ -76,3 +76,40 @@ def replay_pixel_grains(oracle, plane, x, y, state, q):
             r = np.float32(plane.radius_mean)
         out[g, 2] = min(r, np.float32(plane.rm))
     return out
+
+
+def _splitmix_final(v):
+    v = v.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        v = v + np.uint64(0x9E3779B97F4A7C15)
+        v = (v ^ (v >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        v = (v ^ (v >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return v ^ (v >> np.uint64(31))
+
+
+def device_cell_digests(records: np.ndarray, size):
+    """Per cell of the grid (h, w): the grain count and the digest the oracle's
+    fgo_cell_digests computes, from (n, 4) records grouped by cell in draw order."""
+    w, h = int(size[0]), int(size[1])
+    q = np.zeros(w * h, dtype=np.uint32)
+    d = np.zeros(w * h, dtype=np.uint64)
+    n = len(records)
+    if n == 0:
+        return q.reshape(h, w), d.reshape(h, w)
+    key = records[:, 3].view(np.uint32).astype(np.int64)
+    starts = np.concatenate([[0], np.flatnonzero(np.diff(key)) + 1])
+    lengths = np.diff(np.concatenate([starts, [n]]))
+    k = np.arange(n, dtype=np.int64) - np.repeat(starts, lengths)
+    b = records[:, :3].view(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        v = (b[:, 0] << np.uint64(32)) | b[:, 1]
+        v ^= b[:, 2] * np.uint64(0x9E3779B97F4A7C15)
+        v ^= k.astype(np.uint64) * np.uint64(0xD1B54A32D192ED03)
+    mixed = _splitmix_final(v)
+    sums = np.add.reduceat(mixed, starts)
+    cell = key[starts]
+    idx = (cell >> 16) * w + (cell & 0xFFFF)
+    assert len(np.unique(idx)) == len(idx), "a cell's grains are split across the buffer"
+    q[idx] = lengths.astype(np.uint32)
+    d[idx] = sums
+    return q.reshape(h, w), d.reshape(h, w)

@@ -70,3 +70,15 @@ def tiles(rng, out_w, out_h, tile, count):
         y0 = int(rng.integers(0, max(1, out_h - tile + 1)))
         out.append((x0, y0, min(x0 + tile, out_w), min(y0 + tile, out_h)))
     return out
+
+
+def oracle_plane(O, desc, n_samples=None):
+    """The oracle's FgoPlane for a C-ABI plane description (optionally with
+    another sample count, whose offsets are the first n of the same set)."""
+    p = O.FgoPlane()
+    p.in_w, p.in_h, p.out_w, p.out_h = desc.in_w, desc.in_h, desc.out_w, desc.out_h
+    p.n_samples = desc.n_samples if n_samples is None else n_samples
+    p.dist_kind, p.seed = desc.dist_kind, desc.seed
+    p.zoom, p.delta, p.rm, p.radius_mean = desc.zoom, desc.delta, desc.rm, desc.radius_mean
+    p.log_mu, p.log_sigma = desc.log_mu, desc.log_sigma
+    return p

@@ -50,3 +50,29 @@ def test_planted_slips_fail(oracle, setup):
     cells = cellcheck.group_cells(nudged, origin, size)
     _, _, inexact, worst = cellcheck.compare_cells(oracle, case.plane, case.lam, cells, origin, size)
     assert inexact == 1 and worst == 1
+
+
+def test_device_digests_equal_oracle_digests(oracle, setup):
+    case, rec, origin, size = setup
+    q, d = cellcheck.device_cell_digests(rec, size)
+    qo, do = oracle.cell_digests(case.plane, case.lam, origin, size)
+    assert np.array_equal(q, qo) and np.array_equal(d, do)
+    key = rec[:, 3].view(np.uint32)
+    k = int(np.flatnonzero(key[1:] == key[:-1])[0])
+    swapped = rec.copy()
+    swapped[[k, k + 1]] = swapped[[k + 1, k]]
+    assert not np.array_equal(cellcheck.device_cell_digests(swapped, size)[1], do)
+    nudged = rec.copy()
+    nudged[3, 2] = np.nextafter(nudged[3, 2], np.float32(1))
+    assert not np.array_equal(cellcheck.device_cell_digests(nudged, size)[1], do)
+
+
+def test_oracle_pixel_counts_equal_pixel_dumps(oracle):
+    case = fgcases.c3_case(oracle, size=16, n_samples=8)
+    q, st = oracle.pixel_counts(case.plane, case.lam, states=True)
+    for y in range(16):
+        for x in range(16):
+            _, _, qd, g = oracle.pixel_dump(case.plane, case.lam, x, y)
+            assert q[y, x] == qd
+            if qd:
+                assert np.array_equal(cellcheck.replay_pixel_grains(oracle, case.plane, x, y, st[y, x], qd), g)
