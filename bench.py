@@ -7,7 +7,7 @@ sigma 0.8, N = 256, seed 7, with the lambda planes already resident in HBM.
 One step renders the three colour planes.  The value is
 Mpix*samples/s = 3840 * 2160 * 3 * 256 * steps / seconds / 1e6.
 
-Also reported on the same JSON line:
+Also reported on the same JSON line (default workload, after its timed region):
   roofline      FP32 FLOP/s of the pixel-wise kernel against MI355X's FP32 peak;
                 FLOPs per (pixel, sample) = 16 + 6*T, T = disk tests in the
                 reference's loop order with its early exit, counted by the
@@ -17,6 +17,12 @@ Also reported on the same JSON line:
                 on a bounded random sample of 16x16 output tiles of the same
                 frame, on the host's cores; those tiles also spot-check parity.
 
+  grain_c3      the grain-wise 4096x4096 C3 frame (N = 128): Mpix*samples/s,
+                its HBM roofline (8 + N/4 algorithmic bytes per output pixel)
+                and a VALU roofline of the reference's own work (8 FLOPs per
+                (grain, sample) pair plus 5 per disk test, counted exactly)
+  batch_c4      the uneven 64-image batch in one graph-captured launch
+                against one call per plane
   frame         (default workload) the frame from host memory to host memory
                 (PCIe included): host split / lambda / merge against the
                 frame on the device, with identical output.
@@ -73,6 +79,15 @@ def measured_traffic(workload, kernel, cfg):
 
 
 def _cpu_model() -> str:
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -83,12 +98,20 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _host_threads() -> int:
+    """Every core this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     """Times the oracle on random 16x16 output tiles of plane 0 and checks the
     GPU frame on the same tiles; also a one-thread rate on a smaller slice of
     the same kind of sample, and the host's CPU model."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = _host_threads()
     od = O.derive(wl["radius"], wl["sd"], wl["dist"], wl["zoom"], 0.8, wl["iters"], seed=wl["seed"],
                   in_w=wl["w"], in_h=wl["h"])
     p = O.plane(od, wl["iters"], wl["seed"], wl["zoom"], wl["dist"], wl["radius"])
@@ -127,7 +150,8 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
         one = pc.out_w * pc.out_h * wl["iters"] / (time.perf_counter() - t0) / 1e6
         return dict(value=round(n_px * wl["iters"] / t_used / 1e6, 3), unit="Mpix·samples/s", cores=threads,
                     kind="port", sample=sample, one_thread_value=round(one, 3),
-                    one_thread_sample=f"the first of those crops", cpu_model=_cpu_model()), None
+                    one_thread_sample="the first of those crops", cpu_model=_cpu_model(),
+                    nproc=os.cpu_count()), None
     while t_used < budget_s:
         x0 = int(rng.integers(0, od.out_w - tile + 1))
         y0 = int(rng.integers(0, od.out_h - tile + 1))
@@ -156,17 +180,18 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
     one = n1 * wl["iters"] / t1 / 1e6 if t1 > 0 else None
     return dict(value=round(value, 3), unit="Mpix·samples/s", cores=threads, kind="port",
                 sample=sample, one_thread_value=None if one is None else round(one, 3),
-                one_thread_sample=f"the first {min(24, n_tiles)} of those tiles", cpu_model=_cpu_model()), \
+                one_thread_sample=f"the first {min(24, n_tiles)} of those tiles", cpu_model=_cpu_model(),
+                nproc=os.cpu_count()), \
         dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
 
 
-def batch_bench(args):
+def batch_summary(steps: int, warmup: int, n_images: int = 64):
     """The uneven batch: 64 images (sides 128-2048, mixed radii, N and
     channels), every plane resident in HBM; one graph-captured batch run per
     step, against the same planes rendered one call at a time."""
     import torch
     from film_grain_amd import capi, workloads
-    images = workloads.batch_images(master_seed=2024, n_images=args.batch_images)
+    images = workloads.batch_images(master_seed=2024, n_images=n_images)
     planes = workloads.prepare_batch(images)
     dev = torch.device("cuda:0")
     ins = [(d, torch.from_numpy(l).to(dev), torch.from_numpy(o).to(dev)) for d, l, o in planes]
@@ -177,21 +202,22 @@ def batch_bench(args):
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
     batch = capi.Batch(ctx, [(d, l, o, out) for (d, l, o), out in zip(ins, outs)], capture_graph=True)
-    log(f"[bench] batch planned in {time.perf_counter() - t0:.2f} s: {batch.info()}")
-    for _ in range(args.warmup):
+    plan_s = time.perf_counter() - t0
+    log(f"[bench] batch planned in {plan_s:.2f} s: {batch.info()}")
+    for _ in range(warmup):
         batch.run(stream)
     batch.status(stream)
     ref = [o.clone() for o in outs]
 
-    def timed(fn, steps):
+    def timed(fn, n):
         torch.cuda.synchronize()
         a = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(n):
             fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - a) / steps
+        return (time.perf_counter() - a) / n
 
-    batch_s = timed(lambda: batch.run(stream), args.steps)
+    batch_s = timed(lambda: batch.run(stream), steps)
     batch.status(stream)
 
     def per_plane():
@@ -200,26 +226,24 @@ def batch_bench(args):
 
     per_plane()
     ctx.stream_status(stream)
-    single_s = timed(per_plane, max(1, args.steps // 2))
+    single_s = timed(per_plane, max(1, steps // 2))
     ctx.stream_status(stream)
     same = all(torch.equal(a, b) for a, b in zip(outs, ref))
     log(f"[bench] batch {batch_s * 1e3:.2f} ms, one call per plane {single_s * 1e3:.2f} ms, identical planes: {same}")
+    info = batch.info()
     batch.close()
-    line = {
+    ctx.close()
+    return {
         "metric": "Mpix·samples/s (uneven batch, one graph-captured persistent launch)",
-        "value": round(px_samples / batch_s / 1e6, 2), "unit": "Mpix·samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{len(images)} images, sides 128-2048, radius U(0.05,0.3), sigma_r/r U(0,0.5), "
-                               f"N in {{32..512}}, luma/RGB, seed 2024", "planes": len(ins),
-                   "pixel_samples": px_samples},
+        "value": round(px_samples / batch_s / 1e6, 2), "unit": "Mpix·samples/s", "steps": steps,
+        "ms_per_step": round(batch_s * 1e3, 3),
+        "workload": f"{len(images)} images, sides 128-2048, radius U(0.05,0.3), sigma_r/r U(0,0.5), "
+                    f"N in {{32..512}}, luma/RGB, seed 2024",
+        "planes": len(ins), "pixel_samples": px_samples, "plan_s": round(plan_s, 3), "info": info,
         "per_plane_calls_ms": round(single_s * 1e3, 3),
         "speedup_vs_per_plane_calls": round(single_s / batch_s, 3),
         "identical_to_per_plane_calls": bool(same),
-        "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 # Algorithmic FLOPs per row of the error model (12 inputs, 64, 64, 1): the
@@ -335,51 +359,16 @@ def adaptive_summary(steps: int = 3, warmup: int = 1):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4", "adaptive-c5"])
-    ap.add_argument("--no-adaptive", action="store_true",
-                    help="skip the learned-budget comparison appended to the default line")
-    ap.add_argument("--no-frame", action="store_true",
-                    help="skip the host-memory frame comparison appended to the default line")
-    ap.add_argument("--batch-images", type=int, default=64)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
-    ap.add_argument("--pix-variant", type=int, default=None,
-                    help="pixel-wise formulation: 1 gather, 2 tile splat, 3 HBM splat")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="first run one device-to-device copy of this many MiB, so that counter "
-                         "runs (FETCH_SIZE, WRITE_SIZE) can be scaled by a copy of known size")
-    ap.add_argument("--no-stats", action="store_true",
-                    help="skip the instrumented pass (the roofline then has no FLOP count)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    if args.workload == "batch-c4":
-        batch_bench(args)
-        return
-    if args.workload == "adaptive-c5":
-        ad = adaptive_summary(args.steps, args.warmup)
-        line = {"metric": ad["metric"], "value": ad["adaptive_ms_per_frame"], "unit": "ms/frame", "n_gpus": 1,
-                "steps": args.steps, "warmup": args.warmup, "ms_per_step": ad["adaptive_ms_per_frame"],
-                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                "data": "synthetic", "config": {"workload": ad["workload"]}, "roofline": None,
-                "cpu_baseline": None, "adaptive": ad}
-        print(json.dumps(line), flush=True)
-        return
-
+def measure(workload: str, steps: int, warmup: int, cpu: bool, cpu_seconds: float, stats: bool = True,
+            tuning: bool = True, pix_variant=None, copy_calibration_mib: int = 0):
+    """One workload of WORKLOADS on one GPU: the timed steps, the kernel
+    timing and roofline, and the CPU baseline (with its parity spot check)."""
     import torch
     from film_grain_amd import autotune, capi
 
-    wl = WORKLOADS[args.workload]
-    log(f"[bench] workload {args.workload}: {wl['desc']}")
-    prep = prepare(args.workload)
+    wl = WORKLOADS[workload]
+    log(f"[bench] workload {workload}: {wl['desc']}")
+    prep = prepare(workload)
     planes, lams, offsets, d, desc, algo = prep.planes, prep.lams, prep.offsets, prep.derived, prep.desc, prep.algo
 
     dev = torch.device("cuda:0")
@@ -389,46 +378,43 @@ def main():
     out_d = torch.empty((len(lams), d.out_h, d.out_w), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
-    if args.copy_calibration_mib > 0:
-        n = args.copy_calibration_mib << 18  # float32 elements
+    if copy_calibration_mib > 0:
+        n = copy_calibration_mib << 18  # float32 elements
         src = torch.ones(n, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"[bench] calibration copy of {args.copy_calibration_mib} MiB done")
+        log(f"[bench] calibration copy of {copy_calibration_mib} MiB done")
         del src, dst
 
     # instrumented pass: algorithmic work of the frame
     counters = []
-    if not args.no_stats:
+    if stats:
         for i, l in enumerate(lam_d):
             counters.append(ctx.stats_device(algo, desc, l, offs_d, out_d[i], stream))
         torch.cuda.synchronize()
         log(f"[bench] counters per plane {counters}")
 
-    tuned = None if args.no_tuning else autotune.apply_table(ctx, algo, desc)
-    if args.pix_variant is not None:
-        ctx.set_launch_config(pix_variant=args.pix_variant)
+    tuned = autotune.apply_table(ctx, algo, desc) if tuning else None
+    if pix_variant is not None:
+        ctx.set_launch_config(pix_variant=pix_variant)
         tuned = None
     cfg = ctx.launch_config(desc, algo)
     log(f"[bench] launch config {cfg} (tuned table entry: {tuned is not None})")
 
-    def step():
+    for _ in range(warmup):
         for i, l in enumerate(lam_d):
             ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
-
-    for _ in range(args.warmup):
-        step()
     ctx.stream_status(stream)
     torch.cuda.synchronize()
 
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps * len(lam_d))]
+          for _ in range(steps * len(lam_d))]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     k = 0
-    for _ in range(args.steps):
+    for _ in range(steps):
         for i, l in enumerate(lam_d):
             ev[k][0].record(stream)
             ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
@@ -439,15 +425,14 @@ def main():
     ctx.stream_status(stream)
     secs = t1 - t0
     launch_ms = [a.elapsed_time(b) for a, b in ev]
-    avg_launch_ms = float(np.mean(launch_ms))
-    log(f"[bench] {args.steps} steps in {secs:.4f} s; launch ms mean {avg_launch_ms:.3f} "
+    log(f"[bench] {steps} steps in {secs:.4f} s; launch ms mean {float(np.mean(launch_ms)):.3f} "
         f"min {min(launch_ms):.3f} max {max(launch_ms):.3f}")
 
     # kernel timing pass: HIP events recorded by the library on the launch stream
     # around the generation kernels and around the coverage-test kernel
     ctx.set_timing(True)
     kt = []
-    for _ in range(max(2, args.steps)):
+    for _ in range(max(2, steps)):
         for i, l in enumerate(lam_d):
             ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
             kt.append(ctx.timings())
@@ -458,12 +443,13 @@ def main():
     log(f"[bench] per plane: render {render_ms:.3f} ms = generation {gen_ms:.3f} + test kernel {test_ms:.3f}")
 
     px_samples = d.out_w * d.out_h * len(lam_d) * d.n_samples
-    value = px_samples * args.steps / secs / 1e6
+    value = px_samples * steps / secs / 1e6
     kernel_name = {1: "pixelwise_gather", 2: "pixelwise_splat", 3: "splat_hbm_kernel"}.get(cfg.get("pix_variant"), "")
     if algo == "grain":
         kernel_name = "grainwise_sorted" if cfg.get("grain_variant") == 2 else "grainwise_tiles"
-    traffic = measured_traffic(args.workload, kernel_name, cfg)
-    log(f"[bench] traffic key {traffic_key(args.workload, kernel_name, cfg)} -> {traffic}")
+    traffic = measured_traffic(workload, kernel_name, cfg)
+    log(f"[bench] traffic key {traffic_key(workload, kernel_name, cfg)} -> {traffic}")
+    valu = None
     if not counters:
         roofline = None
     elif algo == "pixel":
@@ -481,26 +467,34 @@ def main():
         achieved = bytes_per_launch / (test_ms * 1e-3) / 1e9
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
                         frac=round(achieved / PEAK_HBM_GBS, 6), traffic=traffic,
-                        kernel=kernel_name, kernel_ms=round(test_ms, 3),
-                        bytes_per_launch=bytes_per_launch,
-                        pairs_per_launch=float(np.mean([c[1] for c in counters])))
+                        kernel=kernel_name, kernel_ms=round(test_ms, 3), generation_ms=round(gen_ms, 3),
+                        render_ms=round(render_ms, 3), bytes_per_launch=bytes_per_launch)
+        # the reference's own work: 8 FLOPs per (grain, sample) pair for its
+        # bounds and 5 per disk test, counted by the instrumented twin
+        pairs = float(np.mean([c[1] for c in counters]))
+        tests = float(np.mean([c[2] for c in counters]))
+        vflops = 8.0 * pairs + 5.0 * tests
+        va = vflops / (test_ms * 1e-3) / 1e12
+        valu = dict(bound="valu", achieved=round(va, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=round(va / PEAK_FP32_TFLOPS, 5), flops_per_launch=vflops, pairs_per_launch=pairs,
+                    disk_tests_per_launch=tests, grains_per_launch=float(np.mean([c[0] for c in counters])))
+        log(f"[bench] grain-wise work: {pairs:.4g} pairs, {tests:.4g} disk tests -> {va:.2f} TFLOP/s")
 
-    cpu = None
+    cpu_b = None
     parity = None
-    if not args.no_cpu_baseline:
+    if cpu:
         plane0 = out_d[0].cpu().numpy()
-        cpu, parity = cpu_baseline(wl, planes, lams[0], d, offsets, plane0, args.cpu_seconds)
-        log(f"[bench] cpu baseline {cpu} parity {parity}")
-
+        cpu_b, parity = cpu_baseline(wl, planes, lams[0], d, offsets, plane0, cpu_seconds)
+        log(f"[bench] cpu baseline {cpu_b} parity {parity}")
+    ctx.close()
     line = {
-        "metric": "Mpix·samples/s at 4K RGB N=256" if args.workload == "pixel-c2"
-        else f"Mpix·samples/s ({args.workload})",
+        "metric": "Mpix·samples/s at 4K RGB N=256" if workload == "pixel-c2" else f"Mpix·samples/s ({workload})",
         "value": round(value, 2),
         "unit": "Mpix·samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(secs / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -509,15 +503,80 @@ def main():
         "config": {"workload": wl["desc"], "planes": len(lam_d), "out": [d.out_w, d.out_h],
                    "n_samples": d.n_samples, "launch": cfg},
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_b,
     }
+    if valu is not None:
+        line["roofline_valu"] = valu
     if parity is not None:
         line["parity_spot_check"] = parity
-    ctx.close()
-    if args.workload == "pixel-c2" and not args.no_frame:
-        line["frame"] = frame_summary()
-    if args.workload == "pixel-c2" and not args.no_adaptive:
-        line["adaptive"] = adaptive_summary()
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4", "adaptive-c5"])
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the learned-budget comparison appended to the default line")
+    ap.add_argument("--no-frame", action="store_true",
+                    help="skip the host-memory frame comparison appended to the default line")
+    ap.add_argument("--no-grain", action="store_true",
+                    help="skip the grain-wise C3 summary appended to the default line")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven-batch C4 summary appended to the default line")
+    ap.add_argument("--batch-images", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
+    ap.add_argument("--pix-variant", type=int, default=None,
+                    help="pixel-wise formulation: 1 gather, 2 tile splat, 3 HBM splat")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB, so that counter "
+                         "runs (FETCH_SIZE, WRITE_SIZE) can be scaled by a copy of known size")
+    ap.add_argument("--no-stats", action="store_true",
+                    help="skip the instrumented pass (the roofline then has no FLOP count)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.workload == "batch-c4":
+        b = batch_summary(args.steps, args.warmup, args.batch_images)
+        line = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": b["workload"], "planes": b["planes"], "pixel_samples": b["pixel_samples"]},
+                "roofline": None, "cpu_baseline": None, "batch": b}
+        print(json.dumps(line), flush=True)
+        return
+    if args.workload == "adaptive-c5":
+        ad = adaptive_summary(args.steps, args.warmup)
+        line = {"metric": ad["metric"], "value": ad["adaptive_ms_per_frame"], "unit": "ms/frame", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": ad["adaptive_ms_per_frame"],
+                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": ad["workload"]}, "roofline": None,
+                "cpu_baseline": None, "adaptive": ad}
+        print(json.dumps(line), flush=True)
+        return
+
+    line = measure(args.workload, args.steps, args.warmup, cpu=not args.no_cpu_baseline,
+                   cpu_seconds=args.cpu_seconds, stats=not args.no_stats, tuning=not args.no_tuning,
+                   pix_variant=args.pix_variant, copy_calibration_mib=args.copy_calibration_mib)
+    if args.workload == "pixel-c2":
+        # the other configurations, after the headline's timed region
+        if not args.no_grain:
+            g = measure("grain-c3", max(2, args.steps), 1, cpu=not args.no_cpu_baseline,
+                        cpu_seconds=min(args.cpu_seconds, 8.0))
+            line["grain_c3"] = {k: g[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config",
+                                                  "roofline", "roofline_valu", "cpu_baseline") if k in g}
+        if not args.no_batch:
+            line["batch_c4"] = batch_summary(max(2, args.steps), 1, args.batch_images)
+        if not args.no_frame:
+            line["frame"] = frame_summary()
+        if not args.no_adaptive:
+            line["adaptive"] = adaptive_summary()
     print(json.dumps(line), flush=True)
 
 

@@ -4,6 +4,8 @@
 // whole batch pipeline can be captured in a HIP graph and replayed: the bin
 // scan, the work of each (plane, tile) item, the heaviest-first item order,
 // counter fills and the grain-capacity check.
+#include <algorithm>
+
 #include "fg_device.h"
 #include "fg_pixelwise_hbm.h"
 #include "fg_region.h"
@@ -179,6 +181,30 @@ hipError_t launch_batch_order(const BatchPlane *planes, int n_planes, const uint
     (void)scratch;
     if (n_items <= 0) return hipSuccess;
     hipLaunchKernelGGL(batch_order_kernel, dim3(1), dim3(1024), 0, s, planes, n_planes, w, n_items, items);
+    return hipGetLastError();
+}
+
+// *out += the sum of n u32 values in 64 bits (one atomic per workgroup).
+__global__ __launch_bounds__(kBlock) void sum_u64_kernel(const uint32_t *in, long long n, unsigned long long *out)
+{
+    __shared__ unsigned long long s_part[kBlock / 64];
+    unsigned long long acc = 0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        acc += in[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+hipError_t launch_sum_u64(const uint32_t *in, long long n, unsigned long long *out, hipStream_t s)
+{
+    const long long blocks = std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, 512));
+    hipLaunchKernelGGL(sum_u64_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, n, out);
     return hipGetLastError();
 }
 

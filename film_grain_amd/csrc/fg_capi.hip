@@ -302,7 +302,7 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
                                     {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
     const int q = sample_buckets(d->n_samples);
     const size_t words = (d->n_samples + 31) / 32;
-    const size_t queue = (size_t)splat_hbm_block(ctx) / 64 * fg::kQueueCap * 4;
+    const size_t queue = fg::splat_hbm_static_lds(splat_hbm_block(ctx));
     const size_t limit = (size_t)ctx->max_lds - queue;
     auto fits = [&](int w, int h) { return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) <= limit; };
     // room for two workgroups per CU (at N = 2048 one workgroup per CU made
@@ -613,13 +613,13 @@ int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::Pi
     uint32_t *cnt = static_cast<uint32_t *>(cnt_p);
     uint32_t *start = static_cast<uint32_t *>(start_p);
     unsigned long long *mask = static_cast<unsigned long long *>(mask_p);
-    // the plane's grain total, summed in 64 bits beside the u32 bin counts
-    // (cleared by a kernel: the caller may capture this into a graph)
-    a.grain_sum = ctx->d_sum64;
+    // the plane's grain total: the bin counts summed in 64 bits (cleared by
+    // a kernel: the caller may capture this into a graph)
     e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->d_sum64), 2, 0u, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(cnt + nbins, 1, 0u, s);
     if (e == hipSuccess) e = fg::launch_threshold(a, T, s);
     if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, mask, s);
+    if (e == hipSuccess) e = fg::launch_sum_u64(cnt, nbins, ctx->d_sum64, s);
     if (e == hipSuccess) e = fg::scan_bins(temp_p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
     unsigned long long total = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&total, ctx->d_sum64, 8, hipMemcpyDeviceToHost, s);
@@ -815,7 +815,6 @@ int pixel_args(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, const
     a.r_const = rc;
     a.r2_const = rc <= 0.0f ? -1.0f : rc * rc;
     a.inv_samples = 1.0f / (float)std::max<uint32_t>(d->n_samples, 1);
-    a.grain_sum = nullptr;
     a.grain_total = nullptr;
     a.grain_cap = 0;
     *out = a;
@@ -1281,6 +1280,7 @@ hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
         if (e == hipSuccess)
             e = fg::launch_bin_count(bp.a, bp.geo, b->T[p], b->cnt + b->bin_off[p], b->mask + 4 * b->bin_off[p], s);
     }
+    if (e == hipSuccess) e = fg::launch_sum_u64(b->cnt, b->total_bins, b->total64, s);
     if (e == hipSuccess) e = fg::launch_scan_u32(b->cnt, b->start, b->total_bins, b->partials, s);
     return e;
 }
@@ -1390,7 +1390,7 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     b->n_items = (int)items;
     b->lds = *std::max_element(lds_p.begin(), lds_p.end());
     b->block = splat_hbm_block(ctx);
-    if ((size_t)b->lds + (size_t)b->block / 64 * fg::kQueueCap * 4 > (size_t)ctx->max_lds)
+    if ((size_t)b->lds + fg::splat_hbm_static_lds(b->block) > (size_t)ctx->max_lds)
         return set_error(ctx, FG_EINVAL, "batch planes need more LDS than a workgroup has");
     const long long nb = b->total_bins;
     if ((rc = batch_alloc(b, (void **)&b->cnt, (size_t)nb * 4, what)) != FG_OK) return rc;
@@ -1401,7 +1401,6 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     if ((rc = batch_alloc(b, (void **)&b->items, (size_t)b->n_items * sizeof(int2), what)) != FG_OK) return rc;
     if ((rc = batch_alloc(b, (void **)&b->counter, 4, what)) != FG_OK) return rc;
     if ((rc = batch_alloc(b, (void **)&b->total64, 16, what)) != FG_OK) return rc;
-    for (int p = 0; p < n; ++p) b->planes[p].a.grain_sum = b->total64;
     if ((rc = batch_alloc(b, (void **)&b->d_planes, sizeof(fg::BatchPlane) * n, what)) != FG_OK) return rc;
 
     // counting pass over the current inputs sizes the grain array

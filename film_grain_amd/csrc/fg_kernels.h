@@ -45,10 +45,6 @@ struct PixelArgs {
     int offs_in_lds;            // splat: stage the offsets table in LDS
     int div_fast;               // 1: floor((a)/delta) via the checked reciprocal sequence
     float inv_zoom, zoom, rm, delta, rdelta, uscale, r_const, r2_const, inv_samples;
-    // the plane's (or batch's) grain total in 64 bits, which bin_count adds
-    // each bin's sum to (nullptr: not summed); the per-bin counts and their
-    // scan are u32, valid while this total stays below 2^32
-    unsigned long long *grain_sum;
     // batches: the grain total (device) and the grain array's capacity; when
     // the total exceeds it the grain and splat kernels do nothing (the
     // capacity check reports it).  grain_total == nullptr: unchecked.

@@ -133,15 +133,21 @@ hipError_t launch_batch_weights(const BatchPlane *planes, int n_planes, uint32_t
 // scratch holds 2 * 64 u32.
 hipError_t launch_batch_order(const BatchPlane *planes, int n_planes, const uint32_t *w, int n_items, int2 *items,
                               uint32_t *scratch, hipStream_t s);
-// Sets kStatusCapacity in *status when the batch's grain total (64-bit, as
-// bin_count sums it) exceeds cap.
+// *out += the sum of n u32 values, in 64 bits (the caller clears *out): a
+// plane's or batch's grain total from its bin counts.
+hipError_t launch_sum_u64(const uint32_t *in, long long n, unsigned long long *out, hipStream_t s);
+// Sets kStatusCapacity in *status when the batch's grain total (the 64-bit
+// sum of its bin counts) exceeds cap.
 hipError_t launch_capacity_check(const unsigned long long *total, uint32_t cap, int *status, hipStream_t s);
 // Sets kStatusLambda in *status when a lambda value is NaN, +inf or above
 // kMaxLambda (for the formulations whose kernels read lambda directly).
 hipError_t launch_lambda_check(const float *lambda, long long n, int *status, hipStream_t s);
 
-// LDS bytes the splat kernel needs beyond its static queues for one tile.
+// LDS bytes the splat kernel needs beyond its static arrays for one tile.
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
+// Static LDS of the splat kernels per workgroup of `block` threads: per wave a
+// candidate queue and 64 grain records.
+inline size_t splat_hbm_static_lds(int block) { return (size_t)block / 64 * (kQueueCap * 4 + 64 * 16) + 64; }
 hipError_t launch_splat_hbm(const PixelArgs &a, const PixGeom &g, const SampleTable &t, const uint32_t *bin_start,
                             const float4 *grains, int block, hipStream_t s);
 

@@ -183,11 +183,10 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
         total += x;
     }
     if (lane == 0) {
-        // a bin of 2^32 or more grains saturates; the plane's 64-bit sum then
-        // reaches 2^32 as well, and the host refuses the plane before any
-        // kernel uses the (wrapped) u32 scan
+        // a bin of 2^32 or more grains saturates (the plane's 64-bit sum of
+        // the bin counts, sum_bins_u64, then reaches 2^32 - 1 as well, and the
+        // host refuses the plane before any kernel uses the u32 scan)
         bin_count[bin] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
-        if (a.grain_sum && total) atomicAdd(a.grain_sum, total);
         if (total > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
     }
     if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
@@ -536,14 +535,15 @@ hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, 
     return hipGetLastError();
 }
 
-// Largest sample count whose splat tables hold 8-byte (word, bit) pairs.
-constexpr int kWordBitMaxN = 1024;
+// Steps of the splat's concatenated list walk whose list loads are in flight
+// together (4: 2% slower at C2, 12: 2% slower, 16: spills).
+constexpr int kWalkStage = 8;
 
 size_t splat_hbm_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {
     const size_t words = ((size_t)n_samples + 31) / 32;
     return align16((size_t)tile_w * tile_h * words * 4) + align16((size_t)n_samples * 16) +
-           align16((size_t)n_samples * (n_samples > kWordBitMaxN ? 2 : 8)) + align16((size_t)(qbuckets + 1) * 2);
+           align16((size_t)(qbuckets + 1) * 2);
 }
 
 
@@ -626,7 +626,8 @@ __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_re
 template <int BLOCK, bool LISTS, bool DEV_TABLE>
 __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &geo, const SampleTable &st_all,
                                            const uint32_t *bin_start, const float4 *grains, int tile,
-                                           unsigned char *lds, uint32_t (*s_queue)[kQueueCap])
+                                           unsigned char *lds, uint32_t (*s_queue)[kQueueCap],
+                                           float4 (*s_grec)[64])
 {
     constexpr int NW = BLOCK / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -668,25 +669,23 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
     const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
 
-    // ---- LDS: bitset | (xy, o)[N] | (word, bit)[N] | qidx[Q + 1] ----
-    // a sample's scaled offset and its offset in input pixels side by side:
-    // the direct test reads both with one 16-byte load.  Sample k's bit of
-    // tile pixel i is bit k % 32 of bitset word i * words + k / 32; per
-    // sorted position the LDS holds that word's byte offset for pixel 0 and
-    // the bit's mask, so a hit's address is one multiply-add.  Above
-    // kWordBitMaxN samples it holds k alone (2 B instead of 8, so that two
-    // workgroups still share a CU) and the readers derive the pair.
+    // ---- LDS: bitset | (o, word, bit)[N] | qidx[Q + 1] ----
+    // Sample k's bit of tile pixel i is bit k % 32 of bitset word i * words +
+    // k / 32.  Per sorted position one 16-byte record holds the sample's
+    // offset in input pixels, that word's byte offset for pixel 0 and the
+    // bit's mask: a test reads it with one load, and a hit's address is one
+    // multiply-add.  (Records of 24 bytes that also held the scaled offset,
+    // read with two loads, made the splat 9% slower.)
     const int tile_words = a.tile_w * a.tile_h * words;
     const uint32_t w4 = (uint32_t)words * 4u;
     size_t off = 0;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
     off += align16((size_t)tile_words * 4);
-    float4 *sxo = reinterpret_cast<float4 *>(lds + off);
+    // per sorted position one 16-byte record: the offset in input pixels and
+    // the bitset word's byte offset and bit mask; the scaled offset X = O *
+    // zoom is recomputed (the table's own f32 product)
+    float4 *srec = reinterpret_cast<float4 *>(lds + off);
     off += align16((size_t)N * 16);
-    const bool compact = N > kWordBitMaxN;  // uniform
-    uint2 *sw = reinterpret_cast<uint2 *>(lds + off);
-    uint16_t *sk = reinterpret_cast<uint16_t *>(lds + off);
-    off += align16((size_t)N * (compact ? 2 : 8));
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)(Q + 1) * 2);
     if (off > (size_t)a.lds_bytes) {
@@ -695,22 +694,21 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     }
     for (int w = tid; w < tile_words; w += BLOCK) bits[w] = 0u;
     for (int k = tid; k < N; k += BLOCK) {
-        const float2 X = st.xy[k], O = st.o[k];
-        sxo[k] = make_float4(X.x, X.y, O.x, O.y);
+        const float2 O = st.o[k];
         const uint32_t ki = st.k[k];
-        if (compact)
-            sk[k] = (uint16_t)ki;
-        else
-            sw[k] = make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
+        srec[k] = make_float4(O.x, O.y, __uint_as_float((ki >> 5) * 4u), __uint_as_float(1u << (ki & 31u)));
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
+    // (X, O) of sorted position p: the scaled offset and the offset in
+    // input pixels
+    auto XO_at = [&](int p) {
+        const float4 r = srec[p];
+        return make_float4(r.x * z, r.y * z, r.x, r.y);
+    };
     // the (word byte offset, bit mask) pair of sorted position p
     auto word_bit = [&](int p) {
-        if (compact) {
-            const uint32_t ki = sk[p];
-            return make_uint2((ki >> 5) * 4u, 1u << (ki & 31u));
-        }
-        return sw[p];
+        const float4 r = srec[p];
+        return make_uint2(__float_as_uint(r.z), __float_as_uint(r.w));
     };
     const float4 ext = *st.ext;
     // the tile's 64-grain chunks go to its waves on demand (the first NW
@@ -744,7 +742,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const uint32_t cell = (uint32_t)__shfl((int)gcell, l, 64);
         if (lane >= n) return;
         const int p = (int)((e >> 6) & 0x7FFu), tx = (int)((e >> 17) & 0x7Fu), ty = (int)(e >> 24);
-        const float4 XO = sxo[p];
+        const float4 XO = XO_at(p);
         const float2 O = make_float2(XO.z, XO.w);
         const uint2 wm = word_bit(p);
         const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
@@ -896,6 +894,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             const int excl = (int)(incl - fl);
             const int dl = p0 - excl;
             const float rs2 = rad * rad;
+            // the chunk's grains (centre, r^2) in this wave's LDS records: a
+            // step reads its owner's with one load instead of broadcasting
+            // three values (the wave's LDS accesses run in order, so no
+            // barrier is needed)
+            s_grec[wid][lane] = make_float4(G.x, G.y, rs2, 0.0f);
             int carry = 0;  // the last grain that starts before the step
             // the owner of each lane's entry of the step from s0, and the
             // entry's sorted position (lanes past the stream read its last
@@ -919,25 +922,33 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 const uint32_t pos = (uint32_t)from_lane_i(dl, owner << 2) + (uint32_t)min(s0 + lane, total - 1);
                 return (int)st.cl_list[pos];
             };
-            // the next step's entries load while the current step is tested
-            int own = owner_of(0);
-            int pn = entry(0, own);
-            for (int s0 = 0; s0 < total; s0 += 64) {
-                const int p = pn, owner = own;
-                if (s0 + 64 < total) {
-                    own = owner_of(s0 + 64);
-                    pn = entry(s0 + 64, own);
+            // kWalkStage steps at a time: their owners first, then all their
+            // list loads in flight together (steps past the stream read its
+            // last entry and test nothing), then their tests: one global
+            // latency per batch of steps instead of one per step
+            for (int sb = 0; sb < total; sb += 64 * kWalkStage) {
+                int own[kWalkStage], pn[kWalkStage];
+#pragma unroll
+                for (int i = 0; i < kWalkStage; ++i) own[i] = owner_of(sb + 64 * i);
+#pragma unroll
+                for (int i = 0; i < kWalkStage; ++i) pn[i] = entry(sb + 64 * i, own[i]);
+#pragma unroll
+                for (int i = 0; i < kWalkStage; ++i) {
+                    const int s0 = sb + 64 * i;
+                    if (s0 >= total) break;
+                    const int p = pn[i];
+                    // the owner grain's centre and r^2 from its record; its
+                    // target origin is the same f32 expression as bx, by above
+                    const float4 gr = s_grec[wid][own[i]];
+                    const float ocx = gr.x, ocy = gr.y, ors2 = gr.z;
+                    const float obx = ocx * z - 0.5f, oby = ocy * z - 0.5f;
+                    const float4 XO = XO_at(p);
+                    const uint2 wm = word_bit(p);
+                    // both reads issue before the test's branches (as in the
+                    // per-grain walk)
+                    asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
+                    test(lane < total - s0, XO, wm, obx, oby, ocx, ocy, ors2);
                 }
-                const int ba = owner << 2;
-                const float obx = from_lane_f(bx, ba), oby = from_lane_f(by, ba);
-                const float ocx = from_lane_f(G.x, ba), ocy = from_lane_f(G.y, ba);
-                const float ors2 = from_lane_f(rs2, ba);
-                const float4 XO = sxo[p];
-                const uint2 wm = word_bit(p);
-                // both reads issue before the test's branches (as in the
-                // per-grain walk)
-                asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
-                test(lane < total - s0, XO, wm, obx, oby, ocx, ocy, ors2);
             }
         }
         while (m) {
@@ -997,7 +1008,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
                     const int pa = position(up0 + ia), pb = position(up0 + ib);
-                    const float4 XOa = sxo[pa], XOb = sxo[pb];
+                    const float4 XOa = XO_at(pa), XOb = XO_at(pb);
                     const uint2 wa = word_bit(pa), wb = word_bit(pb);
                     // all the reads issue together, ahead of the tests (an
                     // empty asm that takes the values as operands keeps the
@@ -1013,7 +1024,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             for (int i0 = 0; i0 < ulen; i0 += 128) {
                 const int ia = i0 + lane, ib = ia + 64;
                 const int pa = position(up0 + ia), pb = position(up0 + ib);
-                const float4 Xa = sxo[pa], Xb = sxo[pb];
+                const float4 Xa = XO_at(pa), Xb = XO_at(pb);
                 step(ia, pa, make_float2(Xa.x, Xa.y));
                 if (i0 + 64 < ulen) step(ib, pb, make_float2(Xb.x, Xb.y));
             }
@@ -1051,8 +1062,9 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
+    __shared__ float4 s_grec[BLOCK / 64][64];
     const int tile = a.tile_list ? a.tile_list[blockIdx.x] : (int)blockIdx.x;
-    splat_tile<BLOCK, true, true>(a, geo, st, bin_start, grains, tile, lds, s_queue);
+    splat_tile<BLOCK, true, true>(a, geo, st, bin_start, grains, tile, lds, s_queue, s_grec);
 }
 
 // Persistent batch kernel: every workgroup takes (plane, tile) items from the
@@ -1064,6 +1076,7 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_queue[BLOCK / 64][kQueueCap];
+    __shared__ float4 s_grec[BLOCK / 64][64];
     __shared__ int s_item;
     for (;;) {
         __syncthreads();
@@ -1080,7 +1093,7 @@ __global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *pl
         // (generic loads kept them in VGPRs: 91 VGPRs against 68).
         const auto *cp = (const __attribute__((address_space(4))) BatchPlane *)planes + pl;
         const BatchPlane &p = *(const BatchPlane *)cp;
-        splat_tile<BLOCK, true, false>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue);
+        splat_tile<BLOCK, true, false>(p.a, p.geo, p.st, p.bin_start, p.grains, tile, lds, s_queue, s_grec);
     }
 }
 

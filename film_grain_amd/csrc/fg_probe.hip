@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kProbeThreads) void math_probe_kernel(int fn, const
         case kMathTanPi: r = tan(3.14159265358979323846 * x); break;  // Cauchy draw
         case kMathLogGamma: r = log_gamma(x); break;                  // rand_distr log_gamma
         case kMathSqrt: r = sqrt(x); break;
-        case kMathExpF32: r = (double)(float)fg_exp_dd(x); break;     // LogNormal radius as f32
+        case kMathExpF32: r = (double)exp_to_f32(x); break;           // LogNormal radius as f32
         default: r = 0.0; break;
         }
         out[i] = r;

@@ -33,11 +33,23 @@
 #include "fg_exp_glibc.h"
 #endif
 
+namespace fg {
+// a <= exp(t) (or a < exp(t) with strict set) as the reference decides it
+// with glibc's exp.  On the device the library's exp is within one unit of
+// glibc's, so it decides unless a lies within a few units of it; there
+// fg_exp_dd, glibc's but for its rare misroundings, decides.  (Using fg_exp_dd
+// throughout costs its table loads in every wave that has one such lane.)
+FG_HD bool cmp_exp(double a, double t, double scale, bool strict)
+{
 #if defined(__HIP_DEVICE_COMPILE__)
-#define FG_EXP(x) fg_exp_dd(x)
+    double e = scale * exp(t);
+    if (fabs(a - e) <= 1.0e-15 * fabs(e)) e = scale * fg_exp_dd(t);
 #else
-#define FG_EXP(x) exp(x)
+    const double e = scale * exp(t);
 #endif
+    return strict ? a < e : a <= e;
+}
+}  // namespace fg
 
 namespace fg {
 
@@ -190,9 +202,11 @@ FG_HD double poisson_rejection(Xoshiro &rng, double lambda)
             if (result >= 0.0) break;
         }
         result = floor(result);
-        const double check = 0.9 * (1.0 + comp * comp) *
-                             FG_EXP(result * log_lambda - log_gamma(1.0 + result) - magic);
-        if (standard_f64(rng.next()) <= check) break;
+        // check = 0.9 (1 + comp^2) exp(...), and the draw is accepted when
+        // U <= check
+        if (cmp_exp(standard_f64(rng.next()), result * log_lambda - log_gamma(1.0 + result) - magic,
+                    0.9 * (1.0 + comp * comp), false))
+            break;
     }
     return result;
 }
@@ -243,15 +257,33 @@ FG_HD double standard_normal(NextU64 &&next, const double *zx, const double *zf)
             }
             return u < 0.0 ? xx - FG_ZIG_NORM_R : FG_ZIG_NORM_R - xx;
         }
-        if (zf[i + 1] + (zf[i] - zf[i + 1]) * standard_f64(next()) < FG_EXP(-x * x / 2.0)) return x;
+        if (cmp_exp(zf[i + 1] + (zf[i] - zf[i + 1]) * standard_f64(next()), -x * x / 2.0, 1.0, true)) return x;
     }
+}
+
+// (float)exp(t) as the reference rounds glibc's exp(t) to f32.  On the device
+// the library's exp is within one unit of glibc's, so the two round to the
+// same f32 unless the double lies within a few units of an f32 rounding
+// boundary (its 29 bits below the f32 mantissa near 0x10000000: about one
+// draw in 10^8); there fg_exp_dd, glibc's but for its rare misroundings,
+// decides.
+FG_HD float exp_to_f32(double t)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double e = exp(t);
+    const uint32_t low = (uint32_t)((uint64_t)__double_as_longlong(e) & 0x1FFFFFFFull);
+    if (low - 0x0FFFFFF8u <= 16u) return (float)fg_exp_dd(t);
+    return (float)e;
+#else
+    return (float)exp(t);
+#endif
 }
 
 // LogNormal(mu, sigma).sample(rng) as f32 (src/model.rs:137-148).
 FG_HD float lognormal_f32(Xoshiro &rng, double mu, double sigma, const double *zx, const double *zf)
 {
     const double z = standard_normal([&rng]() { return rng.next(); }, zx, zf);
-    return (float)FG_EXP(mu + sigma * z);
+    return exp_to_f32(mu + sigma * z);
 }
 
 // Rust float->int `as` casts saturate and send NaN to 0.
