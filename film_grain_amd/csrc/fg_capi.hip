@@ -894,7 +894,7 @@ bool sorted_grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, 
                                     {16, 8},   {8, 8},   {8, 4},   {4, 4},   {2, 2},   {1, 1}};
     const int q = sample_buckets(d->n_samples);
     const size_t words = (d->n_samples + 31) / 32;
-    const size_t limit = (size_t)ctx->max_lds - (size_t)grain_block(ctx) / 64 * fg::kQueueCap * 4;
+    const size_t limit = (size_t)ctx->max_lds - fg::grainwise_sorted_static_lds(grain_block(ctx));
     auto fits = [&](int w, int h) { return fg::grainwise_sorted_lds(w, h, (int)d->n_samples, q) <= limit; };
     if (ctx->cfg.grain_tile_w > 0 && ctx->cfg.grain_tile_h > 0 && ctx->cfg.grain_tile_w <= 128 &&
         ctx->cfg.grain_tile_h <= 256 && fits(ctx->cfg.grain_tile_w, ctx->cfg.grain_tile_h)) {

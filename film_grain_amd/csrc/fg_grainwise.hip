@@ -326,12 +326,21 @@ hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, 
     return hipGetLastError();
 }
 
+// Entries of a round's concatenated candidate lists that the per-wave owner
+// map holds (256 B per wave keeps a 64x32 tile's workgroup at 40 KB of LDS,
+// four per CU).
+constexpr int kOwnerMap = 256;
+
 template <int KIND, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, SampleTable st)
 {
     constexpr int NW = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_queue[NW][kQueueCap];
+    // per wave, the owner lane of each entry of a round's concatenated lists
+    __shared__ uint8_t s_owner[NW][kOwnerMap];
+    // the region's next pixel that no lane has taken
+    __shared__ int s_next;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int tile = blockIdx.x;
@@ -365,6 +374,7 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
         sk[k] = st.k[k];
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
+    if (tid == 0) s_next = 0;
     const float4 ext = *st.ext;
 
     const float z = a.zoom, Rmax = a.r_out_max;
@@ -414,25 +424,45 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
         }
     };
 
-    for (int pb = wid * 64; pb < npx; pb += BLOCK) {
-        const int pix = pb + lane;
-        Xoshiro rng;
-        rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
-        uint32_t q = 0, done = 0;
-        float fx = 0.0f, fy = 0.0f;
-        if (pix < npx) {
-            const int x = xi0 + pix % rwi, y = yi0 + pix / rwi;
-            const size_t pi = (size_t)y * a.in_w + x;
-            q = a.pix_q[pi];
-            const ulonglong2 s01 = a.pix_s01[pi], s23 = a.pix_s23[pi];
-            rng.s0 = s01.x;
-            rng.s1 = s01.y;
-            rng.s2 = s23.x;
-            rng.s3 = s23.y;
-            fx = (float)x;
-            fy = (float)y;
+    // Each lane draws the grains of one region pixel at a time; a lane whose
+    // pixel has no grain left takes the region's next pixel, so the wave's
+    // lanes stay busy until the region runs out (pixel counts differ widely).
+    // Grains reach the bitset in another order than the reference's, which
+    // changes nothing: the bitset is an OR of the tests.
+    Xoshiro rng;
+    rng.s0 = rng.s1 = rng.s2 = rng.s3 = 0;
+    uint32_t q = 0, done = 0;
+    float fx = 0.0f, fy = 0.0f;
+    bool more = true;  // uniform: pixels of the region are left
+    for (;;) {
+        while (more) {
+            const bool need = done >= q;
+            const unsigned long long m = __ballot(need);
+            if (m == 0) break;
+            const int cnt = __popcll(m);
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&s_next, cnt);
+            base = __shfl(base, 0, 64);
+            more = base + cnt < npx;
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const int pix = base + rank;
+            if (need && pix < npx) {
+                const int y = yi0 + pix / rwi, x = xi0 + pix % rwi;
+                const size_t pi = (size_t)y * a.in_w + x;
+                q = a.pix_q[pi];
+                const ulonglong2 s01 = a.pix_s01[pi], s23 = a.pix_s23[pi];
+                rng.s0 = s01.x;
+                rng.s1 = s01.y;
+                rng.s2 = s23.x;
+                rng.s3 = s23.y;
+                done = 0;
+                fx = (float)x;
+                fy = (float)y;
+            }
         }
-        while (__any(done < q)) {
+        if (!__any(done < q)) break;
+        {
             // the lane's next grain (src/grainwise.rs:48-67)
             int p0 = 0, len = 0;
             float bx = 0.0f, by = 0.0f, R = 0.0f;
@@ -446,10 +476,14 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
                 bx = cx * z;
                 by = cy * z;
                 // skip radii <= 0 and grains whose boxes miss the tile for
-                // every sample (bounds() is monotone in the target)
-                if (rad > 0.0f && R > 0.0f && floorf(((bx + ext.y) + R) - 0.5f) >= fx0 &&
-                    ceilf(((bx + ext.x) - R) - 0.5f) <= fx1 && floorf(((by + ext.w) + R) - 0.5f) >= fy0 &&
-                    ceilf(((by + ext.z) - R) - 0.5f) <= fy1 && a.cell_q > 0) {
+                // every sample (bounds() is monotone in the target); one mask,
+                // not a chain of branches
+                const bool keep = ((int)(rad > 0.0f) & (int)(R > 0.0f) &
+                                   (int)(floorf(((bx + ext.y) + R) - 0.5f) >= fx0) &
+                                   (int)(ceilf(((bx + ext.x) - R) - 0.5f) <= fx1) &
+                                   (int)(floorf(((by + ext.w) + R) - 0.5f) >= fy0) &
+                                   (int)(ceilf(((by + ext.z) - R) - 0.5f) <= fy1)) != 0;
+                if (keep && a.cell_q > 0) {
                     // the candidate list of the cell of (A, B), the fractions
                     // of the grain's x and y targets (build_cell_lists)
                     const float ncx = R - (bx - 0.5f), ncy = R - (by - 0.5f);
@@ -460,9 +494,7 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
                     const int c = qb * a.cell_q + qa;
                     p0 = (int)a.cell_start[c];
                     len = (int)a.cell_start[c + 1] - p0;
-                } else if (rad > 0.0f && R > 0.0f && floorf(((bx + ext.y) + R) - 0.5f) >= fx0 &&
-                           ceilf(((bx + ext.x) - R) - 0.5f) <= fx1 && floorf(((by + ext.w) + R) - 0.5f) >= fy0 &&
-                           ceilf(((by + ext.z) - R) - 0.5f) <= fy1) {
+                } else if (keep) {
                     const float nc = R - (bx - 0.5f);
                     const float A = nc - floorf(nc);
                     const float lo = (A - (R + R)) - eps, hi = A + eps;
@@ -531,12 +563,26 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
                 // nondecreasing incl), and is its list entry p0 + e - excl.
                 const uint32_t incl = wave_inclusive_sum((uint32_t)len);
                 const int total = __builtin_amdgcn_readlane((int)incl, 63);
-                const int dl = p0 - ((int)incl - len);
+                const int excl = (int)incl - len;
+                const int dl = p0 - excl;
+                // Rounds of at most kOwnerMap entries (nearly all: about two
+                // per grain at C3) write each entry's owner lane into the
+                // wave's map, so that a step reads it with one load; longer
+                // rounds search the running totals.  The wave's LDS accesses
+                // run in order, so no barrier is needed.
+                const bool mapped = total <= kOwnerMap;  // uniform
+                if (mapped)
+                    for (int j = 0; j < len; ++j) s_owner[wid][excl + j] = (uint8_t)lane;
                 for (int s0 = 0; s0 < total; s0 += 64) {
                     const int e = s0 + lane;
                     int g = 0;
+                    if (mapped) {
+                        g = s_owner[wid][min(e, total - 1)];
+                    } else {
 #pragma unroll
-                    for (int b = 32; b >= 1; b >>= 1) g += from_lane_i((int)incl, (g + b - 1) << 2) <= e ? b : 0;
+                        for (int b = 32; b >= 1; b >>= 1)
+                            g += from_lane_i((int)incl, (g + b - 1) << 2) <= e ? b : 0;
+                    }
                     const bool ok = e < total;  // then g <= 63 and its list holds e
                     const int ga = g << 2;
                     const uint32_t pos = (uint32_t)(from_lane_i(dl, ga) + e);
@@ -636,6 +682,8 @@ hipError_t launch_cell_lists(const SampleTable &st, int n, int q, float r_out, f
     hipLaunchKernelGGL(cell_lists_kernel, dim3(1), dim3(1024), 0, s, st, n, q, r_out, c_base, start, list);
     return hipGetLastError();
 }
+
+size_t grainwise_sorted_static_lds(int block) { return (size_t)block / 64 * (kQueueCap * 4 + kOwnerMap) + 64; }
 
 size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets)
 {

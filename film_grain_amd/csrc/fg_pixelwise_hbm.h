@@ -74,6 +74,8 @@ hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind
 // table is built from the output-pixel offsets with zoom 1.  Needs N <= 2048
 // and tiles of at most 128 x 256.
 size_t grainwise_sorted_lds(int tile_w, int tile_h, int n_samples, int qbuckets);
+// Static LDS of the sorted grain-wise kernel per workgroup of `block` threads.
+size_t grainwise_sorted_static_lds(int block);
 // Per input pixel: grain count and stream state after the count draws
 // (a.pix_q, a.pix_s01, a.pix_s23), read by the sorted grain-wise kernel.
 hipError_t launch_pixel_state(const GrainArgs &a, hipStream_t s);
