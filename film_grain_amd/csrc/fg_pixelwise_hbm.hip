@@ -9,9 +9,9 @@
 //               reach) in 16x16-cell bins, one thread per cell: Poisson counts,
 //               summed per bin
 //   scan        exclusive scan of the bin counts (hipCUB) -> bin_start
-//   bin_grains  each bin's grains written contiguously at bin_start[bin]; the
-//               non-empty cells are compacted first so the divergent grain
-//               draws run on lanes that all have work
+//   bin_grains  each bin's grains written contiguously at bin_start[bin]: the
+//               non-empty cells' counts and streams first, then the draws,
+//               a lane taking the bin's next cell when its own is done
 //   samples     the N sample offsets ordered by the fraction of their scaled
 //               x offset, with a bucket index over that fraction
 //   lists       candidate lists over that order per (radius class, cell of a
@@ -109,19 +109,6 @@ __device__ __forceinline__ uint32_t cell_count_full(uint64_t h, double e, Xoshir
     return (uint32_t)r;
 }
 
-// False only for a cell whose Poisson count is 0 by the Knuth branch's first
-// draw (u1 <= exp(-lambda)), which needs just the two seed words that draw
-// depends on.  The rejection branch (e < 0) and the all-zero seed case
-// (reseeded by seed_from_u64) are left to the full count.
-__device__ __forceinline__ bool cell_maybe_nonempty(uint64_t h, double e)
-{
-    if (e > 1.5) return false;
-    if (e < 0.0) return true;
-    const uint64_t s0 = splitmix64(h), s3 = splitmix64(h + 3 * kGolden);
-    if ((s0 | s3) == 0) return true;
-    return standard_f64(rotl64(s0 + s3, 23) + s0) > e;
-}
-
 // Writes the wave's compacted list of the bin's nonempty cells (cell (l, q)
 // as q << 6 | l, in mask order) to list and returns their number.
 __device__ __forceinline__ int compact_cells(const unsigned long long m[4], uint16_t *list)
@@ -139,13 +126,18 @@ __device__ __forceinline__ int compact_cells(const unsigned long long m[4], uint
 }
 
 // Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
-// its 256 cells, the nonempty mask, and the grain count.
-__global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                           uint32_t *bin_count, unsigned long long *bin_mask)
+// its 256 cells, the nonempty mask, and the grain count.  The test's hash,
+// threshold and two seed words of each cell stay in LDS, so that the count
+// of a nonempty cell only adds the other two seed words.
+__global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
+                                                              uint32_t *bin_count, unsigned long long *bin_mask)
 {
-    __shared__ uint16_t s_list[kBlock / 64][256];
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ uint64_t s_h[NW][256], s_s0[NW][256], s_s3[NW][256];
+    __shared__ double s_e[NW][256];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * (kBlock / 64) + wid;
+    const int bin = blockIdx.x * NW + wid;
     if (bin >= g.nbx * g.nby) return;
     int ci0, cj0;
     bin_origin(g, bin, ci0, cj0);
@@ -159,7 +151,25 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
         if (ci < g.W && cj < g.H) {
             float sx, sy;
             const double e = cell_threshold_at(a, T, g.Ix0 + ci, g.Jy0 + cj, sx, sy);
-            ne = cell_maybe_nonempty(mix_row(col, g.Jy0 + cj), e);
+            const uint64_t h = mix_row(col, g.Jy0 + cj);
+            uint64_t s0 = 0, s3 = 0;
+            if (e <= 1.5) {
+                if (e < 0.0) {
+                    ne = true;  // rejection branch: seeded in full below
+                } else {
+                    s0 = splitmix64(h);
+                    s3 = splitmix64(h + 3 * kGolden);
+                    // (s0 | s3) == 0 may be the all-zero seed: counted in full
+                    ne = (s0 | s3) == 0 || standard_f64(rotl64(s0 + s3, 23) + s0) > e;
+                }
+            }
+            if (ne) {
+                const int slot = q << 6 | lane;
+                s_h[wid][slot] = h;
+                s_s0[wid][slot] = s0;
+                s_s3[wid][slot] = s3;
+                s_e[wid][slot] = e;
+            }
         }
         m[q] = __ballot(ne);
     }
@@ -169,13 +179,24 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
     for (int base = 0; base < n; base += 64) {
         uint32_t c = 0;
         if (base + lane < n) {
-            const int cell = s_list[wid][base + lane];
-            const int l = cell & 63, q = cell >> 6;
-            const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
-            float sx, sy;
-            const double e = cell_threshold_at(a, T, i, j, sx, sy);
+            const int slot = s_list[wid][base + lane];
+            const uint64_t h = s_h[wid][slot];
+            const double e = s_e[wid][slot];
             Xoshiro rng;
-            c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
+            if (e < 0.0) {
+                c = cell_count_full(h, e, rng);
+            } else {
+                rng.s0 = s_s0[wid][slot];
+                rng.s1 = splitmix64(h + kGolden);
+                rng.s2 = splitmix64(h + 2 * kGolden);
+                rng.s3 = s_s3[wid][slot];
+                if ((rng.s0 | rng.s1 | rng.s2 | rng.s3) == 0) rng.seed(h);  // seed_from_u64's reseed
+                double p = standard_f64(rng.next());
+                while (p > e) {
+                    p = p * standard_f64(rng.next());
+                    ++c;
+                }
+            }
         }
         unsigned long long x = c;
 #pragma unroll
@@ -192,17 +213,22 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(PixelArgs a, PixGeom 
     if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
 }
 
-// Pass 2: per bin (one wave), the nonempty cells from the mask, compacted onto
-// the lanes; each recomputes its count (leaving its stream after the count
-// draws) and writes its grains at bin_start[bin] + the wave's prefix.
+// Pass 2: per bin (one wave), the nonempty cells from the mask.  First each
+// compacted cell's count, with its stream after the count draws kept in LDS,
+// and the counts' prefix (a cell's grains go to bin_start[bin] + its prefix);
+// then the grain draws, a lane taking the bin's next cell whenever its own has
+// no grain left, so that the lanes stay busy although counts differ.
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
+__global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
                                                             const uint32_t *bin_start,
                                                             const unsigned long long *bin_mask, float4 *grains)
 {
-    __shared__ uint16_t s_list[kBlock / 64][256];
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ uint32_t s_pre[NW][256];  // the cell's count, then its exclusive prefix
+    __shared__ uint4 s_state[NW][2][256];  // the cell's stream after its count: (s0, s1), (s2, s3)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * (kBlock / 64) + wid;
+    const int bin = blockIdx.x * NW + wid;
     if (bin >= g.nbx * g.nby) return;
     if (a.grain_total && *a.grain_total > a.grain_cap) return;
     unsigned long long m[4];
@@ -211,26 +237,69 @@ __global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom
     const int n = compact_cells(m, s_list[wid]);
     int ci0, cj0;
     bin_origin(g, bin, ci0, cj0);
-    uint32_t out = bin_start[bin];
+    auto cell_at = [&](int idx, int &ci, int &cj) {
+        const int cell = s_list[wid][idx];
+        const int l = cell & 63, q = cell >> 6;
+        ci = ci0 + (l & 15);
+        cj = cj0 + 4 * (l >> 4) + q;
+    };
+    // counts and streams
+    uint32_t carry = 0;
     for (int base = 0; base < n; base += 64) {
         uint32_t c = 0;
-        int ci = 0, cj = 0;
-        float sx = 0.0f, sy = 0.0f;
-        Xoshiro rng{};
-        if (base + lane < n) {
-            const int cell = s_list[wid][base + lane];
-            const int l = cell & 63, q = cell >> 6;
-            ci = ci0 + (l & 15);
-            cj = cj0 + 4 * (l >> 4) + q;
+        const int idx = base + lane;
+        if (idx < n) {
+            int ci, cj;
+            cell_at(idx, ci, cj);
             const int i = g.Ix0 + ci, j = g.Jy0 + cj;
+            float sx, sy;
             const double e = cell_threshold_at(a, T, i, j, sx, sy);
+            Xoshiro rng{};
             c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
+            s_state[wid][0][idx] = make_uint4((uint32_t)rng.s0, (uint32_t)(rng.s0 >> 32), (uint32_t)rng.s1,
+                                              (uint32_t)(rng.s1 >> 32));
+            s_state[wid][1][idx] = make_uint4((uint32_t)rng.s2, (uint32_t)(rng.s2 >> 32), (uint32_t)rng.s3,
+                                              (uint32_t)(rng.s3 >> 32));
         }
         const uint32_t incl = wave_inclusive_sum(c);
-        float4 *dst = grains + out + (incl - c);
-        out += __shfl(incl, 63, 64);
-        const float pkf = __uint_as_float((uint32_t)ci | ((uint32_t)cj << 16));
-        for (uint32_t k = 0; k < c; ++k) {
+        if (idx < n) s_pre[wid][idx] = carry + incl - c;
+        carry += __shfl(incl, 63, 64);
+    }
+    const uint32_t out0 = bin_start[bin];
+    // draws: the lane's cell, its next grain k and its count kend
+    Xoshiro rng{};
+    uint32_t k = 0, kend = 0, dst = 0;
+    float sx = 0.0f, sy = 0.0f, pkf = 0.0f;
+    int next = 0;  // uniform: the bin's next cell no lane has taken
+    for (;;) {
+        while (next < n) {
+            const bool need = k >= kend;
+            const unsigned long long mk = __ballot(need);
+            if (mk == 0) break;
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+            const int idx = next + rank;
+            if (need && idx < n) {
+                int ci, cj;
+                cell_at(idx, ci, cj);
+                const uint32_t pre = s_pre[wid][idx];
+                const uint32_t nxt = idx + 1 < n ? s_pre[wid][idx + 1] : carry;
+                const uint4 u0 = s_state[wid][0][idx], u1 = s_state[wid][1][idx];
+                rng.s0 = (uint64_t)u0.x | (uint64_t)u0.y << 32;
+                rng.s1 = (uint64_t)u0.z | (uint64_t)u0.w << 32;
+                rng.s2 = (uint64_t)u1.x | (uint64_t)u1.y << 32;
+                rng.s3 = (uint64_t)u1.z | (uint64_t)u1.w << 32;
+                k = 0;
+                kend = nxt - pre;
+                dst = out0 + pre;
+                sx = (float)(g.Ix0 + ci) * a.delta;
+                sy = (float)(g.Jy0 + cj) * a.delta;
+                pkf = __uint_as_float((uint32_t)ci | ((uint32_t)cj << 16));
+            }
+            next = min(next + __popcll(mk), n);
+        }
+        if (!__any(k < kend)) break;
+        if (k < kend) {
             const float cx = sx + uniform_f32(rng, a.uscale);
             const float cy = sy + uniform_f32(rng, a.uscale);
             float rad = a.r_const;
@@ -238,7 +307,8 @@ __global__ __launch_bounds__(kBlock) void bin_grains_kernel(PixelArgs a, PixGeom
                 rad = lognormal_f32(rng, a.log_mu, a.log_sigma, d_zig_x, d_zig_f);
                 if (rad > a.rm) rad = a.rm;
             }
-            dst[k] = make_float4(cx, cy, rad, pkf);
+            grains[dst + k] = make_float4(cx, cy, rad, pkf);
+            ++k;
         }
     }
 }

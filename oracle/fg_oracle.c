@@ -1074,6 +1074,133 @@ int fgo_glibc_math(int fn, const double *in, int64_t n, double *out, int threads
     return 0;
 }
 
+/* ------------------------------------------------------------------------
+ * Colour workspaces (src/color.rs).  Pins (tests/test_oracle_colour.py):
+ * BT.709's defining identities on the constants and closed forms on pure
+ * colours (Y of white and grey, Cb of blue and Cr of red = 1/2, grey chroma
+ * = 0), the exact inverse over every 8-bit colour (split then merge returns
+ * it), to_u8 on every level k/255 and its saturation, and nearest resize as
+ * pixel replication at integer zoom and as the identity at equal size.
+ * ------------------------------------------------------------------------ */
+static const float FGO_YR = 0.2126f, FGO_YG = 0.7152f, FGO_YB = 0.0722f;  /* src/color.rs:9-11 */
+static const float FGO_CB_DENOM = 1.8556f, FGO_CR_DENOM = 1.5748f;         /* src/color.rs:12-13 */
+
+/* f32::clamp(0.0, 1.0): NaN passes through (src/color.rs:233-235) */
+static float fgo_clamp01(float v)
+{
+    if (v < 0.0f) return 0.0f;
+    if (v > 1.0f) return 1.0f;
+    return v;
+}
+
+/* src/color.rs:237-239: (clamp01(v) * 255 + 0.5).floor() as u8, where `as`
+ * saturates and sends NaN to 0 */
+uint8_t fgo_to_u8(float v)
+{
+    const float f = floorf(fgo_clamp01(v) * 255.0f + 0.5f);
+    if (!(f > 0.0f)) return 0;
+    if (f >= 255.0f) return 255;
+    return (uint8_t)f;
+}
+
+/* src/color.rs:158-213: load_rgb_workspace (mode 1) and load_luma_workspace
+ * (mode 0) on an interleaved RGB f32 image of n pixels. */
+int fgo_color_split(int mode, const float *rgb, uint64_t n, float *p0, float *p1, float *p2)
+{
+    if (mode != 0 && mode != 1) return -1;
+    for (uint64_t k = 0; k < n; ++k) {
+        const float r = fgo_clamp01(rgb[3 * k]);
+        const float g = fgo_clamp01(rgb[3 * k + 1]);
+        const float b = fgo_clamp01(rgb[3 * k + 2]);
+        if (mode == 1) {
+            p0[k] = r;
+            p1[k] = g;
+            p2[k] = b;
+            continue;
+        }
+        const float luma = FGO_YR * r + FGO_YG * g + FGO_YB * b;
+        p0[k] = fgo_clamp01(luma);
+        p1[k] = (b - luma) / FGO_CB_DENOM;
+        p2[k] = (r - luma) / FGO_CR_DENOM;
+    }
+    return 0;
+}
+
+/* Plane::resize_nearest (src/model.rs:77-98): source index
+ * round(clamp((x + 0.5) * (w / nw) - 0.5, 0, w - 1)), rounding half away from
+ * zero as f32::round does. */
+void fgo_resize_nearest(const float *src, uint32_t w, uint32_t h, uint32_t nw, uint32_t nh, float *dst)
+{
+    if (nw == w && nh == h) {
+        memcpy(dst, src, (size_t)w * h * sizeof(float));
+        return;
+    }
+    if (nw == 0 || nh == 0 || w == 0 || h == 0) {
+        memset(dst, 0, (size_t)nw * nh * sizeof(float));
+        return;
+    }
+    const float sx = (float)w / (float)nw, sy = (float)h / (float)nh;
+    for (uint32_t y = 0; y < nh; ++y) {
+        float fy = ((float)y + 0.5f) * sy - 0.5f;
+        fy = fy < 0.0f ? 0.0f : (fy > (float)(h - 1) ? (float)(h - 1) : fy);
+        const size_t syi = (size_t)roundf(fy);
+        for (uint32_t x = 0; x < nw; ++x) {
+            float fx = ((float)x + 0.5f) * sx - 0.5f;
+            fx = fx < 0.0f ? 0.0f : (fx > (float)(w - 1) ? (float)(w - 1) : fx);
+            dst[(size_t)y * nw + x] = src[syi * w + (size_t)roundf(fx)];
+        }
+    }
+}
+
+/* Workspace::into_rgb_image (src/color.rs:66-114): mode 0 takes Y at w x h
+ * and Cb/Cr at cw x ch (resized nearest to w x h when they differ); mode 1
+ * takes R, G, B at w x h.  rgb8 gets w x h interleaved 8-bit RGB. */
+int fgo_color_merge(int mode, const float *p0, const float *p1, const float *p2, uint32_t w, uint32_t h,
+                    uint32_t cw, uint32_t ch, uint8_t *rgb8)
+{
+    const size_t n = (size_t)w * h;
+    if (mode == 1) {
+        for (size_t k = 0; k < n; ++k) {
+            rgb8[3 * k] = fgo_to_u8(p0[k]);
+            rgb8[3 * k + 1] = fgo_to_u8(p1[k]);
+            rgb8[3 * k + 2] = fgo_to_u8(p2[k]);
+        }
+        return 0;
+    }
+    if (mode != 0) return -1;
+    float *cb = (float *)malloc(n * sizeof(float) + 1);
+    float *cr = (float *)malloc(n * sizeof(float) + 1);
+    if (!cb || !cr) {
+        free(cb);
+        free(cr);
+        return -1;
+    }
+    fgo_resize_nearest(p1, cw, ch, w, h, cb);
+    fgo_resize_nearest(p2, cw, ch, w, h, cr);
+    for (size_t k = 0; k < n; ++k) {
+        const float y = p0[k];
+        const float r = fgo_clamp01(y + FGO_CR_DENOM * cr[k]);
+        const float b = fgo_clamp01(y + FGO_CB_DENOM * cb[k]);
+        const float g = fgo_clamp01((y - FGO_YR * r - FGO_YB * b) / FGO_YG);
+        rgb8[3 * k] = fgo_to_u8(r);
+        rgb8[3 * k + 1] = fgo_to_u8(g);
+        rgb8[3 * k + 2] = fgo_to_u8(b);
+    }
+    free(cb);
+    free(cr);
+    return 0;
+}
+
+/* The BT.709 constants as the reference writes them (for the identity checks). */
+void fgo_color_constants(float out[5])
+{
+    out[0] = FGO_YR;
+    out[1] = FGO_YG;
+    out[2] = FGO_YB;
+    out[3] = FGO_CB_DENOM;
+    out[4] = FGO_CR_DENOM;
+}
+
 int fgo_max_threads(void)
 {
 #ifdef _OPENMP

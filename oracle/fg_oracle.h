@@ -15,6 +15,7 @@
  *                       resolve_output_size, Plane::get_clamped
  *   src/params.rs       derive_radius_parameters, default_cell_delta
  *   src/choose.rs       choose_algorithm
+ *   src/color.rs        colour split / merge, to_u8 (with model.rs resize_nearest)
  * Third-party algorithms restated (versions pinned by the reference Cargo.lock):
  *   rand 0.8.5        Xoshiro256PlusPlus (SmallRng), seed_from_u64, UniformFloat<f32>,
  *                     Standard f64, Open01 f64
@@ -166,6 +167,14 @@ int fgo_pixel_counts(const fgo_plane *p, const float *lambda, uint32_t *q_out, u
 /* The C library's f64 math element-wise (fn: 0 exp(-x), 1 exp(x), 2 log(x),
  * 3 tan(pi x), 4 log_gamma(x), 5 sqrt(x), 6 (double)(float)exp(x)). */
 int fgo_glibc_math(int fn, const double *in, int64_t n, double *out, int threads);
+
+/* ---- colour workspaces (src/color.rs, src/model.rs:77-98) ---- */
+uint8_t fgo_to_u8(float v);
+int fgo_color_split(int mode, const float *rgb, uint64_t n, float *p0, float *p1, float *p2); /* 0 luma, 1 rgb */
+void fgo_resize_nearest(const float *src, uint32_t w, uint32_t h, uint32_t nw, uint32_t nh, float *dst);
+int fgo_color_merge(int mode, const float *p0, const float *p1, const float *p2, uint32_t w, uint32_t h,
+                    uint32_t cw, uint32_t ch, uint8_t *rgb8);
+void fgo_color_constants(float out[5]); /* Kr, Kg, Kb, Cb denominator, Cr denominator */
 
 int fgo_max_threads(void);
 

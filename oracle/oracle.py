@@ -175,6 +175,17 @@ def lib():
     L.fgo_pixel_counts.restype = C.c_int
     L.fgo_glibc_math.argtypes = [C.c_int, _F64P, C.c_int64, _F64P, C.c_int]
     L.fgo_glibc_math.restype = C.c_int
+    L.fgo_to_u8.argtypes = [C.c_float]
+    L.fgo_to_u8.restype = C.c_uint8
+    L.fgo_color_split.argtypes = [C.c_int, _F32P, C.c_uint64, _F32P, _F32P, _F32P]
+    L.fgo_color_split.restype = C.c_int
+    L.fgo_resize_nearest.argtypes = [_F32P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _F32P]
+    L.fgo_resize_nearest.restype = None
+    L.fgo_color_merge.argtypes = [C.c_int, _F32P, _F32P, _F32P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.POINTER(C.c_uint8)]
+    L.fgo_color_merge.restype = C.c_int
+    L.fgo_color_constants.argtypes = [_F32P]
+    L.fgo_color_constants.restype = None
     _lib = L
     return L
 
@@ -420,3 +431,53 @@ def max_threads() -> int:
 
 if os.environ.get("FG_ORACLE_REBUILD"):
     build()
+
+
+# ---- colour workspaces (src/color.rs) ---------------------------------------
+COLOR_LUMA, COLOR_RGB = 0, 1
+
+
+def color_constants() -> np.ndarray:
+    """Kr, Kg, Kb and the Cb / Cr denominators as f32 (src/color.rs:9-13)."""
+    out = np.zeros(5, dtype=np.float32)
+    lib().fgo_color_constants(_f32p(out))
+    return out
+
+
+def to_u8(v) -> np.ndarray:
+    """src/color.rs:237-239, element-wise."""
+    v = np.asarray(v, dtype=np.float32)
+    f = lib().fgo_to_u8
+    return np.array([f(float(x)) for x in v.ravel()], dtype=np.uint8).reshape(v.shape)
+
+
+def color_split(rgb: np.ndarray, mode: int) -> np.ndarray:
+    """(h, w, 3) f32 -> (3, h, w): Y, Cb, Cr (mode 0) or R, G, B (mode 1)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    out = np.empty((3, h, w), dtype=np.float32)
+    if lib().fgo_color_split(mode, _f32p(rgb), h * w, _f32p(out[0]), _f32p(out[1]), _f32p(out[2])) != 0:
+        raise ValueError("oracle color_split failed")
+    return out
+
+
+def resize_nearest(src: np.ndarray, nw: int, nh: int) -> np.ndarray:
+    """Plane::resize_nearest (src/model.rs:77-98)."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape
+    out = np.empty((nh, nw), dtype=np.float32)
+    lib().fgo_resize_nearest(_f32p(src), w, h, nw, nh, _f32p(out))
+    return out
+
+
+def color_merge(mode: int, p0: np.ndarray, p1: np.ndarray, p2: np.ndarray) -> np.ndarray:
+    """Workspace::into_rgb_image: p0 at (h, w); p1, p2 at (ch, cw) (luma) or
+    (h, w) (rgb).  Returns (h, w, 3) u8."""
+    p0, p1, p2 = (np.ascontiguousarray(p, dtype=np.float32) for p in (p0, p1, p2))
+    h, w = p0.shape
+    ch, cw = p1.shape
+    out = np.empty((h, w, 3), dtype=np.uint8)
+    if lib().fgo_color_merge(mode, _f32p(p0), _f32p(p1), _f32p(p2), w, h, cw, ch,
+                             out.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+        raise ValueError("oracle color_merge failed")
+    return out

@@ -82,3 +82,26 @@ def oracle_plane(O, desc, n_samples=None):
     p.zoom, p.delta, p.rm, p.radius_mean = desc.zoom, desc.delta, desc.rm, desc.radius_mean
     p.log_mu, p.log_sigma = desc.log_mu, desc.log_sigma
     return p
+
+
+def oracle_frame(O, rgb, color_mode="luma", radius=0.1, radius_stddev=0.0, radius_dist="const", zoom=1.0,
+                 sigma=0.8, iters=32, seed=5489, algo="pixel"):
+    """A whole frame from the oracle alone: colour split (src/color.rs:158-213),
+    normalize_plane + lambda_plane and the integrator per rendered plane, and
+    the merge with the 8-bit quantisation (src/color.rs:66-114).  Returns
+    (rgb8 (h, w, 3), rendered planes (n, h, w))."""
+    mode = O.COLOR_RGB if color_mode == "rgb" else O.COLOR_LUMA
+    planes = O.color_split(rgb, mode)
+    outs = []
+    for i in range(3 if mode == O.COLOR_RGB else 1):
+        c = build_case(O, f"frame{i}", planes[i], radius, radius_stddev, radius_dist, zoom=zoom, sigma=sigma,
+                       n_samples=iters, seed=seed)
+        if algo == "pixel":
+            outs.append(O.render_pixelwise(c.plane, c.lam, c.offsets_input))
+        else:
+            outs.append(O.render_grainwise(c.plane, c.lam, c.offsets))
+    if mode == O.COLOR_RGB:
+        rgb8 = O.color_merge(mode, outs[0], outs[1], outs[2])
+    else:
+        rgb8 = O.color_merge(mode, outs[0], planes[1], planes[2])
+    return rgb8, np.stack(outs)

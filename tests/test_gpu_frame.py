@@ -1,11 +1,15 @@
 """Frames on the device (fg_render_frame*, fg_lambda_device, fg_color_*_device,
 fg_log_f32_device) against the host path, bit for bit: the f32 logarithm over
 its whole domain, the lambda pre-pass, the colour split and merge, and whole
-frames (luma with the chroma resize, RGB, both algorithms, an ROI)."""
+frames (luma with the chroma resize, RGB, both algorithms, an ROI); and the
+colour steps and whole frames against the oracle's restatement of
+src/color.rs, which is pinned independently of the host code."""
 import numpy as np
 import pytest
 
 from film_grain_amd import capi, host
+from oracle import oracle as O
+from tests import fgcases
 
 pytestmark = pytest.mark.gpu
 
@@ -121,3 +125,54 @@ def test_frame_on_device_equals_host_path(hip_ctx, i):
     hip_ctx.stream_status()
     assert np.array_equal(cp.cpu().numpy(), ap)
     assert np.array_equal(c8.cpu().numpy(), a8)
+
+
+def test_colour_split_and_merge_equal_oracle(hip_ctx):
+    """Device split and merge against the oracle: NaN and out-of-range
+    inputs, and the luma merge with Cb/Cr resized nearest from other sizes."""
+    import torch
+    rng = np.random.default_rng(12)
+    rgb = rng.uniform(-0.1, 1.1, (37, 53, 3)).astype(np.float32)
+    rgb[0, 0] = np.nan
+    rgb[1, 1, 2] = np.inf
+    rd = torch.from_numpy(rgb).to("cuda:0")
+    for mode in (O.COLOR_LUMA, O.COLOR_RGB):
+        want = O.color_split(rgb, mode)
+        p = [torch.empty((37, 53), dtype=torch.float32, device="cuda:0") for _ in range(3)]
+        hip_ctx.color_split_device(mode, rd, *p)
+        torch.cuda.synchronize()
+        for i in range(3):
+            assert np.array_equal(p[i].cpu().numpy(), want[i], equal_nan=True), (mode, i)
+    y, cb, cr = O.color_split(rgb, O.COLOR_LUMA)
+    for (h, w) in ((74, 106), (55, 79), (18, 26), (37, 53), (41, 50)):
+        yy = rng.uniform(-0.05, 1.05, (h, w)).astype(np.float32)
+        want = O.color_merge(O.COLOR_LUMA, yy, cb, cr)
+        out8 = torch.empty((h, w, 3), dtype=torch.uint8, device="cuda:0")
+        hip_ctx.color_merge_device(capi.COLOR_LUMA, torch.from_numpy(yy).to("cuda:0"),
+                                   torch.from_numpy(cb).to("cuda:0"), torch.from_numpy(cr).to("cuda:0"), out8)
+        torch.cuda.synchronize()
+        assert np.array_equal(out8.cpu().numpy(), want), (h, w)
+    planes = rng.uniform(-0.05, 1.05, (3, 37, 53)).astype(np.float32)
+    planes[1, 2, 3] = np.nan
+    out8 = torch.empty((37, 53, 3), dtype=torch.uint8, device="cuda:0")
+    pd = [torch.from_numpy(np.ascontiguousarray(planes[i])).to("cuda:0") for i in range(3)]
+    hip_ctx.color_merge_device(capi.COLOR_RGB, *pd, out8)
+    torch.cuda.synchronize()
+    assert np.array_equal(out8.cpu().numpy(), O.color_merge(O.COLOR_RGB, *planes))
+
+
+@pytest.mark.parametrize("i", [i for i in range(len(FRAMES)) if "roi" not in FRAMES[i]])
+def test_frame_on_device_equals_oracle(hip_ctx, i):
+    """A whole frame through the device entry point against a frame the
+    oracle alone composes (split, lambda, integrator, merge)."""
+    kw = dict(FRAMES[i])
+    args = host.CliArgs(**kw)
+    rng = np.random.default_rng(10 + i)
+    rgb = rng.random((34, 48, 3), dtype=np.float32)
+    rgb[5:12, 7:20] = 0.0
+    rgb[20:25, 30:40] = 0.98
+    got8, got_planes, _ = host.render_rgb(args, rgb, frame_on_device=True)
+    want8, want_planes = fgcases.oracle_frame(O, rgb, **{k: v for k, v in kw.items()})
+    print(f"[frame-oracle] {kw}: out {got8.shape}", flush=True)
+    assert np.array_equal(got_planes, want_planes), kw
+    assert np.array_equal(got8, want8), kw

@@ -189,6 +189,18 @@ def test_luma_roundtrip_recovers_colour_within_one_level():
     assert np.abs(out - np.rint(rgb * 255)).max() <= 1
 
 
+def test_host_colour_steps_equal_oracle(oracle):
+    """The host restatement's split and split-then-merge against the
+    oracle's (pinned by BT.709 closed forms in test_oracle_colour)."""
+    rng = np.random.default_rng(8)
+    rgb = rng.uniform(-0.1, 1.1, (23, 31, 3)).astype(np.float32)
+    rgb[2, 3] = np.nan
+    for name, mode in (("luma", oracle.COLOR_LUMA), ("rgb", oracle.COLOR_RGB)):
+        want = oracle.color_split(rgb, mode)
+        assert np.array_equal(host.color_split(rgb, name), want, equal_nan=True), name
+        assert np.array_equal(host.color_roundtrip(rgb, name), oracle.color_merge(mode, *want)), name
+
+
 def _png(path, arr, bitdepth=8, ctype=0):
     h, w = arr.shape[:2]
     dt = ">u2" if bitdepth == 16 else "u1"
