@@ -593,6 +593,20 @@ struct HbmGen {
     unsigned long long total = 0;
 };
 
+// The plane's grain total, refused when bin starts (u32 offsets) cannot
+// address it: a plane of 2^32 - 256 or more grains (a large cell size or a
+// bright plane) is refused before any kernel reads them (the splat's chunk
+// indices run up to 63 past the last grain).
+int check_total(fg_ctx *ctx, unsigned long long total, const char *what)
+{
+    if (total > kMaxGrains)
+        return set_error(ctx, FG_ECAPACITY,
+                         std::string(what) + ": the plane has " + std::to_string(total) + " grains, more than the " +
+                             std::to_string(kMaxGrains) + " one render addresses; use a smaller cell size or "
+                             "split the plane");
+    return FG_OK;
+}
+
 int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::PixGeom &g, int dist_kind,
                  hipStream_t s, HbmGen *out)
 {
@@ -625,14 +639,7 @@ int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::Pi
     if (e == hipSuccess) e = hipMemcpyAsync(&total, ctx->d_sum64, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    // bin starts are u32 offsets: a plane of 2^32 - 256 or more grains (a
-    // large cell size or a bright plane) is refused before any kernel reads
-    // them (the splat's chunk indices run up to 63 past the last grain)
-    if (total > kMaxGrains)
-        return set_error(ctx, FG_ECAPACITY,
-                         std::string(what) + ": the plane has " + std::to_string(total) + " grains, more than the " +
-                             std::to_string(kMaxGrains) + " one render addresses; use a smaller cell size or "
-                             "split the plane");
+    if ((rc = check_total(ctx, total, what)) != FG_OK) return rc;
     if ((rc = scratch_get(ctx, src, ctx->grains, std::max<size_t>((size_t)total, 1) * sizeof(float4), &grains_p,
                           what)) != FG_OK)
         return rc;

@@ -125,25 +125,33 @@ __device__ __forceinline__ int compact_cells(const unsigned long long m[4], uint
     return n;
 }
 
-// Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
-// its 256 cells, the nonempty mask, and the grain count.  The test's hash,
-// threshold and two seed words of each cell stay in LDS, so that the count
-// of a nonempty cell only adds the other two seed words.
-__global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                              uint32_t *bin_count, unsigned long long *bin_mask)
+// A bin's per-cell LDS records, one 32-byte record per cell in two uint4
+// halves: after the empty-cell test, at the cell's slot q << 6 | l, the hash,
+// two seed words and the threshold of each nonempty cell; after the counts,
+// at the cell's compacted index (never above its slot, so the rewrite in
+// place reads every record before it is overwritten), its stream after the
+// count draws.
+typedef uint4 BinRecs[2][256];
+
+__device__ __forceinline__ void put_u64x2(uint4 &r, uint64_t x, uint64_t y)
 {
-    constexpr int NW = kBlock / 64;
-    __shared__ uint16_t s_list[NW][256];
-    __shared__ uint64_t s_h[NW][256], s_s0[NW][256], s_s3[NW][256];
-    __shared__ double s_e[NW][256];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * NW + wid;
-    if (bin >= g.nbx * g.nby) return;
-    int ci0, cj0;
-    bin_origin(g, bin, ci0, cj0);
+    r = make_uint4((uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32));
+}
+__device__ __forceinline__ uint64_t lo64(const uint4 &r) { return (uint64_t)r.x | (uint64_t)r.y << 32; }
+__device__ __forceinline__ uint64_t hi64(const uint4 &r) { return (uint64_t)r.z | (uint64_t)r.w << 32; }
+
+// The empty-cell test of the bin's 256 cells: nonempty masks m (bit l of word
+// q for cell (l, q)) and the records of the nonempty cells.  A cell is empty
+// when its Poisson count is 0 by the Knuth branch's first draw
+// (u1 <= exp(-lambda)), which needs only the two seed words that draw
+// depends on; the rejection branch (e < 0) and the all-zero seed case
+// (reseeded by seed_from_u64) are counted in full.
+__device__ __forceinline__ void bin_test(const PixelArgs &a, const PixGeom &g, const double *T, int ci0, int cj0,
+                                         BinRecs &rec, unsigned long long m[4])
+{
+    const int lane = threadIdx.x & 63;
     const int ci = ci0 + (lane & 15), cjl = cj0 + 4 * (lane >> 4);
     const uint64_t col = mix_col(a.seed, g.Ix0 + ci);
-    unsigned long long m[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int cj = cjl + q;
@@ -155,41 +163,48 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGe
             uint64_t s0 = 0, s3 = 0;
             if (e <= 1.5) {
                 if (e < 0.0) {
-                    ne = true;  // rejection branch: seeded in full below
+                    ne = true;
                 } else {
                     s0 = splitmix64(h);
                     s3 = splitmix64(h + 3 * kGolden);
-                    // (s0 | s3) == 0 may be the all-zero seed: counted in full
                     ne = (s0 | s3) == 0 || standard_f64(rotl64(s0 + s3, 23) + s0) > e;
                 }
             }
             if (ne) {
                 const int slot = q << 6 | lane;
-                s_h[wid][slot] = h;
-                s_s0[wid][slot] = s0;
-                s_s3[wid][slot] = s3;
-                s_e[wid][slot] = e;
+                put_u64x2(rec[0][slot], h, s0);
+                put_u64x2(rec[1][slot], s3, (uint64_t)__double_as_longlong(e));
             }
         }
         m[q] = __ballot(ne);
     }
-    const int n = compact_cells(m, s_list[wid]);
-    // summed in 64 bits: one cell's count alone can reach 2^32 - 1
+}
+
+// Counts of the n compacted nonempty cells from their test records (the
+// count only adds seed words s1 and s2), each cell's stream after the count
+// rewritten into its record at its compacted index and, with pre set, the
+// counts' exclusive prefix in pre[idx].  Returns the bin's total in 64 bits
+// (one cell's count alone can reach 2^32 - 1).
+__device__ __forceinline__ unsigned long long bin_counts(const uint16_t *list, int n, BinRecs &rec, uint32_t *pre)
+{
+    const int lane = threadIdx.x & 63;
     unsigned long long total = 0;
     for (int base = 0; base < n; base += 64) {
+        const int idx = base + lane;
         uint32_t c = 0;
-        if (base + lane < n) {
-            const int slot = s_list[wid][base + lane];
-            const uint64_t h = s_h[wid][slot];
-            const double e = s_e[wid][slot];
-            Xoshiro rng;
+        Xoshiro rng{};
+        if (idx < n) {
+            const int slot = list[idx];
+            const uint4 r0 = rec[0][slot], r1 = rec[1][slot];
+            const uint64_t h = lo64(r0);
+            const double e = __longlong_as_double((long long)hi64(r1));
             if (e < 0.0) {
                 c = cell_count_full(h, e, rng);
             } else {
-                rng.s0 = s_s0[wid][slot];
+                rng.s0 = hi64(r0);
                 rng.s1 = splitmix64(h + kGolden);
                 rng.s2 = splitmix64(h + 2 * kGolden);
-                rng.s3 = s_s3[wid][slot];
+                rng.s3 = lo64(r1);
                 if ((rng.s0 | rng.s1 | rng.s2 | rng.s3) == 0) rng.seed(h);  // seed_from_u64's reseed
                 double p = standard_f64(rng.next());
                 while (p > e) {
@@ -197,76 +212,29 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGe
                     ++c;
                 }
             }
+            put_u64x2(rec[0][idx], rng.s0, rng.s1);
+            put_u64x2(rec[1][idx], rng.s2, rng.s3);
         }
+        const uint32_t incl = wave_inclusive_sum(c);
+        if (pre && idx < n) pre[idx] = (uint32_t)total + incl - c;
         unsigned long long x = c;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         total += x;
     }
-    if (lane == 0) {
-        // a bin of 2^32 or more grains saturates (the plane's 64-bit sum of
-        // the bin counts, sum_bins_u64, then reaches 2^32 - 1 as well, and the
-        // host refuses the plane before any kernel uses the u32 scan)
-        bin_count[bin] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
-        if (total > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
-    }
-    if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
+    return total;
 }
 
-// Pass 2: per bin (one wave), the nonempty cells from the mask.  First each
-// compacted cell's count, with its stream after the count draws kept in LDS,
-// and the counts' prefix (a cell's grains go to bin_start[bin] + its prefix);
-// then the grain draws, a lane taking the bin's next cell whenever its own has
-// no grain left, so that the lanes stay busy although counts differ.
+// The grain draws of the bin's n compacted cells, whose streams are in rec
+// and whose counts' exclusive prefix is in pre (total: the bin's sum): cell
+// idx's grains go to grains[out0 + pre[idx] ...].  A lane takes the bin's
+// next cell whenever its own has no grain left, so the lanes stay busy
+// although counts differ.
 template <int KIND>
-__global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                            const uint32_t *bin_start,
-                                                            const unsigned long long *bin_mask, float4 *grains)
+__device__ __forceinline__ void bin_draws(const PixelArgs &a, const PixGeom &g, int ci0, int cj0,
+                                          const uint16_t *list, int n, const BinRecs &rec, const uint32_t *pre,
+                                          uint32_t total, uint32_t out0, float4 *grains)
 {
-    constexpr int NW = kBlock / 64;
-    __shared__ uint16_t s_list[NW][256];
-    __shared__ uint32_t s_pre[NW][256];  // the cell's count, then its exclusive prefix
-    __shared__ uint4 s_state[NW][2][256];  // the cell's stream after its count: (s0, s1), (s2, s3)
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * NW + wid;
-    if (bin >= g.nbx * g.nby) return;
-    if (a.grain_total && *a.grain_total > a.grain_cap) return;
-    unsigned long long m[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) m[q] = bin_mask[(size_t)bin * 4 + q];
-    const int n = compact_cells(m, s_list[wid]);
-    int ci0, cj0;
-    bin_origin(g, bin, ci0, cj0);
-    auto cell_at = [&](int idx, int &ci, int &cj) {
-        const int cell = s_list[wid][idx];
-        const int l = cell & 63, q = cell >> 6;
-        ci = ci0 + (l & 15);
-        cj = cj0 + 4 * (l >> 4) + q;
-    };
-    // counts and streams
-    uint32_t carry = 0;
-    for (int base = 0; base < n; base += 64) {
-        uint32_t c = 0;
-        const int idx = base + lane;
-        if (idx < n) {
-            int ci, cj;
-            cell_at(idx, ci, cj);
-            const int i = g.Ix0 + ci, j = g.Jy0 + cj;
-            float sx, sy;
-            const double e = cell_threshold_at(a, T, i, j, sx, sy);
-            Xoshiro rng{};
-            c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
-            s_state[wid][0][idx] = make_uint4((uint32_t)rng.s0, (uint32_t)(rng.s0 >> 32), (uint32_t)rng.s1,
-                                              (uint32_t)(rng.s1 >> 32));
-            s_state[wid][1][idx] = make_uint4((uint32_t)rng.s2, (uint32_t)(rng.s2 >> 32), (uint32_t)rng.s3,
-                                              (uint32_t)(rng.s3 >> 32));
-        }
-        const uint32_t incl = wave_inclusive_sum(c);
-        if (idx < n) s_pre[wid][idx] = carry + incl - c;
-        carry += __shfl(incl, 63, 64);
-    }
-    const uint32_t out0 = bin_start[bin];
-    // draws: the lane's cell, its next grain k and its count kend
     Xoshiro rng{};
     uint32_t k = 0, kend = 0, dst = 0;
     float sx = 0.0f, sy = 0.0f, pkf = 0.0f;
@@ -280,18 +248,19 @@ __global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixG
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
             const int idx = next + rank;
             if (need && idx < n) {
-                int ci, cj;
-                cell_at(idx, ci, cj);
-                const uint32_t pre = s_pre[wid][idx];
-                const uint32_t nxt = idx + 1 < n ? s_pre[wid][idx + 1] : carry;
-                const uint4 u0 = s_state[wid][0][idx], u1 = s_state[wid][1][idx];
-                rng.s0 = (uint64_t)u0.x | (uint64_t)u0.y << 32;
-                rng.s1 = (uint64_t)u0.z | (uint64_t)u0.w << 32;
-                rng.s2 = (uint64_t)u1.x | (uint64_t)u1.y << 32;
-                rng.s3 = (uint64_t)u1.z | (uint64_t)u1.w << 32;
+                const int cell = list[idx];
+                const int l = cell & 63, q = cell >> 6;
+                const int ci = ci0 + (l & 15), cj = cj0 + 4 * (l >> 4) + q;
+                const uint32_t p0 = pre[idx];
+                const uint32_t p1 = idx + 1 < n ? pre[idx + 1] : total;
+                const uint4 r0 = rec[0][idx], r1 = rec[1][idx];
+                rng.s0 = lo64(r0);
+                rng.s1 = hi64(r0);
+                rng.s2 = lo64(r1);
+                rng.s3 = hi64(r1);
                 k = 0;
-                kend = nxt - pre;
-                dst = out0 + pre;
+                kend = p1 - p0;
+                dst = out0 + p0;
                 sx = (float)(g.Ix0 + ci) * a.delta;
                 sy = (float)(g.Jy0 + cj) * a.delta;
                 pkf = __uint_as_float((uint32_t)ci | ((uint32_t)cj << 16));
@@ -311,6 +280,77 @@ __global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixG
             ++k;
         }
     }
+}
+
+// Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
+// its 256 cells, the nonempty mask, and the grain count.
+__global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
+                                                              uint32_t *bin_count, unsigned long long *bin_mask)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ BinRecs s_rec[NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int bin = blockIdx.x * NW + wid;
+    if (bin >= g.nbx * g.nby) return;
+    int ci0, cj0;
+    bin_origin(g, bin, ci0, cj0);
+    unsigned long long m[4];
+    bin_test(a, g, T, ci0, cj0, s_rec[wid], m);
+    const int n = compact_cells(m, s_list[wid]);
+    const unsigned long long total = bin_counts(s_list[wid], n, s_rec[wid], nullptr);
+    if (lane == 0) {
+        // a bin of 2^32 or more grains saturates (the plane's 64-bit sum of
+        // the bin counts, sum_bins_u64, then reaches 2^32 - 1 as well, and the
+        // host refuses the plane before any kernel uses the u32 scan)
+        bin_count[bin] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
+        if (total > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
+    }
+    if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
+}
+
+// Pass 2: per bin (one wave), the nonempty cells from the mask, their counts
+// (seeded in full) and their grains at bin_start[bin].
+template <int KIND>
+__global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
+                                                               const uint32_t *bin_start,
+                                                               const unsigned long long *bin_mask, float4 *grains)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ uint32_t s_pre[NW][256];
+    __shared__ BinRecs s_rec[NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int bin = blockIdx.x * NW + wid;
+    if (bin >= g.nbx * g.nby) return;
+    if (a.grain_total && *a.grain_total > a.grain_cap) return;
+    unsigned long long m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = bin_mask[(size_t)bin * 4 + q];
+    const int n = compact_cells(m, s_list[wid]);
+    int ci0, cj0;
+    bin_origin(g, bin, ci0, cj0);
+    // counts from full seeds (the records of bin_counts' rewrite)
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += 64) {
+        uint32_t c = 0;
+        const int idx = base + lane;
+        if (idx < n) {
+            const int cell = s_list[wid][idx];
+            const int l = cell & 63, q = cell >> 6;
+            const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
+            float sx, sy;
+            const double e = cell_threshold_at(a, T, i, j, sx, sy);
+            Xoshiro rng{};
+            c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
+            put_u64x2(s_rec[wid][0][idx], rng.s0, rng.s1);
+            put_u64x2(s_rec[wid][1][idx], rng.s2, rng.s3);
+        }
+        const uint32_t incl = wave_inclusive_sum(c);
+        if (idx < n) s_pre[wid][idx] = carry + incl - c;
+        carry += __shfl(incl, 63, 64);
+    }
+    bin_draws<KIND>(a, g, ci0, cj0, s_list[wid], n, s_rec[wid], s_pre[wid], carry, bin_start[bin], grains);
 }
 
 // ---------------------------------------------------------------------------
@@ -1141,7 +1181,7 @@ __global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom g
 // batch's launch order through an atomic counter until the items run out
 // (each workgroup exits after reading an index past the end).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void splat_batch_kernel(const BatchPlane *planes, const int2 *items, int n_items,
+__global__ __launch_bounds__(BLOCK, 6) void splat_batch_kernel(const BatchPlane *planes, const int2 *items, int n_items,
                                                             unsigned int *counter)
 {
     extern __shared__ __align__(16) unsigned char lds[];

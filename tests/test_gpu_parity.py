@@ -106,6 +106,35 @@ def test_pixelwise_c2_full_frame_splat_equals_gather_and_oracle_tiles(hip_ctx, o
         _compare(f"c2_full:tile{x0},{y0}", a[y0:y1, x0:x1], want, case.n_samples)
 
 
+@pytest.mark.parametrize("dist", ["const", "lognorm"])
+def test_generation_buffers_reused_and_grown(oracle, dist):
+    """A plane rendered on a fresh context, and on a context whose grain
+    buffers a smaller plane sized first (so they grow) and then again (so
+    they are reused), gives the same result; random tiles match the oracle."""
+    from film_grain_amd import capi
+    big = fgcases.build_case(oracle, f"regrow_{dist}", synth.c2_rgb(7, 480, 272)[0], 0.1,
+                             0.05 if dist == "lognorm" else 0.0, dist, zoom=2.0, n_samples=64, seed=7)
+    small = fgcases.c1_case(oracle, size=32, n_samples=16)
+    first = capi.Context(0)
+    try:
+        a = _render(first, "hbm", big)
+    finally:
+        first.close()
+    ctx = capi.Context(0)
+    try:
+        _render(ctx, "hbm", small)
+        b = _render(ctx, "hbm", big)
+        c = _render(ctx, "hbm", big)
+    finally:
+        ctx.close()
+    _compare(f"{big.name}:grown_vs_fresh", b, a, big.n_samples)
+    _compare(f"{big.name}:reused_vs_fresh", c, a, big.n_samples)
+    rng = np.random.default_rng(3)
+    for (x0, y0, x1, y1) in fgcases.tiles(rng, big.derived.out_w, big.derived.out_h, 16, 6):
+        want = oracle.render_pixelwise(big.plane, big.lam, big.offsets_input, region=(x0, y0, x1, y1))
+        _compare(f"{big.name}:tile{x0},{y0}", c[y0:y1, x0:x1], want, big.n_samples)
+
+
 def test_grainwise_c3_small(hip_ctx, oracle):
     case = fgcases.c3_case(oracle, size=64, n_samples=128)
     want = oracle.render_grainwise(case.plane, case.lam, case.offsets)
