@@ -185,7 +185,7 @@ def cpu_baseline(wl, planes, lam0, derived, offs, gpu_plane0, budget_s=15.0):
         dict(tiles=n_tiles, pixels=n_px, mismatched_pixels=mism)
 
 
-def batch_summary(steps: int, warmup: int, n_images: int = 64):
+def batch_summary(steps: int, warmup: int, n_images: int = 64, tuning: bool = True):
     """The uneven batch: 64 images (sides 128-2048, mixed radii, N and
     channels), every plane resident in HBM; one graph-captured batch run per
     step, against the same planes rendered one call at a time."""
@@ -199,6 +199,8 @@ def batch_summary(steps: int, warmup: int, n_images: int = 64):
     px_samples = sum(d.out_w * d.out_h * d.n_samples for d, _, _ in ins)
     log(f"[bench] batch: {len(images)} images, {len(ins)} planes, {px_samples:.4g} pixel-samples")
     ctx = capi.Context(0)
+    if not tuning:
+        ctx.set_launch_table([])
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
     batch = capi.Batch(ctx, [(d, l, o, out) for (d, l, o), out in zip(ins, outs)], capture_graph=True)
@@ -396,12 +398,16 @@ def measure(workload: str, steps: int, warmup: int, cpu: bool, cpu_seconds: floa
         torch.cuda.synchronize()
         log(f"[bench] counters per plane {counters}")
 
-    tuned = autotune.apply_table(ctx, algo, desc) if tuning else None
+    # the library's built-in tuned table serves the render unless disabled
+    if not tuning:
+        ctx.set_launch_table([])
+    key = autotune.bucket_key(algo, desc)
+    tuned = tuning and any(capi.bucket_name(k) == key for k, _ in ctx.launch_table())
     if pix_variant is not None:
         ctx.set_launch_config(pix_variant=pix_variant)
-        tuned = None
+        tuned = False
     cfg = ctx.launch_config(desc, algo)
-    log(f"[bench] launch config {cfg} (tuned table entry: {tuned is not None})")
+    log(f"[bench] launch config {cfg} (bucket {key}, tuned table entry: {tuned})")
 
     for _ in range(warmup):
         for i, l in enumerate(lam_d):
@@ -529,7 +535,7 @@ def main():
     ap.add_argument("--batch-images", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-tuning", action="store_true", help="ignore the autotuned launch table")
+    ap.add_argument("--no-tuning", action="store_true", help="empty the library's tuned launch table (built-in choices)")
     ap.add_argument("--pix-variant", type=int, default=None,
                     help="pixel-wise formulation: 1 gather, 2 tile splat, 3 HBM splat")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -543,7 +549,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "batch-c4":
-        b = batch_summary(args.steps, args.warmup, args.batch_images)
+        b = batch_summary(args.steps, args.warmup, args.batch_images, tuning=not args.no_tuning)
         line = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "n_gpus": 1, "steps": args.steps,
                 "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -572,7 +578,7 @@ def main():
             line["grain_c3"] = {k: g[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config",
                                                   "roofline", "roofline_valu", "cpu_baseline") if k in g}
         if not args.no_batch:
-            line["batch_c4"] = batch_summary(max(2, args.steps), 1, args.batch_images)
+            line["batch_c4"] = batch_summary(max(2, args.steps), 1, args.batch_images, tuning=not args.no_tuning)
         if not args.no_frame:
             line["frame"] = frame_summary()
         if not args.no_adaptive:

@@ -18,13 +18,11 @@ TABLE_PATH = Path(__file__).resolve().parent / "tuning" / "launch_table.json"
 
 
 def bucket_key(algo: str, desc) -> str:
-    """Workload bucket: configurations tuned for one key are reused for all
-    planes in it."""
-    nb = 1 << max(0, math.ceil(math.log2(max(1, desc.n_samples))))
-    ratio = desc.rm / desc.delta if algo == "pixel" else desc.rm * desc.zoom
-    # half-octave buckets of the ratio
-    rb = round(2 ** (round(math.log2(max(ratio, 1e-3)) * 2) / 2), 3) if ratio > 0 else 0
-    return f"{algo}|{'lognorm' if desc.dist_kind else 'const'}|z{desc.zoom:g}|r{rb:g}|N{nb}"
+    """Workload bucket of a plane, as the library computes it
+    (fg_launch_bucket): configurations tuned for one key serve every plane
+    in it."""
+    from film_grain_amd import capi
+    return capi.bucket_name(capi.launch_bucket(desc, algo))
 
 
 def pixel_space(desc):
@@ -107,14 +105,53 @@ def save_table(table: dict, path: Path = TABLE_PATH) -> None:
     Path(path).write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
 
 
-def apply_table(ctx, algo, desc, table: dict | None = None) -> dict | None:
-    """Sets the tuned launch config for this workload bucket, if one is known."""
+def table_entries(table: dict) -> list:
+    """The JSON table as fg_set_launch_table entries: (LaunchKey, config)."""
+    from film_grain_amd import capi
+    from film_grain_amd.build import parse_bucket
+    out = []
+    for key, entry in sorted(table.items()):
+        algo, dist, zq, rh, nb = parse_bucket(key)
+        out.append((capi.LaunchKey(algo, dist, zq, rh, nb), dict(entry["config"])))
+    return out
+
+
+def apply_table(ctx, table: dict | None = None) -> int:
+    """Installs a table (default: the JSON file as it is now, which may be
+    newer than the table the library was built with) into a context; returns
+    its size.  The library's own table serves every context without this."""
     table = load_table() if table is None else table
-    cfg = table.get(bucket_key(algo, desc))
-    if cfg:
-        ctx.set_launch_config(**cfg["config"])
-        return cfg["config"]
-    return None
+    entries = table_entries(table)
+    ctx.set_launch_table(entries)
+    return len(entries)
+
+
+def tune_batch_c4(ctx, reps: int, log) -> dict:
+    """One entry per bucket of the uneven batch's planes (64 images, sides
+    128-2048, mixed radii, N and channels): each bucket tuned on its largest
+    plane."""
+    import numpy as np
+    import torch
+    from film_grain_amd import workloads
+    planes = workloads.prepare_batch(workloads.batch_images(master_seed=2024, n_images=64))
+    best_plane = {}
+    for d, lam, offs in planes:
+        key = bucket_key("pixel", d)
+        if key not in best_plane or d.out_w * d.out_h > best_plane[key][0].out_w * best_plane[key][0].out_h:
+            best_plane[key] = (d, lam, offs)
+    dev = torch.device("cuda:0")
+    out_table = {}
+    for key, (d, lam, offs) in sorted(best_plane.items()):
+        lam_t = torch.from_numpy(lam).to(dev)
+        offs_t = torch.from_numpy(np.ascontiguousarray(offs)).to(dev)
+        out = torch.empty((d.out_h, d.out_w), dtype=torch.float32, device=dev)
+        best, results = tune(ctx, "pixel", d, lam_t, offs_t, out, reps=reps)
+        if best is None:
+            continue
+        out_table[key] = {"config": best, "ms": round(results[0][0], 3), "candidates": len(results),
+                          "workload": f"uneven batch plane {d.in_w}x{d.in_h}, N={d.n_samples}"}
+        log(f"[tune] {key}: {best} {results[0][0]:.3f} ms (of {len(results)})")
+    return out_table
 
 
 def main(argv=None):
@@ -131,12 +168,23 @@ def main(argv=None):
     from film_grain_amd.workloads import WORKLOADS, prepare
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="pixel-c2", choices=sorted(WORKLOADS) + ["batch-c4"])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--save", action="store_true", help="merge the winner into the launch table")
     ap.add_argument("--table", default=str(TABLE_PATH))
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "batch-c4":
+        ctx = capi.Context(0)
+        ctx.set_launch_table([])  # time the candidates, not the table
+        found = tune_batch_c4(ctx, args.reps, log)
+        ctx.close()
+        print(json.dumps(found))
+        if args.save:
+            table = load_table(Path(args.table))
+            table.update(found)
+            save_table(table, Path(args.table))
+        return 0
     prep = prepare(args.workload)
     dev = torch.device("cuda:0")
     ctx = capi.Context(0)

@@ -1,5 +1,6 @@
-"""Builds libfilmgrain_hip.so (gfx950 kernels + C ABI + C++ host library) and
-the filmgrain CLI, in-tree, with hipcc.  Cross-compiles without a GPU.
+"""Builds libfilmgrain_hip.so (gfx950 kernels + C ABI + C++ host library)
+in-tree with hipcc (the CLI is `python -m film_grain_amd`).  Cross-compiles
+without a GPU.
 
     python -m film_grain_amd.build          # or film_grain_amd.build.build()
 """
@@ -17,7 +18,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libfilmgrain_hip.so"
-CLI = PKG / "bin" / "filmgrain"
+TUNING_TABLE = PKG / "tuning" / "launch_table.json"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FG_OFFLOAD_ARCH", "gfx950")
@@ -95,10 +96,50 @@ def exp_table() -> Path:
     return header
 
 
+LAUNCH_FIELDS = ["pix_tile_w", "pix_tile_h", "pix_lds_bytes", "grain_tile_w", "grain_tile_h", "grain_lds_bytes",
+                 "grain_grains_per_thread", "pix_variant", "pix_block", "grain_variant", "grain_block"]
+
+
+def parse_bucket(key: str) -> tuple[int, int, int, int, int]:
+    """'pixel|lognorm|zq4|rh4|N256' -> (algo, dist_kind, zoom_q, ratio_h, n_bucket)."""
+    algo, dist, zq, rh, nb = key.split("|")
+    if algo not in ("pixel", "grain") or dist not in ("const", "lognorm") or not zq.startswith("zq") \
+            or not rh.startswith("rh") or not nb.startswith("N"):
+        raise ValueError(f"bad launch table key {key!r}")
+    return (0 if algo == "pixel" else 1, 0 if dist == "const" else 1, int(zq[2:]), int(rh[2:]), int(nb[1:]))
+
+
+def launch_table_header() -> Path:
+    """The tuned launch table as the library's built-in table (C header)."""
+    import json
+    table = json.loads(TUNING_TABLE.read_text()) if TUNING_TABLE.exists() else {}
+    rows = []
+    for key in sorted(table):
+        algo, dist, zq, rh, nb = parse_bucket(key)
+        cfg = table[key]["config"]
+        unknown = set(cfg) - set(LAUNCH_FIELDS)
+        if unknown:
+            raise ValueError(f"launch table entry {key!r}: unknown fields {sorted(unknown)}")
+        vals = ", ".join(str(int(cfg.get(f, 0))) for f in LAUNCH_FIELDS)
+        rows.append(f"    {{{{{algo}u, {dist}u, {zq}, {rh}, {nb}u}}, {{{vals}}}}},  // {key}")
+    header = BUILD / "fg_launch_table.h"
+    text = ("// Generated by film_grain_amd/build.py from tuning/launch_table.json: the\n"
+            "// library's built-in tuned launch table (fg_launch_entry, declared in\n"
+            "// film_grain_hip.h, which the including file includes first).\n"
+            "#pragma once\n"
+            "static const fg_launch_entry kBuiltinLaunchTable[] = {\n"
+            + ("\n".join(rows) if rows else "    {}") + "\n};\n"
+            f"static const int kBuiltinLaunchCount = {len(rows)};\n")
+    if not header.exists() or header.read_text() != text:
+        header.write_text(text)
+    return header
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     logf_table()
     exp_table()
+    launch_table_header()
     jobs = []
     objs = []
     for src in HIP_SOURCES:
@@ -114,11 +155,6 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, *DEVICE, "-shared", "-fPIC", *LINK_SANITIZE, "-o", str(tmp), *map(str, objs), "-lz", "-lpthread"])
     os.replace(tmp, LIB)
-    cli_src = CSRC / "cli" / "filmgrain_main.cpp"
-    if cli_src.exists():
-        CLI.parent.mkdir(exist_ok=True)
-        _run([HIPCC, *COMMON, "-o", str(CLI), str(cli_src), f"-L{PKG}", "-lfilmgrain_hip",
-              "-Wl,-rpath,$ORIGIN/..", "-lz", "-lpthread"])
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -126,7 +162,7 @@ def build(verbose: bool = False) -> Path:
 
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
-    for p in (LIB, CLI):
+    for p in (LIB,):
         if p.exists():
             p.unlink()
 

@@ -37,7 +37,7 @@ EXPORTED = [
     "fg_render_pixelwise_adaptive_device", "fg_render_frame_device", "fg_render_frame",
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
     "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_render_pixelwise_device_noalloc",
-    "fg_math_device",
+    "fg_math_device", "fg_launch_bucket", "fg_set_launch_table", "fg_get_launch_table",
     "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
@@ -73,6 +73,15 @@ class LaunchConfig(C.Structure):
         ("grain_grains_per_thread", C.c_int), ("pix_variant", C.c_int), ("pix_block", C.c_int),
         ("grain_variant", C.c_int), ("grain_block", C.c_int),
     ]
+
+
+class LaunchKey(C.Structure):
+    _fields_ = [("algo", C.c_uint32), ("dist_kind", C.c_uint32), ("zoom_q", C.c_int32), ("ratio_h", C.c_int32),
+                ("n_bucket", C.c_uint32)]
+
+
+class LaunchEntry(C.Structure):
+    _fields_ = [("key", LaunchKey), ("cfg", LaunchConfig)]
 
 
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
@@ -131,6 +140,12 @@ def lib() -> C.CDLL:
     L.fg_set_launch_config.restype = C.c_int
     L.fg_get_launch_config.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchConfig)]
     L.fg_get_launch_config.restype = C.c_int
+    L.fg_launch_bucket.argtypes = [C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchKey)]
+    L.fg_launch_bucket.restype = C.c_int
+    L.fg_set_launch_table.argtypes = [vp, C.POINTER(LaunchEntry), C.c_int]
+    L.fg_set_launch_table.restype = C.c_int
+    L.fg_get_launch_table.argtypes = [vp, C.POINTER(LaunchEntry), C.c_int]
+    L.fg_get_launch_table.restype = C.c_int
     L.fg_set_timing.argtypes = [vp, C.c_int]
     L.fg_set_timing.restype = C.c_int
     L.fg_get_timings.argtypes = [vp, C.POINTER(C.c_float)]
@@ -461,6 +476,26 @@ class Context:
     def reset_launch_config(self):
         self._check(self._L.fg_set_launch_config(self.handle, None))
 
+    def set_launch_table(self, entries=None):
+        """Installs a tuned launch table: a list of (LaunchKey, config dict);
+        [] empties it; None restores the library's built-in table."""
+        if entries is None:
+            self._check(self._L.fg_set_launch_table(self.handle, None, -1))
+            return
+        arr = (LaunchEntry * max(1, len(entries)))()
+        for i, (key, cfg) in enumerate(entries):
+            arr[i].key = key
+            for k, v in cfg.items():
+                setattr(arr[i].cfg, k, int(v))
+        self._check(self._L.fg_set_launch_table(self.handle, arr, len(entries)))
+
+    def launch_table(self) -> list:
+        """The context's table as (LaunchKey, config dict) pairs."""
+        n = self._L.fg_get_launch_table(self.handle, None, 0)
+        arr = (LaunchEntry * max(1, n))()
+        self._L.fg_get_launch_table(self.handle, arr, n)
+        return [(arr[i].key, {f: getattr(arr[i].cfg, f) for f, _ in LaunchConfig._fields_}) for i in range(n)]
+
     def launch_config(self, desc: PlaneDesc, algo: str) -> dict:
         cfg = LaunchConfig()
         self._check(self._L.fg_get_launch_config(self.handle, C.byref(desc), 1 if algo == "grain" else 2,
@@ -519,3 +554,19 @@ def _stream_handle(stream) -> int:
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
+
+
+def launch_bucket(desc: PlaneDesc, algo: str) -> LaunchKey:
+    """The tuned launch table's bucket of a plane (fg_launch_bucket; no
+    device needed)."""
+    key = LaunchKey()
+    rc = lib().fg_launch_bucket(C.byref(desc), 1 if algo == "grain" else 0, C.byref(key))
+    if rc != FG_OK:
+        raise FilmGrainError(rc, "fg_launch_bucket failed")
+    return key
+
+
+def bucket_name(key: LaunchKey) -> str:
+    """'pixel|lognorm|zq4|rh4|N256': the launch table's JSON key."""
+    return (f"{'grain' if key.algo == 1 else 'pixel'}|{'lognorm' if key.dist_kind else 'const'}|"
+            f"zq{key.zoom_q}|rh{key.ratio_h}|N{key.n_bucket}")

@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/film_grain_hip.h"
 #include "fg_frame.h"
@@ -22,6 +23,7 @@
 #include "fg_mlp.h"
 #include "fg_pixelwise_hbm.h"
 #include "fg_rng.h"
+#include "fg_launch_table.h"
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -77,6 +79,8 @@ struct fg_ctx {
     bool timed = false;
     hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
     fg_launch_config cfg{};
+    bool cfg_explicit = false;            // cfg set by the caller: it replaces the table
+    std::vector<fg_launch_entry> table;   // tuned launch shapes per workload bucket
     std::map<uint32_t, bool> div_checked;  // cell size -> reciprocal division verified
 };
 
@@ -272,14 +276,49 @@ void choose_splat_tile(const fg_plane_desc *d, int lds, int *tw, int *th)
 // in LDS (18 bytes per sample plus the bucket index).
 constexpr uint32_t kHbmMaxSamples = 2048;
 
+// The workload bucket of a plane for the tuned launch table.
+fg_launch_key launch_key(const fg_plane_desc *d, int algo)
+{
+    fg_launch_key k{};
+    k.algo = algo == 1 ? 1u : 0u;
+    k.dist_kind = d->dist_kind ? 1u : 0u;
+    const double zoom = (double)d->zoom;
+    k.zoom_q = zoom > 0.0 ? (int32_t)std::floor(4.0 * std::log2(zoom) + 0.5) : -1000;
+    const double ratio = algo == 1 ? (double)d->rm * zoom : (d->delta > 0.0f ? (double)d->rm / (double)d->delta : 0.0);
+    k.ratio_h = ratio > 0.0 ? (int32_t)std::floor(2.0 * std::log2(ratio) + 0.5) : -1000;
+    uint32_t nb = 1;
+    while (nb < d->n_samples && nb < 0x80000000u) nb <<= 1;
+    k.n_bucket = nb;
+    return k;
+}
+
+// The launch shape of a render: the caller's explicit configuration, else the
+// tuned table's entry for the plane's bucket, else all zeros (the built-in
+// choices below).
+const fg_launch_config &launch_cfg(const fg_ctx *ctx, const fg_plane_desc *d, int algo)
+{
+    static const fg_launch_config none{};
+    if (ctx->cfg_explicit) return ctx->cfg;
+    const fg_launch_key k = launch_key(d, algo);
+    for (const fg_launch_entry &e : ctx->table)
+        if (e.key.algo == k.algo && e.key.dist_kind == k.dist_kind && e.key.zoom_q == k.zoom_q &&
+            e.key.ratio_h == k.ratio_h && e.key.n_bucket == k.n_bucket)
+            return e.cfg;
+    return none;
+}
+
 int pixel_variant(fg_ctx *ctx, const fg_plane_desc *d)
 {
-    if (ctx->cfg.pix_variant == 1 || d->n_samples > 65535) return 1;
-    if (ctx->cfg.pix_variant == 2 || d->n_samples > kHbmMaxSamples) return 2;
+    const fg_launch_config &cfg = launch_cfg(ctx, d, 0);
+    if (cfg.pix_variant == 1 || d->n_samples > 65535) return 1;
+    if (cfg.pix_variant == 2 || d->n_samples > kHbmMaxSamples) return 2;
     return 3;
 }
 
-int splat_hbm_block(const fg_ctx *ctx) { return ctx->cfg.pix_block == 256 ? 256 : 512; }
+int splat_hbm_block(const fg_ctx *ctx, const fg_plane_desc *d)
+{
+    return launch_cfg(ctx, d, 0).pix_block == 256 ? 256 : 512;
+}
 
 // Buckets of the sample order's fraction index: the next power of two >= N,
 // within [64, 4096].
@@ -290,19 +329,21 @@ int sample_buckets(uint32_t n)
     return q;
 }
 
-// HBM splat kernel: the configured tile, else a tile of the ladder chosen
+// HBM splat kernel (block threads per workgroup, 0: the configured size): the
+// configured tile, else a tile of the ladder chosen
 // below (with min_tiles > 0, stepped down until the plane has that many
 // tiles); the LDS is exactly what the tile and the sample table need.
 // Returns false when no tile fits the workgroup LDS limit.  Queue entries
 // hold a tile column in 7 bits and a row in 8 bits, so tiles are at most
 // 128 x 256.
-bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, long long min_tiles)
+bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, long long min_tiles, int block = 0)
 {
     static const int ladder[][2] = {{64, 32}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8},
                                     {8, 4},   {4, 4},   {4, 2},   {2, 2},   {2, 1}, {1, 1}};
     const int q = sample_buckets(d->n_samples);
     const size_t words = (d->n_samples + 31) / 32;
-    const size_t queue = fg::splat_hbm_static_lds(splat_hbm_block(ctx));
+    const fg_launch_config &cfg = launch_cfg(ctx, d, 0);
+    const size_t queue = fg::splat_hbm_static_lds(block > 0 ? block : splat_hbm_block(ctx, d));
     const size_t limit = (size_t)ctx->max_lds - queue;
     auto fits = [&](int w, int h) { return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) <= limit; };
     // room for two workgroups per CU (at N = 2048 one workgroup per CU made
@@ -310,10 +351,10 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
     auto fits2 = [&](int w, int h) {
         return fg::splat_hbm_lds(w, h, (int)d->n_samples, q) + queue <= (size_t)ctx->max_lds / 2;
     };
-    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 && ctx->cfg.pix_tile_w <= 128 &&
-        ctx->cfg.pix_tile_h <= 256 && fits(ctx->cfg.pix_tile_w, ctx->cfg.pix_tile_h)) {
-        *tw = ctx->cfg.pix_tile_w;
-        *th = ctx->cfg.pix_tile_h;
+    if (cfg.pix_tile_w > 0 && cfg.pix_tile_h > 0 && cfg.pix_tile_w <= 128 &&
+        cfg.pix_tile_h <= 256 && fits(cfg.pix_tile_w, cfg.pix_tile_h)) {
+        *tw = cfg.pix_tile_w;
+        *th = cfg.pix_tile_h;
     } else {
         // The largest ladder tile of 128 to max(128, 512 zoom^2) output
         // pixels (about 512 input pixels: larger tiles leave too few
@@ -340,21 +381,22 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
             }
         }
         if (*tw == 0) return false;
-        // A plane alone on the device: smaller tiles until it has at least
-        // four per CU, so the heaviest tile does not set the frame time
-        // (480x272, N = 64: 255 tiles of 32x16 take 2.6 ms).
-        if (min_tiles > 0) {
-            size_t i = 0;
-            while (i < sizeof ladder / sizeof ladder[0] && (ladder[i][0] != *tw || ladder[i][1] != *th)) ++i;
-            auto tiles = [&](int w, int h) {
-                return (long long)((d->out_w + w - 1) / w) * (long long)((d->out_h + h - 1) / h);
-            };
-            while (i + 1 < sizeof ladder / sizeof ladder[0] && tiles(*tw, *th) < min_tiles &&
-                   ladder[i + 1][0] * ladder[i + 1][1] >= 32 && fits(ladder[i + 1][0], ladder[i + 1][1])) {
-                ++i;
-                *tw = ladder[i][0];
-                *th = ladder[i][1];
-            }
+    }
+    // A plane alone on the device: smaller tiles until it has at least four
+    // per CU, so the heaviest tile does not set the frame time (480x272,
+    // N = 64: 255 tiles of 32x16 take 2.6 ms).  A tuned tile steps down too
+    // (its bucket holds planes of every size); an explicit one does not.
+    if (min_tiles > 0 && !ctx->cfg_explicit) {
+        size_t i = 0;
+        while (i < sizeof ladder / sizeof ladder[0] && (ladder[i][0] != *tw || ladder[i][1] != *th)) ++i;
+        auto tiles = [&](int w, int h) {
+            return (long long)((d->out_w + w - 1) / w) * (long long)((d->out_h + h - 1) / h);
+        };
+        while (i + 1 < sizeof ladder / sizeof ladder[0] && tiles(*tw, *th) < min_tiles &&
+               ladder[i + 1][0] * ladder[i + 1][1] >= 32 && fits(ladder[i + 1][0], ladder[i + 1][1])) {
+            ++i;
+            *tw = ladder[i][0];
+            *th = ladder[i][1];
         }
     }
     *lds = (int)fg::splat_hbm_lds(*tw, *th, (int)d->n_samples, q);
@@ -363,15 +405,16 @@ bool hbm_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds,
 
 void pixel_config(fg_ctx *ctx, const fg_plane_desc *d, int variant, int *tw, int *th, int *lds)
 {
+    const fg_launch_config &cfg = launch_cfg(ctx, d, 0);
     const int def = variant == 1 ? 64 * 1024 : 72 * 1024;
-    const int budget = ctx->cfg.pix_lds_bytes > 0 ? ctx->cfg.pix_lds_bytes : def;
+    const int budget = cfg.pix_lds_bytes > 0 ? cfg.pix_lds_bytes : def;
     *lds = std::min(budget, ctx->max_lds - 6 * 1024);
     const int max_px = variant == 1 ? fg::kMaxTilePixels : 256 * 256;
-    if (ctx->cfg.pix_tile_w > 0 && ctx->cfg.pix_tile_h > 0 &&
-        ctx->cfg.pix_tile_w * ctx->cfg.pix_tile_h <= max_px && ctx->cfg.pix_tile_w <= 256 &&
-        ctx->cfg.pix_tile_h <= 256) {
-        *tw = ctx->cfg.pix_tile_w;
-        *th = ctx->cfg.pix_tile_h;
+    if (cfg.pix_tile_w > 0 && cfg.pix_tile_h > 0 &&
+        cfg.pix_tile_w * cfg.pix_tile_h <= max_px && cfg.pix_tile_w <= 256 &&
+        cfg.pix_tile_h <= 256) {
+        *tw = cfg.pix_tile_w;
+        *th = cfg.pix_tile_h;
     } else if (variant == 1) {
         choose_gather_tile(d, *lds, tw, th);
     } else {
@@ -394,14 +437,15 @@ bool division_fast_ok(fg_ctx *ctx, float delta, float rdelta)
 
 void grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, int *lds, int *gpt)
 {
+    const fg_launch_config &cfg = launch_cfg(ctx, d, 1);
     const int words = (int)((d->n_samples + 31) / 32);
-    *gpt = ctx->cfg.grain_grains_per_thread > 0 ? ctx->cfg.grain_grains_per_thread : 4;
-    const int budget = ctx->cfg.grain_lds_bytes > 0 ? ctx->cfg.grain_lds_bytes : 64 * 1024;
+    *gpt = cfg.grain_grains_per_thread > 0 ? cfg.grain_grains_per_thread : 4;
+    const int budget = cfg.grain_lds_bytes > 0 ? cfg.grain_lds_bytes : 64 * 1024;
     const int cap = std::min(budget, ctx->max_lds - 2 * 1024);
     const int slots = fg::kBlock * (*gpt) * 16;
-    if (ctx->cfg.grain_tile_w > 0 && ctx->cfg.grain_tile_h > 0) {
-        *tw = ctx->cfg.grain_tile_w;
-        *th = ctx->cfg.grain_tile_h;
+    if (cfg.grain_tile_w > 0 && cfg.grain_tile_h > 0) {
+        *tw = cfg.grain_tile_w;
+        *th = cfg.grain_tile_h;
     } else {
         static const int ladder[][2] = {{128, 64}, {64, 64}, {64, 32}, {32, 32}, {32, 16},
                                         {16, 16}, {16, 8},   {8, 8},   {8, 4},   {4, 4},
@@ -657,7 +701,7 @@ int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::Pi
 // Per-plane generation into HBM, then the splat-from-HBM kernel.  With an
 // arena, every scratch buffer comes from it and nothing is allocated.
 int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
-                  hipStream_t s, ScratchSource src = ScratchSource())
+                  int block, hipStream_t s, ScratchSource src = ScratchSource())
 {
     const char *what = "Pixel renderer";
     int rc = FG_OK;
@@ -773,7 +817,6 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     a.tile_list = order.order;
     mark(ctx, 1, s);
     mark(ctx, 2, s);
-    const int block = splat_hbm_block(ctx);
     if (e == hipSuccess) e = fg::launch_splat_hbm(a, g, st, start, grains, block, s);
     mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
@@ -864,7 +907,7 @@ int do_pixelwise_budget(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lamb
                      bits + (size_t)d->n_samples * 8 + min_grains * 32 <= (size_t)lds) ? 1 : 0;
     if (splat && bits + (a.offs_in_lds ? (size_t)d->n_samples * 8 : 0) + min_grains * 32 > (size_t)lds)
         return set_error(ctx, FG_EINVAL, "sample count too large for one pixel-wise tile");
-    if (variant == 3) return run_pixel_hbm(ctx, a, geom, range, (int)d->dist_kind, s);
+    if (variant == 3) return run_pixel_hbm(ctx, a, geom, range, (int)d->dist_kind, splat_hbm_block(ctx, d), s);
     mark(ctx, 0, s);
     e = fg::launch_lambda_check(d_lambda, (long long)d->in_w * d->in_h, a.status, s);
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
@@ -886,11 +929,14 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
 // sample) pairs, 2 runs of the sorted sample order (automatic; needs N <= 2048).
 int grain_variant(fg_ctx *ctx, const fg_plane_desc *d)
 {
-    if (ctx->cfg.grain_variant == 1 || d->n_samples > kHbmMaxSamples) return 1;
+    if (launch_cfg(ctx, d, 1).grain_variant == 1 || d->n_samples > kHbmMaxSamples) return 1;
     return 2;
 }
 
-int grain_block(const fg_ctx *ctx) { return ctx->cfg.grain_block == 256 ? 256 : 512; }
+int grain_block(const fg_ctx *ctx, const fg_plane_desc *d)
+{
+    return launch_cfg(ctx, d, 1).grain_block == 256 ? 256 : 512;
+}
 
 // Sorted grain-wise kernel: the configured tile, else the largest tile of the
 // ladder whose bitset takes at most 32 KB; queue entries address at most
@@ -899,14 +945,15 @@ bool sorted_grain_config(fg_ctx *ctx, const fg_plane_desc *d, int *tw, int *th, 
 {
     static const int ladder[][2] = {{128, 64}, {64, 64}, {64, 32}, {32, 32}, {32, 16}, {16, 16},
                                     {16, 8},   {8, 8},   {8, 4},   {4, 4},   {2, 2},   {1, 1}};
+    const fg_launch_config &cfg = launch_cfg(ctx, d, 1);
     const int q = sample_buckets(d->n_samples);
     const size_t words = (d->n_samples + 31) / 32;
-    const size_t limit = (size_t)ctx->max_lds - fg::grainwise_sorted_static_lds(grain_block(ctx));
+    const size_t limit = (size_t)ctx->max_lds - fg::grainwise_sorted_static_lds(grain_block(ctx, d));
     auto fits = [&](int w, int h) { return fg::grainwise_sorted_lds(w, h, (int)d->n_samples, q) <= limit; };
-    if (ctx->cfg.grain_tile_w > 0 && ctx->cfg.grain_tile_h > 0 && ctx->cfg.grain_tile_w <= 128 &&
-        ctx->cfg.grain_tile_h <= 256 && fits(ctx->cfg.grain_tile_w, ctx->cfg.grain_tile_h)) {
-        *tw = ctx->cfg.grain_tile_w;
-        *th = ctx->cfg.grain_tile_h;
+    if (cfg.grain_tile_w > 0 && cfg.grain_tile_h > 0 && cfg.grain_tile_w <= 128 &&
+        cfg.grain_tile_h <= 256 && fits(cfg.grain_tile_w, cfg.grain_tile_h)) {
+        *tw = cfg.grain_tile_w;
+        *th = cfg.grain_tile_h;
     } else {
         size_t i = 0, n = sizeof ladder / sizeof ladder[0];
         while (i < n && !((size_t)ladder[i][0] * ladder[i][1] * words * 4 <= 32768 && fits(ladder[i][0], ladder[i][1])))
@@ -1013,7 +1060,7 @@ int do_grainwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
         a.tile_list = order.order;
         mark(ctx, 1, s);
         mark(ctx, 2, s);
-        if (e == hipSuccess) e = fg::launch_grainwise_sorted(a, st, (int)d->dist_kind, grain_block(ctx), s);
+        if (e == hipSuccess) e = fg::launch_grainwise_sorted(a, st, (int)d->dist_kind, grain_block(ctx, d), s);
         mark(ctx, 3, s);
         if (e != hipSuccess) return hip_error(ctx, e, what);
         return scratch_release(ctx, s, what);
@@ -1355,13 +1402,15 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     b->list_mem.resize(n);
     std::vector<int> lds_p(n);
     long long items = 0;
+    // one workgroup size for the whole batch: the explicit one, else 512
+    b->block = ctx->cfg_explicit && ctx->cfg.pix_block == 256 ? 256 : 512;
     for (int p = 0; p < n; ++p) {
         const fg_plane_desc *d = &descs[p];
         fg::BatchPlane &bp = b->planes[p];
         if (!plane_geometry(d, r[p], &bp.geo))
             return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": cell grid exceeds 65535 cells per side");
         int tw, th;
-        if (!hbm_config(ctx, d, &tw, &th, &lds_p[p], 0))
+        if (!hbm_config(ctx, d, &tw, &th, &lds_p[p], 0, b->block))
             return set_error(ctx, FG_EINVAL, "batch plane " + std::to_string(p) + ": no tile fits the LDS");
         if ((rc = pixel_args(ctx, d, lams[p], offs[p], outs[p], tw, th, lds_p[p], b->ranges + p, b->status,
                              &bp.a)) != FG_OK)
@@ -1396,7 +1445,6 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
         return set_error(ctx, FG_EINVAL, "batch too large");
     b->n_items = (int)items;
     b->lds = *std::max_element(lds_p.begin(), lds_p.end());
-    b->block = splat_hbm_block(ctx);
     if ((size_t)b->lds + fg::splat_hbm_static_lds(b->block) > (size_t)ctx->max_lds)
         return set_error(ctx, FG_EINVAL, "batch planes need more LDS than a workgroup has");
     const long long nb = b->total_bins;
@@ -1466,6 +1514,7 @@ int fg_ctx_create(int device_ordinal, fg_ctx **out)
     if (device_ordinal < 0 || device_ordinal >= count) return FG_ENODEV;
     fg_ctx *ctx = new fg_ctx();
     ctx->device = device_ordinal;
+    ctx->table.assign(kBuiltinLaunchTable, kBuiltinLaunchTable + kBuiltinLaunchCount);
     hipError_t e = hipSetDevice(device_ordinal);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device_ordinal);
@@ -1583,7 +1632,7 @@ int fg_render_pixelwise_device_noalloc(fg_ctx *ctx, const fg_plane_desc *desc, c
     ScratchSource src;
     src.arena = ctx->noalloc_scratch;
     src.cap = kNoAllocScratchBytes;
-    return run_pixel_hbm(ctx, a, geom, range, (int)desc->dist_kind, s, src);
+    return run_pixel_hbm(ctx, a, geom, range, (int)desc->dist_kind, splat_hbm_block(ctx, desc), s, src);
 }
 
 int fg_render_pixelwise_adaptive_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
@@ -1652,10 +1701,52 @@ int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg)
             cfg->grain_variant > 2 || (cfg->grain_block != 0 && cfg->grain_block != 256 && cfg->grain_block != 512))
             return set_error(ctx, FG_EINVAL, "invalid launch configuration");
         ctx->cfg = *cfg;
+        const fg_launch_config zero{};
+        ctx->cfg_explicit = std::memcmp(cfg, &zero, sizeof zero) != 0;
     } else {
         ctx->cfg = fg_launch_config{};
+        ctx->cfg_explicit = false;
     }
     return FG_OK;
+}
+
+int fg_launch_bucket(const fg_plane_desc *desc, int algo, fg_launch_key *out)
+{
+    if (!desc || !out || (algo != 0 && algo != 1)) return FG_EINVAL;
+    *out = launch_key(desc, algo);
+    return FG_OK;
+}
+
+int fg_set_launch_table(fg_ctx *ctx, const fg_launch_entry *entries, int n)
+{
+    if (!ctx) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n < 0) {
+        ctx->table.assign(kBuiltinLaunchTable, kBuiltinLaunchTable + kBuiltinLaunchCount);
+        return FG_OK;
+    }
+    if (n > 0 && !entries) return set_error(ctx, FG_EINVAL, "null launch table");
+    for (int i = 0; i < n; ++i) {
+        const fg_launch_config &c = entries[i].cfg;
+        if (entries[i].key.algo > 1 || entries[i].key.dist_kind > 1 || c.pix_tile_w < 0 || c.pix_tile_h < 0 ||
+            c.pix_tile_w > 256 || c.pix_tile_h > 256 || c.grain_tile_w < 0 || c.grain_tile_h < 0 ||
+            c.pix_variant < 0 || c.pix_variant > 3 || (c.pix_block != 0 && c.pix_block != 256 && c.pix_block != 512) ||
+            c.grain_variant < 0 || c.grain_variant > 2 ||
+            (c.grain_block != 0 && c.grain_block != 256 && c.grain_block != 512) || c.pix_lds_bytes < 0 ||
+            c.grain_lds_bytes < 0 || c.grain_grains_per_thread < 0 || c.grain_grains_per_thread > 64)
+            return set_error(ctx, FG_EINVAL, "invalid launch table entry " + std::to_string(i));
+    }
+    ctx->table.assign(entries, entries + n);
+    return FG_OK;
+}
+
+int fg_get_launch_table(fg_ctx *ctx, fg_launch_entry *out, int cap)
+{
+    if (!ctx) return FG_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int n = (int)ctx->table.size();
+    for (int i = 0; i < n && i < cap && out; ++i) out[i] = ctx->table[i];
+    return n;
 }
 
 int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out)
@@ -1667,7 +1758,7 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
     *out = fg_launch_config{};
     if (algo == 1) {
         out->grain_variant = grain_variant(ctx, desc);
-        out->grain_block = grain_block(ctx);
+        out->grain_block = grain_block(ctx, desc);
         if (out->grain_variant == 2 &&
             !sorted_grain_config(ctx, desc, &out->grain_tile_w, &out->grain_tile_h, &out->grain_lds_bytes))
             out->grain_variant = 1;
@@ -1676,7 +1767,7 @@ int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_la
                          &out->grain_grains_per_thread);
     } else {
         out->pix_variant = pixel_variant(ctx, desc);
-        out->pix_block = splat_hbm_block(ctx);
+        out->pix_block = splat_hbm_block(ctx, desc);
         if (out->pix_variant == 3 &&
             !hbm_config(ctx, desc, &out->pix_tile_w, &out->pix_tile_h, &out->pix_lds_bytes, 4LL * ctx->num_cus))
             out->pix_variant = 2;

@@ -191,8 +191,41 @@ typedef struct {
                           offset fraction; identical results */
     int grain_block; /* threads per workgroup of the sorted grain-wise kernel: 512 (default) or 256 */
 } fg_launch_config;
+/* An explicit launch configuration (any field nonzero) replaces the tuned
+ * table below for the following renders; NULL (or all zeros) returns to it. */
 int fg_set_launch_config(fg_ctx *ctx, const fg_launch_config *cfg);
+/* The launch shape the next render of desc with algo (0 pixel-wise, 1
+ * grain-wise) takes: the explicit configuration, else the tuned table's entry
+ * for the plane's bucket, else the built-in choice. */
 int fg_get_launch_config(fg_ctx *ctx, const fg_plane_desc *desc, int algo, fg_launch_config *out);
+
+/* Tuned launch table.  No reference counterpart: the reference dispatches
+ * every plane with one fixed workgroup shape (src/wgpu/mod.rs:439-441).  The
+ * autotuner (film_grain_amd/autotune.py) times candidate shapes per workload
+ * bucket and records the fastest; every candidate gives the same result, so
+ * the table changes only speed.  A context starts with the table the library
+ * was built with (film_grain_amd/tuning/launch_table.json).  Buckets: the
+ * integrator, the radius law, zoom in quarter octaves, the radius ratio (rm /
+ * cell size for pixel-wise, rm * zoom for grain-wise) in half octaves, and N
+ * rounded up to a power of two. */
+typedef struct {
+    uint32_t algo;      /* 0 pixel-wise, 1 grain-wise */
+    uint32_t dist_kind; /* 0 constant, 1 lognormal */
+    int32_t zoom_q;     /* round(4 log2 zoom) */
+    int32_t ratio_h;    /* round(2 log2 ratio); -1000 for ratio <= 0 */
+    uint32_t n_bucket;  /* smallest power of two >= N */
+} fg_launch_key;
+typedef struct {
+    fg_launch_key key;
+    fg_launch_config cfg;
+} fg_launch_entry;
+/* The bucket of desc for algo (no context or device needed). */
+int fg_launch_bucket(const fg_plane_desc *desc, int algo, fg_launch_key *out);
+/* Replaces the context's table with n entries (copied); n = 0 empties it,
+ * n < 0 restores the built-in table.  fg_get_launch_table copies up to cap
+ * entries and returns the table's size. */
+int fg_set_launch_table(fg_ctx *ctx, const fg_launch_entry *entries, int n);
+int fg_get_launch_table(fg_ctx *ctx, fg_launch_entry *out, int cap);
 
 /* Kernel timing of the following renders on this context: with enable set,
  * HIP events are recorded on the launch stream around the grain-generation
