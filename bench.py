@@ -286,37 +286,72 @@ def frame_summary(reps: int = 3):
 
 def adaptive_summary(steps: int = 3, warmup: int = 1):
     """The learned sample budget on one 1920x1080 RGB frame (lognormal
-    0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024): ms/frame of the adaptive
-    render (features, error model, budgets and renders, all timed) at the
-    RMSE that fixed N = 1024 reaches against the N = 4096 render, the fixed-N
-    ms/frame at the adaptive frame's RMSE, the allocators at equal sample
-    totals, and the error model's MFMA throughput."""
+    0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024).  The headline allocator
+    gives each tile of the adaptive renderer one sample level, chosen from
+    the error model's per-pixel spreads and the tile's measured per-sample
+    cost (n ~ s / sqrt(c)); it is timed whole (features, error model,
+    levels and renders) at the RMSE that fixed N = 256, 512 and 1024 reach
+    against the N = 4096 render, beside the fixed-N ms/frame at the adaptive
+    frame's own RMSE.  Also: the per-pixel allocator, the allocators at
+    equal sample totals, and the error model's MFMA throughput."""
     import torch
     from film_grain_amd import budget, capi
     ctx = capi.Context(0)
     model = budget.ErrorModel.load()
-    planes = [budget.upload(sp) for sp in budget.image_planes(50_000)]
+    specs = budget.image_planes(50_000)
+    cal = budget.calibrate_cost(ctx, specs[0])
+    log(f"[bench] cost calibration: kappa {cal['kappa']:.4g} (ms per sample {cal['ms_per_sample']}, "
+        f"lambda {cal['lambdas']})")
+    planes = [budget.upload(sp) for sp in specs]
     refs = [budget.render_reference(ctx, dp) for dp in planes]
-    curve = budget.fixed_curve(ctx, planes, refs, ns=(256, 512, 1024, 2048), reps=steps)
+    curve = budget.fixed_curve(ctx, planes, refs, ns=(128, 256, 512, 1024, 2048), reps=steps)
     bufs = budget.frame_buffers(planes)
-    target = next(e for n, ms, e in curve if n == budget.C5["n_fixed"])
-    for _ in range(warmup):
-        budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2, "learned")
-    torch.cuda.synchronize()
-    ts = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        outs, c = budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2, "learned")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
         torch.cuda.synchronize()
-        ts.append((time.perf_counter() - t0) * 1e3)
-    ms = float(np.median(ts))
-    rmse = budget._frame_rmse(outs, refs)
-    mean_n = float(np.mean([float((b.to(torch.int32) & 0xFFFF).float().mean()) for b in bufs.budget]))
-    fixed_ms = budget.ms_at_rmse(curve, rmse)
+        ts = []
+        res = None
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            res = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts)), res
+
+    def mean_budget():
+        return float(np.mean([float((b.to(torch.int32) & 0xFFFF).float().mean()) for b in bufs.budget]))
+
+    targets = []
+    for tn in (256, 512, 1024):
+        target = next(e for n, ms_, e in curve if n == tn)
+        fixed_tn = next(ms_ for n, ms_, e in curve if n == tn)
+        row = dict(target_fixed_n=tn, target_rmse_255=round(target * 255, 4), fixed_ms=round(fixed_tn, 3))
+        for kind in ("learned", "oracle"):
+            ms_, (outs, info) = timed(lambda: budget.render_tiled_frame(ctx, model, planes, bufs, target ** 2,
+                                                                       cal["kappa"], kind, refs=refs))
+            rmse = budget._frame_rmse(outs, refs)
+            fx = budget.ms_at_rmse(curve, rmse)
+            row[kind] = dict(ms=round(ms_, 3), rmse_255=round(rmse * 255, 4), mean_budget=round(mean_budget(), 1),
+                             fixed_ms_at_rmse=None if fx is None else round(fx, 3),
+                             fixed_over_adaptive=None if fx is None else round(fx / ms_, 4))
+            log(f"[bench] adaptive ({kind}, tile levels) at fixed N={tn}'s rmse {target * 255:.3f}/255: "
+                f"{ms_:.2f} ms/frame rmse {rmse * 255:.3f}/255 mean n {mean_budget():.1f}; fixed at that rmse "
+                f"{fx if fx is None else round(fx, 2)} ms (fixed N={tn}: {fixed_tn:.2f} ms)")
+        targets.append(row)
+    head = targets[-1]["learned"]
+    ms, rmse, mean_n, fixed_ms = head["ms"], head["rmse_255"] / 255, head["mean_budget"], head["fixed_ms_at_rmse"]
+    # the per-pixel allocator (budget = ceil(c s) per pixel) at the N = 1024 target
+    target = next(e for n, ms_, e in curve if n == budget.C5["n_fixed"])
+    pp_ms, (pp_outs, _) = timed(lambda: budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2,
+                                                                     "learned"))
+    pp_rmse = budget._frame_rmse(pp_outs, refs)
+    pp_fixed = budget.ms_at_rmse(curve, pp_rmse)
+    log(f"[bench] adaptive (learned, per-pixel budgets) {pp_ms:.2f} ms/frame rmse {pp_rmse * 255:.3f}/255; fixed at "
+        f"that rmse {pp_fixed} ms")
     fixed_1024 = next(ms_ for n, ms_, e in curve if n == budget.C5["n_fixed"])
     ratio = budget.s_ratio(refs)
-    log(f"[bench] adaptive {ms:.2f} ms/frame rmse {rmse * 255:.3f}/255 mean n {mean_n:.1f}; fixed at that rmse "
-        f"{fixed_ms} ms; fixed N=1024 {fixed_1024:.2f} ms; E[s^2]/E[s]^2 {ratio:.4f}")
     eq = budget.compare_at_equal_budget(ctx, model, budget.image_planes(50_000), mean_budgets=(256,))
     # the error model's kernels on the frame's features
     X = torch.cat(bufs.X)
@@ -347,11 +382,18 @@ def adaptive_summary(steps: int = 3, warmup: int = 1):
     return {
         "metric": "ms/frame adaptive vs fixed-N at equal RMSE",
         "workload": "1920x1080 RGB, lognormal 0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024, seed 50000; "
-                    "RMSE against the N=4096 render; target = the RMSE of fixed N=1024",
+                    "RMSE against the N=4096 render; targets = the RMSE of fixed N=256, 512, 1024; "
+                    "headline at the N=1024 target",
+        "allocator": "tile levels from the error model's spreads and the measured per-sample cost (n ~ s/sqrt(c))",
         "adaptive_ms_per_frame": round(ms, 3), "rmse_255": round(rmse * 255, 4), "mean_budget": round(mean_n, 1),
-        "fixed_ms_at_equal_rmse": None if fixed_ms is None else round(fixed_ms, 3),
+        "fixed_ms_at_equal_rmse": fixed_ms,
         "fixed_over_adaptive": None if fixed_ms is None else round(fixed_ms / ms, 4),
         "fixed_n1024_ms_per_frame": round(fixed_1024, 3),
+        "targets": targets,
+        "cost_model": {"kappa": round(cal["kappa"], 6), "ms_per_sample": [round(v, 6) for v in cal["ms_per_sample"]],
+                       "lambdas": [round(v, 3) for v in cal["lambdas"]]},
+        "per_pixel_allocator": {"ms": round(pp_ms, 3), "rmse_255": round(pp_rmse * 255, 4),
+                                "fixed_ms_at_rmse": None if pp_fixed is None else round(pp_fixed, 3)},
         "fixed_curve": [dict(n=n, ms=round(t, 3), rmse_255=round(e_ * 255, 4)) for n, t, e_ in curve],
         "allocation_bound_E_s2_over_E_s_sq": round(ratio, 4),
         "equal_samples_256_rmse_255": {r["kind"]: round(r["rmse"] * 255, 4) for r in eq},

@@ -419,6 +419,180 @@ def _render_learned_frame(ctx, model: ErrorModel, planes: list[DevicePlane], buf
     return bufs.out, sc[:1]
 
 
+# ------------------------------------------------- tile-level allocation ----
+#
+# The adaptive renderer (fg_render_pixelwise_adaptive_device) renders each
+# tile at one sample level: the first of n_min, 2 n_min, ..., n_max that
+# covers the largest budget among the tile's pixels, and its work is that
+# level times the tile's per-sample cost.  A pixel budget below its tile's
+# level saves nothing, so the allocation that matches the kernel gives every
+# pixel of tile t the level L_t itself, and picks the levels that minimise the
+# frame's work  sum_t c_t L_t  subject to its predicted error
+#     sum_t S_t (1 / L_t - 1 / N_REF) <= target * pixels,
+# with S_t the sum of s^2 over the tile and c_t its per-sample cost.  The
+# Lagrange condition gives L_t = sqrt(mu S_t / c_t), the n ~ s / sqrt(c) rule
+# at the kernel's granularity; with discrete levels each tile takes the level
+# minimising c_t L + mu S_t / L, and mu is the smallest meeting the target.
+# The per-sample cost of a pixel is a + b lambda (the splat's work per sample
+# has a fixed part and a part that grows with the grains its window holds),
+# c_t = sum over the tile of 1 + kappa lambda, kappa = b / a measured by
+# calibrate_cost.
+
+
+def tile_levels_list(n_min: int, n_max: int) -> list[int]:
+    """The adaptive renderer's sample levels: n_min doubling below n_max,
+    then n_max."""
+    out, level = [], n_min
+    while level < n_max:
+        out.append(level)
+        level *= 2
+    return out + [n_max]
+
+
+def out_lambda(dp: DevicePlane):
+    """lambda at each output pixel's input position (nearest pixel)."""
+    import torch
+    z = dp.spec.zoom
+    dev = dp.lam.device
+    ys = torch.clamp(torch.floor((torch.arange(dp.out_h, device=dev) + 0.5) / z).long(), 0, dp.lam.shape[0] - 1)
+    xs = torch.clamp(torch.floor((torch.arange(dp.out_w, device=dev) + 0.5) / z).long(), 0, dp.lam.shape[1] - 1)
+    return dp.lam[ys][:, xs]
+
+
+def tile_sums(x, tw: int, th: int):
+    """Sums of an (h, w) tensor over tw x th tiles (edge tiles partial)."""
+    import torch
+    h, w = x.shape
+    ny, nx = (h + th - 1) // th, (w + tw - 1) // tw
+    p = torch.zeros((ny * th, nx * tw), dtype=x.dtype, device=x.device)
+    p[:h, :w] = x
+    return p.view(ny, th, nx, tw).sum(dim=(1, 3))
+
+
+def calibrate_cost(ctx, spec: PlaneSpec | None = None, us=(0.3, 0.9), ns=(64, 512), reps: int = 3) -> dict:
+    """kappa = b / a of the per-sample cost a + b lambda of a pixel, from the
+    fixed-N render time of two planes of uniform intensity (one pixel at 1 so
+    that normalisation keeps the level) at two sample counts: the slope in N
+    removes the per-plane generation, and the two lambdas separate a and b."""
+    spec = spec or image_planes(1)[0]
+    slopes, lams = [], []
+    for u in us:
+        plane = np.full_like(spec.u, u)
+        plane[0, 0] = 1.0
+        dp = upload(PlaneSpec(plane, spec.channel, spec.radius, spec.sd, spec.sigma, spec.zoom, spec.seed))
+        out = None
+        ts = []
+        for n in ns:
+            def run():
+                render_fixed(ctx, dp, n, out=out)
+            out = render_fixed(ctx, dp, n)
+            ts.append(_time_ms(run, reps))
+        slopes.append((ts[1] - ts[0]) / (ns[1] - ns[0]))
+        lams.append(float(dp.lam[dp.lam.shape[0] // 2, dp.lam.shape[1] // 2]))
+        del dp
+    b = (slopes[1] - slopes[0]) / (lams[1] - lams[0])
+    a = slopes[0] - b * lams[0]
+    kappa = b / a if a > 0 and b > 0 else 0.0
+    return dict(kappa=kappa, ms_per_sample=slopes, lambdas=lams)
+
+
+def choose_tile_levels(S, C, pixels: int, target_mse: float, levels, n_ref: int = N_REF, probes: int = 64,
+                       rounds: int = 3):
+    """Per tile the level minimising C L + mu S / L for the smallest mu whose
+    predicted frame error sum S (1/L - 1/n_ref) / pixels is <= target_mse
+    (rounds of `probes` geometric candidates).  S, C: 1-D tile tensors.
+    Returns (levels per tile, mu); a target below what the top level reaches
+    everywhere gives the top level everywhere."""
+    import torch
+    L = torch.tensor(levels, dtype=torch.float64, device=S.device)
+    inv = 1.0 / L
+    S, C = S.double(), C.double()
+
+    def pick(mu):  # mu: (k,) -> level index (k, T), frame mse (k,)
+        j = torch.argmin(C[None, :, None] * L[None, None, :] + mu[:, None, None] * S[None, :, None] * inv, dim=2)
+        mse = (S[None, :] * (inv[j] - 1.0 / n_ref)).sum(dim=1) / pixels
+        return j, mse
+
+    pos = S > 0
+    if not bool(pos.any()):
+        return torch.full_like(S, levels[0]), 0.0
+    ratio = C[pos] / S[pos]
+    lo = float(ratio.min()) * levels[0] ** 2 * 0.25
+    hi = float(ratio.max()) * levels[-1] ** 2 * 4.0
+    for _ in range(rounds):
+        mus = torch.exp(torch.linspace(math.log(lo), math.log(hi), probes, device=S.device, dtype=torch.float64))
+        _, mse = pick(mus)
+        ok = (mse <= target_mse).nonzero()
+        if ok.numel() == 0:
+            lo = hi = float(mus[-1])
+            break
+        k = int(ok[0])
+        if k == 0:
+            lo = hi = float(mus[0])
+            break
+        lo, hi = float(mus[k - 1]), float(mus[k])
+    j, _ = pick(torch.tensor([hi], dtype=torch.float64, device=S.device))
+    return L[j[0]], hi
+
+
+def render_tiled_frame(ctx, model: ErrorModel | None, planes: list[DevicePlane], bufs: FrameBuffers,
+                       target_mse: float, kappa: float, kind: str = "learned", n_min: int = C5["n_min"],
+                       n_max: int = C5["n_max"], refs=None):
+    """The frame with tile-level budgets (the allocation above): features,
+    per-pixel spreads, per-tile S and c, the levels, budgets equal to the
+    tile's level, and the adaptive renders.  The launch is sized for the
+    largest level the frame uses (smaller per-sample bitsets allow larger
+    tiles): the levels are chosen on the tiles of an n_max launch, and again
+    on the tiles of a launch for the top level they use, when those differ.
+    Returns (outputs, info)."""
+    import torch
+    s_list = []
+    for i, dp in enumerate(planes):
+        features(ctx, dp, out=bufs.X[i])
+        if kind == "learned":
+            model.predict(ctx, bufs.X[i], out=bufs.pred[i])
+            s = torch.exp(bufs.pred[i]) * SIGMA64_TO_S
+        else:
+            s = allocator_s(kind, model, ctx, bufs.X[i], None if refs is None else refs[i])
+        s_list.append((s.reshape(dp.out_h, dp.out_w).float()) ** 2)
+    cost_px = [1.0 + kappa * out_lambda(dp) for dp in planes]
+    pixels = sum(dp.out_w * dp.out_h for dp in planes)
+
+    def allocate(n_top):
+        levels = tile_levels_list(n_min, n_top)
+        S_all, C_all, shapes = [], [], []
+        for i, dp in enumerate(planes):
+            cfg = ctx.launch_config(_desc_n(dp.desc, n_top), "pixel")
+            tw, th = cfg["pix_tile_w"], cfg["pix_tile_h"]
+            S = tile_sums(s_list[i], tw, th)
+            S_all.append(S.reshape(-1))
+            C_all.append(tile_sums(cost_px[i], tw, th).reshape(-1))
+            shapes.append((S.shape, tw, th))
+        lv, mu = choose_tile_levels(torch.cat(S_all), torch.cat(C_all), pixels, target_mse, levels)
+        return lv, mu, shapes, torch.cat(C_all)
+
+    n_top = n_max
+    lv, mu, shapes, C = allocate(n_top)
+    top = int(lv.max())
+    if top < n_max:
+        shapes_top = [ctx.launch_config(_desc_n(dp.desc, top), "pixel") for dp in planes]
+        if any((c["pix_tile_w"], c["pix_tile_h"]) != (sh[1], sh[2]) for c, sh in zip(shapes_top, shapes)):
+            n_top = top
+            lv, mu, shapes, C = allocate(n_top)
+        else:
+            n_top = top
+    at = 0
+    for i, dp in enumerate(planes):
+        (ny, nx), tw, th = shapes[i]
+        t = lv[at:at + ny * nx].reshape(ny, nx).to(torch.int16)
+        at += ny * nx
+        full = t.repeat_interleave(th, dim=0).repeat_interleave(tw, dim=1)[:dp.out_h, :dp.out_w]
+        bufs.budget[i].copy_(full.reshape(-1))
+    for i, dp in enumerate(planes):
+        ctx.render_adaptive_device(_desc_n(dp.desc, n_top), dp.lam, dp.offs[:n_top], bufs.budget[i], bufs.out[i])
+    return bufs.out, dict(mu=mu, work=float((C.double() * lv).sum()), n_top=n_top)
+
+
 def _frame_rmse(outs, refs) -> float:
     import torch
     se = sum(float(torch.sum((o - r).double() ** 2)) for o, r in zip(outs, refs))

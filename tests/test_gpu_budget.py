@@ -120,3 +120,48 @@ def test_device_scale_search_matches_choose_scale(hip_ctx):
             b_dev = hip_ctx.mlp_budget_scale(pred, None, n, sc[1:], 16, 1024)
             b_host = hip_ctx.mlp_budget(pred, None, n, float(sc[1]), 16, 1024)
             assert torch.equal(b_dev, b_host)
+
+
+def test_tile_level_budgets_render_each_tile_at_its_level(hip_ctx):
+    """Tile-level allocation: every pixel of a tile gets the tile's level (one
+    of 16, 32, ..., 1024), each tile's pixels equal the fixed-N render at
+    that level, cheap low-spread tiles get fewer samples, and the frame
+    meets its predicted error to within the model's accuracy."""
+    import torch
+    model = budget.ErrorModel.load()
+    planes = [budget.upload(s) for s in budget.image_planes(50_003, 480, 272)]
+    refs = [budget.render_reference(hip_ctx, dp) for dp in planes]
+    bufs = budget.frame_buffers(planes)
+    curve = budget.fixed_curve(hip_ctx, planes, refs, ns=(256,), reps=1)
+    target = curve[0][2]
+    outs, info = budget.render_tiled_frame(hip_ctx, model, planes, bufs, target ** 2, kappa=0.05)
+    torch.cuda.synchronize()
+    levels = budget.tile_levels_list(16, 1024)
+    rng = np.random.default_rng(4)
+    for i, dp in enumerate(planes):
+        cfg = hip_ctx.launch_config(budget._desc_n(dp.desc, info["n_top"]), "pixel")
+        tw, th = cfg["pix_tile_w"], cfg["pix_tile_h"]
+        b = (bufs.budget[i].to(torch.int32) & 0xFFFF).reshape(dp.out_h, dp.out_w).cpu().numpy()
+        assert set(np.unique(b)) <= set(levels)
+        seen = set()
+        for ty in range(0, dp.out_h, th):
+            for tx in range(0, dp.out_w, tw):
+                t = b[ty:ty + th, tx:tx + tw]
+                assert (t == t[0, 0]).all(), (i, tx, ty)
+        # a few tiles against the fixed render at their level
+        out = outs[i].cpu().numpy()
+        for _ in range(6):
+            ty = int(rng.integers(0, (dp.out_h + th - 1) // th)) * th
+            tx = int(rng.integers(0, (dp.out_w + tw - 1) // tw)) * tw
+            lv = int(b[ty, tx])
+            if lv in seen:
+                continue
+            seen.add(lv)
+            want = budget.render_fixed(hip_ctx, dp, lv).cpu().numpy()
+            assert np.array_equal(out[ty:ty + th, tx:tx + tw], want[ty:ty + th, tx:tx + tw]), (i, tx, ty, lv)
+    rmse = budget._frame_rmse(outs, refs)
+    mean_n = float(np.mean([float((bb.to(torch.int32) & 0xFFFF).float().mean()) for bb in bufs.budget]))
+    print(f"[budget] tile levels: rmse {rmse * 255:.3f}/255 (target {target * 255:.3f}) mean n {mean_n:.1f}",
+          flush=True)
+    assert rmse <= target * 1.15
+    assert mean_n < 1024
