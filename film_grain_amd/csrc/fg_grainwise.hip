@@ -530,21 +530,24 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
                 const float yl = ceilf((typ - eR) - 0.5f), yh = floorf((typ + eR) - 0.5f);
                 const bool cand = ok & (xl <= xh) & (yl <= yh) & (xl <= fx1) & (xh >= fx0) & (yl <= fy1) &
                                   (yh >= fy0);
-                const uint32_t base = (uint32_t)gl | ((uint32_t)p << 6);
-                const float cxl = fmaxf(xl, fx0), cxh = fminf(xh, fx1);
-                const float cyl = fmaxf(yl, fy0), cyh = fminf(yh, fy1);
-                const int tx = (int)(cxl - fx0), ty = (int)(cyl - fy0);
-                const bool multi = cand & ((cxh > cxl) | (cyh > cyl));
+                // a candidate box of one pixel lies in the tile (it meets
+                // it); wider boxes queue their pixels inside the tile
+                const bool multi = cand & ((xh > xl) | (yh > yl));
                 if (!__any(multi)) {
                     // one pixel per candidate: the drain's test of it, run
                     // here in the lane (same f32 expression)
                     if (cand) {
                         const int k = sk[p];
-                        const float dx = (cxl + 0.5f) - txp, dy = (cyl + 0.5f) - typ;
+                        const int tx = (int)(xl - fx0), ty = (int)(yl - fy0);
+                        const float dx = (xl + 0.5f) - txp, dy = (yl + 0.5f) - typ;
                         if (dx * dx + dy * dy <= eR * eR)
                             atomicOr(&bits[(ty * a.tile_w + tx) * words + (k >> 5)], 1u << (k & 31));
                     }
                 } else {
+                    const uint32_t base = (uint32_t)gl | ((uint32_t)p << 6);
+                    const float cxl = fmaxf(xl, fx0), cxh = fminf(xh, fx1);
+                    const float cyl = fmaxf(yl, fy0), cyh = fminf(yh, fy1);
+                    const int tx = (int)(cxl - fx0), ty = (int)(cyl - fy0);
                     const int bw = cand ? (int)(cxh - cxl) + 1 : 0, bh = cand ? (int)(cyh - cyl) + 1 : 0;
                     const int nb = bw * bh;
                     for (int j = 0; __any(j < nb); ++j) {
@@ -571,23 +574,30 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
                 // rounds search the running totals.  The wave's LDS accesses
                 // run in order, so no barrier is needed.
                 const bool mapped = total <= kOwnerMap;  // uniform
-                if (mapped)
+                if (mapped) {
                     for (int j = 0; j < len; ++j) s_owner[wid][excl + j] = (uint8_t)lane;
-                for (int s0 = 0; s0 < total; s0 += 64) {
-                    const int e = s0 + lane;
-                    int g = 0;
-                    if (mapped) {
-                        g = s_owner[wid][min(e, total - 1)];
-                    } else {
+                    for (int s0 = 0; s0 < total; s0 += 64) {
+                        const int e = s0 + lane;
+                        const int g = s_owner[wid][min(e, total - 1)];
+                        const bool ok = e < total;
+                        const int ga = g << 2;
+                        const uint32_t pos = (uint32_t)(from_lane_i(dl, ga) + e);
+                        const int p = ok ? (int)a.cell_list[pos] : 0;
+                        visit(ok, p, g, from_lane_f(bx, ga), from_lane_f(by, ga), from_lane_f(R, ga));
+                    }
+                } else {
+                    for (int s0 = 0; s0 < total; s0 += 64) {
+                        const int e = s0 + lane;
+                        int g = 0;
 #pragma unroll
                         for (int b = 32; b >= 1; b >>= 1)
                             g += from_lane_i((int)incl, (g + b - 1) << 2) <= e ? b : 0;
+                        const bool ok = e < total;  // then g <= 63 and its list holds e
+                        const int ga = g << 2;
+                        const uint32_t pos = (uint32_t)(from_lane_i(dl, ga) + e);
+                        const int p = ok ? (int)a.cell_list[pos] : 0;
+                        visit(ok, p, g, from_lane_f(bx, ga), from_lane_f(by, ga), from_lane_f(R, ga));
                     }
-                    const bool ok = e < total;  // then g <= 63 and its list holds e
-                    const int ga = g << 2;
-                    const uint32_t pos = (uint32_t)(from_lane_i(dl, ga) + e);
-                    const int p = ok ? (int)a.cell_list[pos] : 0;
-                    visit(ok, p, g, from_lane_f(bx, ga), from_lane_f(by, ga), from_lane_f(R, ga));
                 }
             } else {
                 for (int it = 0; __any(it < len); ++it) {

@@ -1,15 +1,17 @@
 """Summaries of rocprofv3 output for the kernels of this library.
 
 rocprofv3 writes its results as SQLite databases (`<dir>/<name>_results.db`).
-This module reads three of them -- a kernel trace, a FETCH_SIZE counter run and
-a WRITE_SIZE counter run of the same command -- and reports, per kernel, the
-average duration and the HBM bytes per dispatch.  FETCH_SIZE and WRITE_SIZE
-(KiB per dispatch) are scaled by what the same counters read for a
-device-to-device copy of known size that the profiled command performs first
-(bench.py --copy-calibration-mib M): factor = counter KiB / (M * 1024).
+This module reads a kernel trace and, optionally, a FETCH_SIZE and a
+WRITE_SIZE counter run of the same command, and reports per kernel the average
+duration and the HBM bytes per dispatch.  FETCH_SIZE and WRITE_SIZE (KiB per
+dispatch) are scaled by what the same counters read for a device-to-device
+copy of known size that the profiled command performs first (bench.py
+--copy-calibration-mib M): factor = counter KiB / (M * 1024).  Further counter
+runs (--pmc, any counters) add a table of each counter's average per
+dispatch for the longest kernels.
 
-    python -m film_grain_amd.profiling --trace T --fetch F --write W --copy-mib 256 \
-        --kernel splat_hbm_kernel --markdown out.md [--json traffic.json --key K]
+    python -m film_grain_amd.profiling --trace T [--fetch F --write W --copy-mib 256] \
+        --kernel splat_hbm_kernel [--pmc P ...] --markdown out.md [--json traffic.json --key K]
 """
 from __future__ import annotations
 
@@ -62,12 +64,27 @@ def copy_factor(values: dict, copy_kib: float) -> float:
     return best / copy_kib
 
 
-def summarize(trace: str, fetch: str, write: str, copy_mib: int, kernel: str) -> dict:
+def counter_table(dbs: list[str]) -> dict:
+    """kernel name -> {counter: average per dispatch} over counter runs."""
+    out: dict = {}
+    for db in dbs:
+        con = sqlite3.connect(_db(db))
+        vals: dict = {}
+        for name, cn, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+            vals.setdefault((name, cn), []).append(float(v))
+        for (name, cn), v in vals.items():
+            out.setdefault(name, {})[cn] = statistics.fmean(v)
+    return out
+
+
+def summarize(trace: str, fetch: str | None, write: str | None, copy_mib: int, kernel: str,
+              pmc: list[str] | None = None) -> dict:
     times = kernel_times(trace)
-    fv = counter_values(fetch, "FETCH_SIZE")
-    wv = counter_values(write, "WRITE_SIZE")
+    fv = counter_values(fetch, "FETCH_SIZE") if fetch else {}
+    wv = counter_values(write, "WRITE_SIZE") if write else {}
     kib = copy_mib * 1024.0
-    ff, wf = copy_factor(fv, kib), copy_factor(wv, kib)
+    ff = copy_factor(fv, kib) if fetch else None
+    wf = copy_factor(wv, kib) if write else None
     rows = []
     for name, (calls, avg_us) in sorted(times.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         f = statistics.fmean(fv[name]) if name in fv else None
@@ -79,17 +96,18 @@ def summarize(trace: str, fetch: str, write: str, copy_mib: int, kernel: str) ->
                          traffic_bytes=traffic))
     main = [r for r in rows if kernel in r["kernel"]]
     return dict(fetch_factor=ff, write_factor=wf, copy_mib=copy_mib, rows=rows,
-                kernel=main[0] if main else None)
+                kernel=main[0] if main else None, counters=counter_table(pmc or []))
 
 
 def markdown(s: dict, title: str, commands: list[str], notes: list[str]) -> str:
     lines = [f"# {title}", "", "Commands (run from the repository root on one MI355X):", "", "```"]
     lines += commands
-    lines += ["```", "",
-              f"Counter calibration: a {s['copy_mib']} MiB device-to-device copy reads "
-              f"FETCH_SIZE = {s['fetch_factor']:.4f} x its bytes and WRITE_SIZE = "
-              f"{s['write_factor']:.4f} x its bytes; per-kernel traffic below divides by these factors.",
-              "", "| kernel | calls | avg us | FETCH KiB/dispatch | WRITE KiB/dispatch | HBM bytes/dispatch (scaled) |",
+    lines += ["```", ""]
+    if s["fetch_factor"] is not None and s["write_factor"] is not None:
+        lines += [f"Counter calibration: a {s['copy_mib']} MiB device-to-device copy reads "
+                  f"FETCH_SIZE = {s['fetch_factor']:.4f} x its bytes and WRITE_SIZE = "
+                  f"{s['write_factor']:.4f} x its bytes; per-kernel traffic below divides by these factors.", ""]
+    lines += ["| kernel | calls | avg us | FETCH KiB/dispatch | WRITE KiB/dispatch | HBM bytes/dispatch (scaled) |",
               "|---|---:|---:|---:|---:|---:|"]
     for r in s["rows"]:
         if r["avg_us"] < 1.0 and r["fetch_kib"] is None:
@@ -98,6 +116,15 @@ def markdown(s: dict, title: str, commands: list[str], notes: list[str]) -> str:
         w = "" if r["write_kib"] is None else f"{r['write_kib']:.0f}"
         t = "" if r["traffic_bytes"] is None else f"{r['traffic_bytes']:.4g}"
         lines.append(f"| `{r['kernel'][:90]}` | {r['calls']} | {r['avg_us']:.1f} | {f} | {w} | {t} |")
+    counters = s.get("counters") or {}
+    if counters:
+        names = sorted({c for d in counters.values() for c in d})
+        top = [r["kernel"] for r in s["rows"] if r["kernel"] in counters][:6]
+        lines += ["", "Counters, average per dispatch (SQ_BUSY_CYCLES is summed over the shader engines):", "",
+                  "| kernel | " + " | ".join(names) + " |", "|---|" + "---:|" * len(names)]
+        for k in top:
+            lines.append(f"| `{k[:60]}` | " + " | ".join(f"{counters[k].get(n, float('nan')):.4g}" for n in names)
+                         + " |")
     lines += [""] + notes + [""]
     return "\n".join(lines)
 
@@ -105,9 +132,10 @@ def markdown(s: dict, title: str, commands: list[str], notes: list[str]) -> str:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-mib", type=int, required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--pmc", action="append", default=[], help="a counter run's output (repeatable)")
     ap.add_argument("--kernel", required=True, help="substring of the dominant kernel's name")
     ap.add_argument("--markdown")
     ap.add_argument("--title", default="Kernel profile")
@@ -116,13 +144,13 @@ def main(argv=None):
     ap.add_argument("--json", help="merge {key: traffic record} into this JSON file")
     ap.add_argument("--key")
     args = ap.parse_args(argv)
-    s = summarize(args.trace, args.fetch, args.write, args.copy_mib, args.kernel)
+    s = summarize(args.trace, args.fetch, args.write, args.copy_mib, args.kernel, args.pmc)
     if args.markdown:
         with open(args.markdown, "w") as f:
             f.write(markdown(s, args.title, args.command, args.note))
     k = s["kernel"]
     print(json.dumps(dict(kernel=k, fetch_factor=s["fetch_factor"], write_factor=s["write_factor"])))
-    if args.json and args.key and k:
+    if args.json and args.key and k and k["traffic_bytes"] is not None:
         try:
             table = json.load(open(args.json))
         except (OSError, ValueError):
