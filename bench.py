@@ -328,28 +328,28 @@ def adaptive_summary(steps: int = 3, warmup: int = 1):
         target = next(e for n, ms_, e in curve if n == tn)
         fixed_tn = next(ms_ for n, ms_, e in curve if n == tn)
         row = dict(target_fixed_n=tn, target_rmse_255=round(target * 255, 4), fixed_ms=round(fixed_tn, 3))
-        for kind in ("learned", "oracle"):
-            ms_, (outs, info) = timed(lambda: budget.render_tiled_frame(ctx, model, planes, bufs, target ** 2,
-                                                                       cal["kappa"], kind, refs=refs))
+        runs = {
+            "per_pixel": lambda: budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2, "learned"),
+            "tile_levels": lambda: budget.render_tiled_frame(ctx, model, planes, bufs, target ** 2, cal["kappa"],
+                                                             "learned", refs=refs),
+            "tile_levels_oracle": lambda: budget.render_tiled_frame(ctx, model, planes, bufs, target ** 2,
+                                                                    cal["kappa"], "oracle", refs=refs),
+        }
+        for kind, fn in runs.items():
+            ms_, (outs, _info) = timed(fn)
             rmse = budget._frame_rmse(outs, refs)
             fx = budget.ms_at_rmse(curve, rmse)
             row[kind] = dict(ms=round(ms_, 3), rmse_255=round(rmse * 255, 4), mean_budget=round(mean_budget(), 1),
                              fixed_ms_at_rmse=None if fx is None else round(fx, 3),
                              fixed_over_adaptive=None if fx is None else round(fx / ms_, 4))
-            log(f"[bench] adaptive ({kind}, tile levels) at fixed N={tn}'s rmse {target * 255:.3f}/255: "
+            log(f"[bench] adaptive ({kind}) at fixed N={tn}'s rmse {target * 255:.3f}/255: "
                 f"{ms_:.2f} ms/frame rmse {rmse * 255:.3f}/255 mean n {mean_budget():.1f}; fixed at that rmse "
                 f"{fx if fx is None else round(fx, 2)} ms (fixed N={tn}: {fixed_tn:.2f} ms)")
         targets.append(row)
-    head = targets[-1]["learned"]
+    # the headline: the per-pixel allocator (the device scale search, no host
+    # round trip) at the RMSE of fixed N = 1024
+    head = targets[-1]["per_pixel"]
     ms, rmse, mean_n, fixed_ms = head["ms"], head["rmse_255"] / 255, head["mean_budget"], head["fixed_ms_at_rmse"]
-    # the per-pixel allocator (budget = ceil(c s) per pixel) at the N = 1024 target
-    target = next(e for n, ms_, e in curve if n == budget.C5["n_fixed"])
-    pp_ms, (pp_outs, _) = timed(lambda: budget.render_adaptive_frame(ctx, model, planes, bufs, target ** 2,
-                                                                     "learned"))
-    pp_rmse = budget._frame_rmse(pp_outs, refs)
-    pp_fixed = budget.ms_at_rmse(curve, pp_rmse)
-    log(f"[bench] adaptive (learned, per-pixel budgets) {pp_ms:.2f} ms/frame rmse {pp_rmse * 255:.3f}/255; fixed at "
-        f"that rmse {pp_fixed} ms")
     fixed_1024 = next(ms_ for n, ms_, e in curve if n == budget.C5["n_fixed"])
     ratio = budget.s_ratio(refs)
     eq = budget.compare_at_equal_budget(ctx, model, budget.image_planes(50_000), mean_budgets=(256,))
@@ -384,7 +384,8 @@ def adaptive_summary(steps: int = 3, warmup: int = 1):
         "workload": "1920x1080 RGB, lognormal 0.1/0.05, sigma 0.8, zoom 1, budgets 16..1024, seed 50000; "
                     "RMSE against the N=4096 render; targets = the RMSE of fixed N=256, 512, 1024; "
                     "headline at the N=1024 target",
-        "allocator": "tile levels from the error model's spreads and the measured per-sample cost (n ~ s/sqrt(c))",
+        "allocator": "per-pixel budgets ceil(c s) from the error model's spreads (headline); tile levels with the "
+                     "measured per-sample cost (n ~ s/sqrt(c)) reported per target",
         "adaptive_ms_per_frame": round(ms, 3), "rmse_255": round(rmse * 255, 4), "mean_budget": round(mean_n, 1),
         "fixed_ms_at_equal_rmse": fixed_ms,
         "fixed_over_adaptive": None if fixed_ms is None else round(fixed_ms / ms, 4),
@@ -392,8 +393,6 @@ def adaptive_summary(steps: int = 3, warmup: int = 1):
         "targets": targets,
         "cost_model": {"kappa": round(cal["kappa"], 6), "ms_per_sample": [round(v, 6) for v in cal["ms_per_sample"]],
                        "lambdas": [round(v, 3) for v in cal["lambdas"]]},
-        "per_pixel_allocator": {"ms": round(pp_ms, 3), "rmse_255": round(pp_rmse * 255, 4),
-                                "fixed_ms_at_rmse": None if pp_fixed is None else round(pp_fixed, 3)},
         "fixed_curve": [dict(n=n, ms=round(t, 3), rmse_255=round(e_ * 255, 4)) for n, t, e_ in curve],
         "allocation_bound_E_s2_over_E_s_sq": round(ratio, 4),
         "equal_samples_256_rmse_255": {r["kind"]: round(r["rmse"] * 255, 4) for r in eq},
@@ -451,30 +450,37 @@ def measure(workload: str, steps: int, warmup: int, cpu: bool, cpu_seconds: floa
     cfg = ctx.launch_config(desc, algo)
     log(f"[bench] launch config {cfg} (bucket {key}, tuned table entry: {tuned})")
 
+    # a frame's pixel-wise planes go through one call, which hashes each
+    # cell's stream once for all of them (the planes share the seed)
+    joint = algo == "pixel" and 1 < len(lam_d) <= 3 and pix_variant is None
+    outs = [out_d[i] for i in range(len(lam_d))]
+
+    def step_frame():
+        if joint:
+            ctx.render_planes_device(desc, lam_d, offs_d, outs, stream)
+        else:
+            for i, l in enumerate(lam_d):
+                ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
+
     for _ in range(warmup):
-        for i, l in enumerate(lam_d):
-            ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
+        step_frame()
     ctx.stream_status(stream)
     torch.cuda.synchronize()
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps * len(lam_d))]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    k = 0
-    for _ in range(steps):
-        for i, l in enumerate(lam_d):
-            ev[k][0].record(stream)
-            ctx.render_device(algo, desc, l, offs_d, out_d[i], stream)
-            ev[k][1].record(stream)
-            k += 1
+    for k in range(steps):
+        ev[k][0].record(stream)
+        step_frame()
+        ev[k][1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.stream_status(stream)
     secs = t1 - t0
     launch_ms = [a.elapsed_time(b) for a, b in ev]
-    log(f"[bench] {steps} steps in {secs:.4f} s; launch ms mean {float(np.mean(launch_ms)):.3f} "
-        f"min {min(launch_ms):.3f} max {max(launch_ms):.3f}")
+    log(f"[bench] {steps} steps in {secs:.4f} s; step ms mean {float(np.mean(launch_ms)):.3f} "
+        f"min {min(launch_ms):.3f} max {max(launch_ms):.3f} ({'one call for the frame' if joint else 'one call per plane'})")
 
     # kernel timing pass: HIP events recorded by the library on the launch stream
     # around the generation kernels and around the coverage-test kernel

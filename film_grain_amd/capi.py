@@ -38,6 +38,7 @@ EXPORTED = [
     "fg_lambda_device", "fg_color_split_device", "fg_color_merge_device", "fg_log_f32_device", "fg_cancel",
     "fg_pixelwise_grains", "fg_grainwise_pixel_state", "fg_max_grains", "fg_render_pixelwise_device_noalloc",
     "fg_math_device", "fg_launch_bucket", "fg_set_launch_table", "fg_get_launch_table",
+    "fg_render_pixelwise_planes_device",
     "fg_version",
 ]
 COLOR_LUMA, COLOR_RGB = 0, 1
@@ -140,6 +141,9 @@ def lib() -> C.CDLL:
     L.fg_set_launch_config.restype = C.c_int
     L.fg_get_launch_config.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchConfig)]
     L.fg_get_launch_config.restype = C.c_int
+    L.fg_render_pixelwise_planes_device.argtypes = [vp, C.POINTER(PlaneDesc), C.c_int, C.POINTER(vp), vp,
+                                                    C.POINTER(vp), vp]
+    L.fg_render_pixelwise_planes_device.restype = C.c_int
     L.fg_launch_bucket.argtypes = [C.POINTER(PlaneDesc), C.c_int, C.POINTER(LaunchKey)]
     L.fg_launch_bucket.restype = C.c_int
     L.fg_set_launch_table.argtypes = [vp, C.POINTER(LaunchEntry), C.c_int]
@@ -261,6 +265,20 @@ class Context:
         fn = self._L.fg_render_pixelwise_device if algo == "pixel" else self._L.fg_render_grainwise_device
         rc = fn(self.handle, C.byref(desc), C.c_void_p(lam.data_ptr()), C.c_void_p(offsets.data_ptr()),
                 C.c_void_p(out.data_ptr()), C.c_void_p(_stream_handle(stream)))
+        self._check(rc)
+
+    def render_planes_device(self, desc: PlaneDesc, lams, offsets, outs, stream=None):
+        """The pixel-wise planes of one frame (1-3 lambda tensors, one
+        description): the cells' random streams are hashed once for all of
+        them (fg_render_pixelwise_planes_device)."""
+        n = len(lams)
+        if n != len(outs):
+            raise ValueError("one output per lambda plane")
+        la = (C.c_void_p * n)(*[l.data_ptr() for l in lams])
+        oa = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
+        rc = self._L.fg_render_pixelwise_planes_device(self.handle, C.byref(desc), n, la,
+                                                       C.c_void_p(offsets.data_ptr()), oa,
+                                                       C.c_void_p(_stream_handle(stream)))
         self._check(rc)
 
     def render_device_noalloc(self, desc: PlaneDesc, lam, offsets, out, stream=None):

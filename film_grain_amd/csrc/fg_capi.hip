@@ -62,6 +62,7 @@ struct fg_ctx {
     DevBuf thresholds, bin_count, bin_start, scan_temp, grains;
     DevBuf samples;   // sample order of the splat kernel
     DevBuf bin_mask;  // nonempty cells per bin (256 bits)
+    DevBuf plane_counts;  // planes of one frame: thresholds, bin counts and masks per plane
     DevBuf pix_state; // grain-wise: per input pixel grain count and stream state (36 B)
     DevBuf tiles, tile_temp;  // heaviest-first tile order: weights and ids (in, out), sort workspace
     DevBuf levels;            // adaptive sampling: level sizes, level tables, per-tile levels
@@ -134,6 +135,7 @@ void free_buffers(fg_ctx *ctx)
     free_buf(ctx->grains);
     free_buf(ctx->samples);
     free_buf(ctx->bin_mask);
+    free_buf(ctx->plane_counts);
     free_buf(ctx->pix_state);
     free_buf(ctx->tiles);
     free_buf(ctx->tile_temp);
@@ -629,6 +631,14 @@ int scratch_get(fg_ctx *ctx, ScratchSource &src, DevBuf &b, size_t bytes, void *
 // The per-plane generation of the HBM path: Poisson thresholds, per-bin
 // counts and nonempty masks, their scan, and every cell's grains written
 // contiguously per bin.
+// A plane's thresholds, bin counts (nbins + 1 entries) and nonempty masks,
+// computed beforehand with the other planes of its frame.
+struct PreCounts {
+    double *T = nullptr;
+    uint32_t *cnt = nullptr;
+    unsigned long long *mask = nullptr;
+};
+
 struct HbmGen {
     double *T = nullptr;
     uint32_t *start = nullptr;
@@ -652,17 +662,23 @@ int check_total(fg_ctx *ctx, unsigned long long total, const char *what)
 }
 
 int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::PixGeom &g, int dist_kind,
-                 hipStream_t s, HbmGen *out)
+                 hipStream_t s, HbmGen *out, const PreCounts *pre = nullptr)
 {
     const char *what = "Pixel renderer";
     int rc;
     const long long nbins = (long long)g.nbx * g.nby;
     const size_t npx = (size_t)a.in_w * a.in_h;
     void *T_p, *cnt_p, *start_p, *mask_p, *temp_p, *grains_p;
-    if ((rc = scratch_get(ctx, src, ctx->thresholds, npx * sizeof(double), &T_p, what)) != FG_OK) return rc;
-    if ((rc = scratch_get(ctx, src, ctx->bin_count, (size_t)(nbins + 1) * 4, &cnt_p, what)) != FG_OK) return rc;
+    if (pre) {
+        T_p = pre->T;
+        cnt_p = pre->cnt;
+        mask_p = pre->mask;
+    } else {
+        if ((rc = scratch_get(ctx, src, ctx->thresholds, npx * sizeof(double), &T_p, what)) != FG_OK) return rc;
+        if ((rc = scratch_get(ctx, src, ctx->bin_count, (size_t)(nbins + 1) * 4, &cnt_p, what)) != FG_OK) return rc;
+        if ((rc = scratch_get(ctx, src, ctx->bin_mask, (size_t)nbins * 32, &mask_p, what)) != FG_OK) return rc;
+    }
     if ((rc = scratch_get(ctx, src, ctx->bin_start, (size_t)(nbins + 1) * 4, &start_p, what)) != FG_OK) return rc;
-    if ((rc = scratch_get(ctx, src, ctx->bin_mask, (size_t)nbins * 32, &mask_p, what)) != FG_OK) return rc;
     size_t temp_bytes = 0;
     hipError_t e = fg::scan_bins(nullptr, &temp_bytes, nullptr, nullptr, (int)(nbins + 1), s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
@@ -675,8 +691,10 @@ int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::Pi
     // a kernel: the caller may capture this into a graph)
     e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->d_sum64), 2, 0u, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(cnt + nbins, 1, 0u, s);
-    if (e == hipSuccess) e = fg::launch_threshold(a, T, s);
-    if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, mask, s);
+    if (!pre) {
+        if (e == hipSuccess) e = fg::launch_threshold(a, T, s);
+        if (e == hipSuccess) e = fg::launch_bin_count(a, g, T, cnt, mask, s);
+    }
     if (e == hipSuccess) e = fg::launch_sum_u64(cnt, nbins, ctx->d_sum64, s);
     if (e == hipSuccess) e = fg::scan_bins(temp_p, &temp_bytes, cnt, start, (int)(nbins + 1), s);
     unsigned long long total = 0;
@@ -701,7 +719,7 @@ int hbm_generate(fg_ctx *ctx, ScratchSource &src, fg::PixelArgs &a, const fg::Pi
 // Per-plane generation into HBM, then the splat-from-HBM kernel.  With an
 // arena, every scratch buffer comes from it and nothing is allocated.
 int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const float4 &range, int dist_kind,
-                  int block, hipStream_t s, ScratchSource src = ScratchSource())
+                  int block, hipStream_t s, ScratchSource src = ScratchSource(), const PreCounts *pre = nullptr)
 {
     const char *what = "Pixel renderer";
     int rc = FG_OK;
@@ -716,7 +734,7 @@ int run_pixel_hbm(fg_ctx *ctx, fg::PixelArgs &a, const fg::PixGeom &g, const flo
     splat_bounds(a, g, range, &st.eps, &st.safe_r);
     mark(ctx, 0, s);
     HbmGen gen;
-    if ((rc = hbm_generate(ctx, src, a, g, dist_kind, s, &gen)) != FG_OK) return rc;
+    if ((rc = hbm_generate(ctx, src, a, g, dist_kind, s, &gen, pre)) != FG_OK) return rc;
     uint32_t *start = gen.start;
     const float4 *grains = gen.grains;
     // heaviest-first tile order: weights and ids (in, out), sort workspace
@@ -923,6 +941,72 @@ int do_pixelwise(fg_ctx *ctx, const fg_plane_desc *d, const float *d_lambda, con
                  float *d_out, hipStream_t s, bool stats)
 {
     return do_pixelwise_budget(ctx, d, d_lambda, d_offs, d_out, s, stats, nullptr);
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The planes of one frame (one description, n λ planes): on the HBM path one
+// counting pass serves all of them (bin_count_planes_kernel), then each plane
+// draws its grains and is splatted as a single plane is.  Other formulations
+// render the planes one by one.
+int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float *const *lams, const float *d_offs,
+                        float *const *outs, hipStream_t s)
+{
+    const char *what = "Pixel renderer";
+    int variant = pixel_variant(ctx, d);
+    float4 range{};
+    fg::PixGeom geom{};
+    int tw = 0, th = 0, lds = 0;
+    bool joint = n > 1 && n <= fg::kFramePlanes && variant == 3;
+    if (joint) {
+        hipError_t e = fg::launch_offsets_range(reinterpret_cast<const float2 *>(d_offs), (int)d->n_samples,
+                                                ctx->d_range, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&range, ctx->d_range, sizeof range, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_error(ctx, e, what);
+        joint = plane_geometry(d, range, &geom) && hbm_config(ctx, d, &tw, &th, &lds, 4LL * ctx->num_cus);
+    }
+    if (!joint) {
+        for (int i = 0; i < n; ++i) {
+            const int rc = do_pixelwise(ctx, d, lams[i], d_offs, outs[i], s, false);
+            if (rc != FG_OK) return rc;
+        }
+        return FG_OK;
+    }
+    fg::PixelArgs a[fg::kFramePlanes];
+    for (int i = 0; i < n; ++i) {
+        const int rc = pixel_args(ctx, d, lams[i], d_offs, outs[i], tw, th, lds, ctx->d_range, ctx->d_status, &a[i]);
+        if (rc != FG_OK) return rc;
+    }
+    // the previous call's kernels may still read the counts
+    int rc = scratch_acquire(ctx, what);
+    if (rc != FG_OK) return rc;
+    const long long nbins = (long long)geom.nbx * geom.nby;
+    const size_t t_b = align256((size_t)d->in_w * d->in_h * sizeof(double));
+    const size_t c_b = align256((size_t)(nbins + 1) * 4), m_b = align256((size_t)nbins * 32);
+    if ((rc = ensure_bytes(ctx, ctx->plane_counts, (size_t)n * (t_b + c_b + m_b), what)) != FG_OK) return rc;
+    unsigned char *base = static_cast<unsigned char *>(ctx->plane_counts.p);
+    PreCounts pre[fg::kFramePlanes];
+    fg::FramePlanes fp{};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n; ++i) {
+        unsigned char *b = base + (size_t)i * (t_b + c_b + m_b);
+        pre[i].T = reinterpret_cast<double *>(b);
+        pre[i].cnt = reinterpret_cast<uint32_t *>(b + t_b);
+        pre[i].mask = reinterpret_cast<unsigned long long *>(b + t_b + c_b);
+        fp.T[i] = pre[i].T;
+        fp.cnt[i] = pre[i].cnt;
+        fp.mask[i] = pre[i].mask;
+        if (e == hipSuccess) e = fg::launch_threshold(a[i], pre[i].T, s);
+    }
+    if (e == hipSuccess) e = fg::launch_bin_count_planes(a[0], geom, fp, n, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    for (int i = 0; i < n; ++i) {
+        rc = run_pixel_hbm(ctx, a[i], geom, range, (int)d->dist_kind, splat_hbm_block(ctx, d), s, ScratchSource(),
+                           &pre[i]);
+        if (rc != FG_OK) return rc;
+    }
+    return FG_OK;
 }
 
 // Grain-wise formulation: 1 the reference's loop order over all (grain,
@@ -1218,7 +1302,6 @@ int validate_frame(fg_ctx *ctx, const fg_frame_desc *fd)
     return FG_OK;
 }
 
-size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // One frame on the device (src/lib.rs:141-165 with src/color.rs:47-114):
 // split, then per rendered plane the lambda pre-pass and the integrator, then
@@ -1233,7 +1316,9 @@ int frame_device(fg_ctx *ctx, const fg_frame_desc *fd, const float *d_rgb, const
     const size_t nin = (size_t)d.in_w * d.in_h, nout = (size_t)d.out_w * d.out_h;
     const int nr = fd->color_mode == FG_COLOR_RGB ? 3 : 1;
     const size_t in_b = align256(nin * 4), out_b = align256(nout * 4);
-    const size_t bytes = 4 * in_b + 256 + (d_planes ? 0 : (size_t)nr * out_b);
+    // RGB pixel-wise frames keep all three lambda planes for one joint render
+    const bool joint = nr == 3 && fd->algo == FG_ALGO_PIXEL;
+    const size_t bytes = (joint ? 6 : 4) * in_b + 256 + (d_planes ? 0 : (size_t)nr * out_b);
     // an earlier call's kernels may still read the scratch (on another stream)
     int rc = scratch_acquire(ctx, what);
     if (rc != FG_OK) return rc;
@@ -1241,18 +1326,31 @@ int frame_device(fg_ctx *ctx, const fg_frame_desc *fd, const float *d_rgb, const
     unsigned char *base = static_cast<unsigned char *>(ctx->frame.p);
     float *sp[3] = {reinterpret_cast<float *>(base), reinterpret_cast<float *>(base + in_b),
                     reinterpret_cast<float *>(base + 2 * in_b)};
-    float *lam = reinterpret_cast<float *>(base + 3 * in_b);
-    unsigned *word = reinterpret_cast<unsigned *>(base + 4 * in_b);
-    float *outp = d_planes ? d_planes : reinterpret_cast<float *>(base + 4 * in_b + 256);
+    const size_t lam_at = joint ? 6 * in_b : 4 * in_b;  // after the split planes and the lambda planes
+    float *lam3[3] = {reinterpret_cast<float *>(base + 3 * in_b), reinterpret_cast<float *>(base + 4 * in_b),
+                      reinterpret_cast<float *>(base + 5 * in_b)};
+    float *lam = lam3[0];
+    unsigned *word = reinterpret_cast<unsigned *>(base + lam_at);
+    float *outp = d_planes ? d_planes : reinterpret_cast<float *>(base + lam_at + 256);
     hipError_t e = fg::launch_split((int)fd->color_mode, d_rgb, (long long)nin, sp[0], sp[1], sp[2], s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    for (int i = 0; i < nr; ++i) {
+    if (joint) {
         if (cancel && cancel(user)) return set_error(ctx, FG_ECANCELLED, "cancelled");
-        e = fg::launch_lambda(sp[i], (long long)nin, fd->inv_e_pi_r2, word, nullptr, lam, s);
+        for (int i = 0; i < 3 && e == hipSuccess; ++i)
+            e = fg::launch_lambda(sp[i], (long long)nin, fd->inv_e_pi_r2, word, nullptr, lam3[i], s);
         if (e != hipSuccess) return hip_error(ctx, e, what);
-        rc = fd->algo == FG_ALGO_PIXEL ? do_pixelwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false)
-                                       : do_grainwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false);
-        if (rc != FG_OK) return rc;
+        float *outs[3] = {outp, outp + nout, outp + 2 * nout};
+        const float *lams[3] = {lam3[0], lam3[1], lam3[2]};
+        if ((rc = do_pixelwise_planes(ctx, &d, 3, lams, d_offs, outs, s)) != FG_OK) return rc;
+    } else {
+        for (int i = 0; i < nr; ++i) {
+            if (cancel && cancel(user)) return set_error(ctx, FG_ECANCELLED, "cancelled");
+            e = fg::launch_lambda(sp[i], (long long)nin, fd->inv_e_pi_r2, word, nullptr, lam, s);
+            if (e != hipSuccess) return hip_error(ctx, e, what);
+            rc = fd->algo == FG_ALGO_PIXEL ? do_pixelwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false)
+                                           : do_grainwise(ctx, &d, lam, d_offs, outp + (size_t)i * nout, s, false);
+            if (rc != FG_OK) return rc;
+        }
     }
     if (fd->color_mode == FG_COLOR_RGB)
         e = fg::launch_merge(FG_COLOR_RGB, outp, outp + nout, outp + 2 * nout, (int)d.out_w, (int)d.out_h,
@@ -1599,6 +1697,23 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
     if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
     return do_pixelwise(ctx, desc, d_lambda, d_offsets_input_xy, d_out,
                         reinterpret_cast<hipStream_t>(hip_stream), false);
+}
+
+int fg_render_pixelwise_planes_device(fg_ctx *ctx, const fg_plane_desc *desc, int n_planes,
+                                      const float *const *d_lambdas, const float *d_offsets_input_xy,
+                                      float *const *d_outs, void *hip_stream)
+{
+    int rc = validate(ctx, desc);
+    if (rc != FG_OK) return rc;
+    if (n_planes < 1 || n_planes > 3) return set_error(ctx, FG_EINVAL, "n_planes must be 1, 2 or 3");
+    if (!d_lambdas || !d_outs || !d_offsets_input_xy) return set_error(ctx, FG_EINVAL, "null buffer");
+    for (int i = 0; i < n_planes; ++i)
+        if (!d_lambdas[i] || !d_outs[i]) return set_error(ctx, FG_EINVAL, "null buffer");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_error(ctx, e, "Pixel renderer");
+    return do_pixelwise_planes(ctx, desc, n_planes, d_lambdas, d_offsets_input_xy, d_outs,
+                               reinterpret_cast<hipStream_t>(hip_stream));
 }
 
 int fg_render_pixelwise_device_noalloc(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,

@@ -67,6 +67,19 @@ hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *
                             unsigned long long *bin_mask, hipStream_t s);
 // hipCUB exclusive sum; with temp == nullptr only sets *temp_bytes.
 hipError_t scan_bins(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, int n, hipStream_t s);
+// The planes of one frame (same seed and geometry): their thresholds, and
+// where their bin counts and nonempty masks go.
+constexpr int kFramePlanes = 3;
+struct FramePlanes {
+    const double *T[kFramePlanes];
+    uint32_t *cnt[kFramePlanes];
+    unsigned long long *mask[kFramePlanes];
+};
+// bin_count for n_planes (1..3) planes in one pass: each cell hashed once
+// (the planes' cell streams are the same), counted against every plane's
+// threshold.
+hipError_t launch_bin_count_planes(const PixelArgs &a, const PixGeom &g, const FramePlanes &fp, int n_planes,
+                                   hipStream_t s);
 hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind, const double *T,
                              const uint32_t *bin_start, const unsigned long long *bin_mask, float4 *grains,
                              hipStream_t s);

@@ -309,6 +309,157 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGe
     if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
 }
 
+// Pass 1 for the planes of one frame (NP <= kFramePlanes).  The reference
+// renders every plane with the same seed, and a cell's stream depends only
+// on the seed and the cell (src/rng.rs:26-29, src/lib.rs:146-153), so the
+// planes share each cell's hash, seed words and Knuth uniforms; only their
+// thresholds differ.  The hashes and the empty-cell test's two seed words
+// are computed once per cell, and each plane's count of a cell runs on the
+// one uniform sequence against its own threshold.  The counts and masks
+// equal NP runs of bin_count_kernel.
+template <int NP>
+__global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a, PixGeom g, FramePlanes fp)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ BinRecs s_rec[NW];
+    __shared__ unsigned long long s_mask[NW][NP][4];  // the planes' nonempty masks
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int bin = blockIdx.x * NW + wid;
+    if (bin >= g.nbx * g.nby) return;
+    int ci0, cj0;
+    bin_origin(g, bin, ci0, cj0);
+    const int ci = ci0 + (lane & 15), cjl = cj0 + 4 * (lane >> 4);
+    const uint64_t col = mix_col(a.seed, g.Ix0 + ci);
+    unsigned long long mu[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int cj = cjl + q;
+        bool ne[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) ne[p] = false;
+        if (ci < g.W && cj < g.H) {
+            double e[NP];
+            bool knuth = false;
+            float sx, sy;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                e[p] = cell_threshold_at(a, fp.T[p], g.Ix0 + ci, g.Jy0 + cj, sx, sy);
+                knuth |= e[p] >= 0.0 && e[p] <= 1.5;
+            }
+            const uint64_t h = mix_row(col, g.Jy0 + cj);
+            uint64_t s0 = 0, s3 = 0;
+            double u1 = 0.0;
+            if (knuth) {
+                s0 = splitmix64(h);
+                s3 = splitmix64(h + 3 * kGolden);
+                u1 = standard_f64(rotl64(s0 + s3, 23) + s0);
+            }
+            bool any = false;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                // as bin_test: the rejection branch and the all-zero seed
+                // case are counted in full
+                ne[p] = e[p] <= 1.5 && (e[p] < 0.0 || (s0 | s3) == 0 || u1 > e[p]);
+                any |= ne[p];
+            }
+            if (any) {
+                const int slot = q << 6 | lane;
+                put_u64x2(s_rec[wid][0][slot], h, s0);
+                put_u64x2(s_rec[wid][1][slot], s3, 0ull);
+            }
+        }
+        mu[q] = 0ull;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const unsigned long long mp = __ballot(ne[p]);
+            if (lane == 0) s_mask[wid][p][q] = mp;
+            mu[q] |= mp;
+        }
+    }
+    const int n = compact_cells(mu, s_list[wid]);
+    unsigned long long total[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) total[p] = 0;
+    for (int base = 0; base < n; base += 64) {
+        uint32_t c[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) c[p] = 0;
+        if (base + lane < n) {
+            const int slot = s_list[wid][base + lane];
+            const int l = slot & 63, q = slot >> 6;
+            const uint4 r0 = s_rec[wid][0][slot], r1 = s_rec[wid][1][slot];
+            const uint64_t h = lo64(r0);
+            const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
+            double e[NP];
+            bool on[NP], knuth = false;
+            float sx, sy;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                on[p] = (s_mask[wid][p][q] >> l) & 1ull;  // this plane's bit of the cell
+                e[p] = cell_threshold_at(a, fp.T[p], i, j, sx, sy);
+                knuth |= on[p] && e[p] >= 0.0;
+            }
+            Xoshiro seeded;
+            seeded.s0 = hi64(r0);
+            seeded.s1 = splitmix64(h + kGolden);
+            seeded.s2 = splitmix64(h + 2 * kGolden);
+            seeded.s3 = lo64(r1);
+            // a cell counted in full for the rejection branch alone has no
+            // recorded seed words
+            if (!knuth || (seeded.s0 | seeded.s1 | seeded.s2 | seeded.s3) == 0) seeded.seed(h);
+            if (knuth) {
+                // Knuth: count = multiplications until the product <= e
+                Xoshiro rng = seeded;
+                double pr = standard_f64(rng.next());
+                uint32_t k = 0;
+                bool pend[NP], left = false;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    pend[p] = on[p] && e[p] >= 0.0 && pr > e[p];
+                    left |= pend[p];
+                }
+                while (left) {
+                    pr = pr * standard_f64(rng.next());
+                    ++k;
+                    left = false;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        if (pend[p] && !(pr > e[p])) {
+                            c[p] = k;
+                            pend[p] = false;
+                        }
+                        left |= pend[p];
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                if (on[p] && e[p] < 0.0) {
+                    Xoshiro rng = seeded;
+                    const double r = poisson_rejection(rng, -e[p]);
+                    c[p] = !(r > 0.0) ? 0u : (r >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)r);
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            unsigned long long x = c[p];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            total[p] += x;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        if (lane == 0) {
+            fp.cnt[p][bin] = total[p] > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total[p];
+            if (total[p] > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
+        }
+        if (lane < 4) fp.mask[p][(size_t)bin * 4 + lane] = s_mask[wid][p][lane];
+    }
+}
+
 // Pass 2: per bin (one wave), the nonempty cells from the mask, their counts
 // (seeded in full) and their grains at bin_start[bin].
 template <int KIND>
@@ -1265,6 +1416,18 @@ hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *
                             unsigned long long *bin_mask, hipStream_t s)
 {
     hipLaunchKernelGGL(bin_count_kernel, bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_count, bin_mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_count_planes(const PixelArgs &a, const PixGeom &g, const FramePlanes &fp, int n_planes,
+                                   hipStream_t s)
+{
+    switch (n_planes) {
+    case 1: hipLaunchKernelGGL(bin_count_planes_kernel<1>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
+    case 2: hipLaunchKernelGGL(bin_count_planes_kernel<2>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
+    case 3: hipLaunchKernelGGL(bin_count_planes_kernel<3>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -109,6 +109,15 @@ int fg_render_pixelwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const flo
                                const float *d_offsets_input_xy, float *d_out, void *hip_stream);
 int fg_render_grainwise_device(fg_ctx *ctx, const fg_plane_desc *desc, const float *d_lambda,
                                const float *d_offsets_xy, float *d_out, void *hip_stream);
+/* The planes of one frame on the device: n_planes (1-3) lambda planes rendered
+ * with one description, as n_planes calls of fg_render_pixelwise_device give
+ * them.  The reference renders a frame's planes one after another with the
+ * same seed (src/lib.rs:146-153), so their cells draw the same random streams
+ * (src/rng.rs:26-29) and only the Poisson thresholds differ; here every
+ * cell's stream is hashed once for all planes.  Asynchronous on hip_stream. */
+int fg_render_pixelwise_planes_device(fg_ctx *ctx, const fg_plane_desc *desc, int n_planes,
+                                      const float *const *d_lambdas, const float *d_offsets_input_xy,
+                                      float *const *d_outs, void *hip_stream);
 
 /* Allocation-free variant of fg_render_pixelwise_device, with the same
  * arguments, result and asynchrony.  No counterpart in the reference: it is

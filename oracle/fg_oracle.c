@@ -31,6 +31,15 @@
  *   pixel-wise / grain-wise brute force on tiny images (every grain enumerated
  *                           without the window), coverage invariant E[out] = u on
  *                           flat fields, grain-wise/pixel-wise agreement of means.
+ *   cell / pixel dumps,     the same generator code as the integrators; digests
+ *   digests, counts         and counts equal the dumps cell by cell
+ *                           (tests/test_cellcheck.py, planted slips fail).
+ *   colour split / merge,   BT.709's defining identities and closed forms on
+ *   to_u8, resize_nearest   pure colours, split-then-merge exact on all 2^24
+ *                           8-bit colours, to_u8 on every level, integer-zoom
+ *                           replication and half-away rounding of the resize
+ *                           (tests/test_oracle_colour.py).
+ *   glibc_math              the C library itself (the reference's exp, log, tan).
  * Parity unpinned (no import of the Rust crates is possible here): the exact
  * decimal literals of rand_distr's ZIG_NORM tables beyond the entries checked,
  * rand_core's PCG32 increment constant in the StdRng seed fill, rand_distr's
