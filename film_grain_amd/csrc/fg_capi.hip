@@ -1379,6 +1379,9 @@ struct fg_batch {
     std::vector<long long> bin_off;    // first bin of each plane in the batch's bins
     std::vector<void *> samples;       // sample table buffer per plane
     std::vector<int> qb;               // sample buckets per plane
+    // runs of consecutive planes with one description and offset range (the
+    // planes of one image): counted in one pass, each cell hashed once
+    std::vector<std::pair<int, int>> count_groups;  // (first plane, planes)
     fg::BatchPlane *d_planes = nullptr;
     float4 *ranges = nullptr;
     int *status = nullptr;
@@ -1426,11 +1429,21 @@ hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
 {
     hipError_t e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->status), 1, 0u, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->total64), 2, 0u, s);
-    for (int p = 0; p < b->n && e == hipSuccess; ++p) {
-        const fg::BatchPlane &bp = b->planes[p];
-        e = fg::launch_threshold(bp.a, b->T[p], s);
+    for (const auto &grp : b->count_groups) {
+        if (e != hipSuccess) break;
+        const int p0 = grp.first, np = grp.second;
+        fg::FramePlanes fp{};
+        for (int k = 0; k < np && e == hipSuccess; ++k) {
+            const int p = p0 + k;
+            e = fg::launch_threshold(b->planes[p].a, b->T[p], s);
+            fp.T[k] = b->T[p];
+            fp.cnt[k] = b->cnt + b->bin_off[p];
+            fp.mask[k] = b->mask + 4 * b->bin_off[p];
+        }
+        const fg::BatchPlane &bp = b->planes[p0];
         if (e == hipSuccess)
-            e = fg::launch_bin_count(bp.a, bp.geo, b->T[p], b->cnt + b->bin_off[p], b->mask + 4 * b->bin_off[p], s);
+            e = np == 1 ? fg::launch_bin_count(bp.a, bp.geo, fp.T[0], fp.cnt[0], fp.mask[0], s)
+                        : fg::launch_bin_count_planes(bp.a, bp.geo, fp, np, s);
     }
     if (e == hipSuccess) e = fg::launch_sum_u64(b->cnt, b->total_bins, b->total64, s);
     if (e == hipSuccess) e = fg::launch_scan_u32(b->cnt, b->start, b->total_bins, b->partials, s);
@@ -1542,6 +1555,21 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     if (items > 0x7fffffffLL || b->total_bins >= 0xffffffffLL)
         return set_error(ctx, FG_EINVAL, "batch too large");
     b->n_items = (int)items;
+    // planes that share the description and the offsets' range share every
+    // cell's stream (the same seed and grid)
+    b->count_groups.clear();
+    for (int p = 0; p < n; ++p) {
+        if (!b->count_groups.empty()) {
+            auto &g = b->count_groups.back();
+            const int q = g.first;
+            if (g.second < fg::kFramePlanes && std::memcmp(&descs[q], &descs[p], sizeof(fg_plane_desc)) == 0 &&
+                std::memcmp(&r[q], &r[p], sizeof(float4)) == 0) {
+                ++g.second;
+                continue;
+            }
+        }
+        b->count_groups.push_back({p, 1});
+    }
     b->lds = *std::max_element(lds_p.begin(), lds_p.end());
     if ((size_t)b->lds + fg::splat_hbm_static_lds(b->block) > (size_t)ctx->max_lds)
         return set_error(ctx, FG_EINVAL, "batch planes need more LDS than a workgroup has");

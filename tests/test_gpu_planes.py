@@ -73,3 +73,30 @@ def test_joint_planes_full_c2_frame(hip_ctx, oracle):
     for c, a, b in zip(cases, single, joint):
         print(f"[planes] {c.name}: mean {a.mean():.5f}", flush=True)
         assert np.array_equal(a, b), c.name
+
+
+def test_batch_groups_an_images_planes(hip_ctx, oracle):
+    """A batch whose consecutive planes share a description and offsets (one
+    RGB image), between planes of other descriptions: the grouped planes are
+    counted in one pass and every plane still equals its single-plane render."""
+    import torch
+    from film_grain_amd import capi
+    rgb = synth.c2_rgb(9, 72, 48)
+    image = _planes(oracle, rgb, radius=0.1, stddev=0.05, dist="lognorm", zoom=2.0, n_samples=64, seed=9)
+    other = [fgcases.c1_case(oracle, size=40, n_samples=32), fgcases.build_case(oracle, "solo", rgb[0], 0.15,
+                                                                                 n_samples=48, seed=3)]
+    cases = [other[0]] + image + [other[1]]
+    ins = [(_dev(torch, c.lam), _dev(torch, c.offsets_input)) for c in cases]
+    want = []
+    for c, (lam, offs) in zip(cases, ins):
+        out = torch.zeros((c.desc.out_h, c.desc.out_w), dtype=torch.float32, device="cuda:0")
+        hip_ctx.render_device("pixel", c.desc, lam, offs, out)
+        want.append(out)
+    outs = [torch.full((c.desc.out_h, c.desc.out_w), -1.0, dtype=torch.float32, device="cuda:0") for c in cases]
+    batch = capi.Batch(hip_ctx, [(c.desc, t[0], t[1], o) for c, t, o in zip(cases, ins, outs)], capture_graph=True)
+    for _ in range(2):
+        batch.run()
+        batch.status()
+        for c, a, b in zip(cases, want, outs):
+            assert torch.equal(a, b), c.name
+    batch.close()
