@@ -309,6 +309,16 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGe
     if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
 }
 
+// Count of a rejection-branch cell (lambda * delta^2 >= 12) from its seeded
+// stream, saturated as cell_count_full does.  Kept out of line: few cells take
+// it, and inlined its f64 transcendentals would hold registers across the
+// whole counting loop of bin_count_planes_kernel.
+__device__ __noinline__ uint32_t rejection_count(Xoshiro rng, double lambda)
+{
+    const double r = poisson_rejection(rng, lambda);
+    return !(r > 0.0) ? 0u : (r >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)r);
+}
+
 // Pass 1 for the planes of one frame (NP <= kFramePlanes).  The reference
 // renders every plane with the same seed, and a cell's stream depends only
 // on the seed and the cell (src/rng.rs:26-29, src/lib.rs:146-153), so the
@@ -323,7 +333,7 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
     constexpr int NW = kBlock / 64;
     __shared__ uint16_t s_list[NW][256];
     __shared__ BinRecs s_rec[NW];
-    __shared__ unsigned long long s_mask[NW][NP][4];  // the planes' nonempty masks
+    __shared__ unsigned long long s_mask[NW][NP + 1][4];  // the planes' nonempty masks, then their union
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int bin = blockIdx.x * NW + wid;
     if (bin >= g.nbx * g.nby) return;
@@ -331,8 +341,7 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
     bin_origin(g, bin, ci0, cj0);
     const int ci = ci0 + (lane & 15), cjl = cj0 + 4 * (lane >> 4);
     const uint64_t col = mix_col(a.seed, g.Ix0 + ci);
-    unsigned long long mu[4];
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const int cj = cjl + q;
         bool ne[NP];
@@ -369,14 +378,18 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
                 put_u64x2(s_rec[wid][1][slot], s3, 0ull);
             }
         }
-        mu[q] = 0ull;
+        unsigned long long any = 0ull;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             const unsigned long long mp = __ballot(ne[p]);
             if (lane == 0) s_mask[wid][p][q] = mp;
-            mu[q] |= mp;
+            any |= mp;
         }
+        if (lane == 0) s_mask[wid][NP][q] = any;
     }
+    unsigned long long mu[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mu[q] = s_mask[wid][NP][q];
     const int n = compact_cells(mu, s_list[wid]);
     unsigned long long total[NP];
 #pragma unroll
@@ -435,11 +448,7 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                if (on[p] && e[p] < 0.0) {
-                    Xoshiro rng = seeded;
-                    const double r = poisson_rejection(rng, -e[p]);
-                    c[p] = !(r > 0.0) ? 0u : (r >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)r);
-                }
+                if (on[p] && e[p] < 0.0) c[p] = rejection_count(seeded, -e[p]);
             }
         }
 #pragma unroll
