@@ -1382,6 +1382,10 @@ struct fg_batch {
     // runs of consecutive planes with one description and offset range (the
     // planes of one image): counted in one pass, each cell hashed once
     std::vector<std::pair<int, int>> count_groups;  // (first plane, planes)
+    // the generation passes, each one launch over all planes (its offset
+    // and pointer arrays in gen_mem)
+    fg::BatchGen gen{};
+    unsigned char *gen_mem = nullptr;
     fg::BatchPlane *d_planes = nullptr;
     float4 *ranges = nullptr;
     int *status = nullptr;
@@ -1429,21 +1433,23 @@ hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
 {
     hipError_t e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->status), 1, 0u, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(b->total64), 2, 0u, s);
+    if (e == hipSuccess) e = fg::launch_batch_threshold(b->gen, s);
+    // counts per group of planes: a launch per group keeps the plane's
+    // arguments in the kernel's argument segment (one launch over all
+    // groups, each wave searching for its group, was 10% slower)
     for (const auto &grp : b->count_groups) {
         if (e != hipSuccess) break;
         const int p0 = grp.first, np = grp.second;
         fg::FramePlanes fp{};
-        for (int k = 0; k < np && e == hipSuccess; ++k) {
+        for (int k = 0; k < np; ++k) {
             const int p = p0 + k;
-            e = fg::launch_threshold(b->planes[p].a, b->T[p], s);
             fp.T[k] = b->T[p];
             fp.cnt[k] = b->cnt + b->bin_off[p];
             fp.mask[k] = b->mask + 4 * b->bin_off[p];
         }
         const fg::BatchPlane &bp = b->planes[p0];
-        if (e == hipSuccess)
-            e = np == 1 ? fg::launch_bin_count(bp.a, bp.geo, fp.T[0], fp.cnt[0], fp.mask[0], s)
-                        : fg::launch_bin_count_planes(bp.a, bp.geo, fp, np, s);
+        e = np == 1 ? fg::launch_bin_count(bp.a, bp.geo, fp.T[0], fp.cnt[0], fp.mask[0], s)
+                    : fg::launch_bin_count_planes(bp.a, bp.geo, fp, np, s);
     }
     if (e == hipSuccess) e = fg::launch_sum_u64(b->cnt, b->total_bins, b->total64, s);
     if (e == hipSuccess) e = fg::launch_scan_u32(b->cnt, b->start, b->total_bins, b->partials, s);
@@ -1455,26 +1461,97 @@ hipError_t batch_enqueue(fg_batch *b, hipStream_t s)
 {
     hipError_t e = batch_enqueue_counts(b, s);
     if (e == hipSuccess) e = fg::launch_capacity_check(b->total64, b->cap, b->status, s);
-    for (int p = 0; p < b->n && e == hipSuccess; ++p) {
-        const fg::BatchPlane &bp = b->planes[p];
-        e = fg::launch_bin_grains(bp.a, bp.geo, b->kind[p], b->T[p], bp.bin_start, b->mask + 4 * b->bin_off[p],
-                                  b->grains, s);
-        fg::SampleTable st{};
-        if (e == hipSuccess)
-            e = fg::launch_sample_table(bp.a.offs, bp.a.n_samples, bp.a.zoom, b->qb[p], b->samples[p], &st, s);
-        if (e == hipSuccess && b->list_mem[p]) {
-            // the plane record's table (its eps, and the list pointers the
-            // records already hold) over the table just written
-            fg::SampleTable t = bp.st;
-            e = build_lists(b->list_plans[p], t, b->list_mem[p], b->status, s);
-        }
-    }
+    if (e == hipSuccess) e = fg::launch_batch_bin_grains(b->gen, s);
+    if (e == hipSuccess) e = fg::launch_batch_sample_tables(b->gen, s);
     if (e == hipSuccess) e = fg::launch_batch_weights(b->d_planes, b->n, b->weights, b->n_items, s);
     if (e == hipSuccess) e = fg::launch_batch_order(b->d_planes, b->n, b->weights, b->n_items, b->items, nullptr, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(b->counter, 1, 0u, s);
     if (e == hipSuccess)
         e = fg::launch_splat_batch(b->d_planes, b->items, b->n_items, b->counter, b->grid, b->block, b->lds, s);
     return e;
+}
+
+// The offset and pointer arrays of the batched generation passes, laid out
+// in one device buffer (b->gen_mem) and described by b->gen.
+int batch_gen_setup(fg_batch *b, const char *what, hipStream_t s)
+{
+    const int n = b->n;
+    std::vector<unsigned char> host;
+    std::vector<std::pair<size_t, const void **>> fixups;  // (offset in gen_mem, field of b->gen)
+    auto put = [&](const void *src, size_t bytes, const void **field) {
+        const size_t off = (host.size() + 15) & ~(size_t)15;
+        host.resize(off + std::max<size_t>(bytes, 16));
+        if (bytes) std::memcpy(host.data() + off, src, bytes);
+        fixups.push_back({off, field});
+    };
+    fg::BatchGen &g = b->gen;
+    g = fg::BatchGen{};
+    g.n = n;
+    // thresholds: kBlock pixels a workgroup
+    std::vector<long long> thr(n + 1, 0), bins(n + 1, 0), lists(n + 1, 0);
+    std::vector<double *> T(n);
+    std::vector<int4> plan(n);
+    std::vector<uint32_t *> fill(n);
+    int max_samples = 1;
+    for (int p = 0; p < n; ++p) {
+        const fg::BatchPlane &bp = b->planes[p];
+        const long long px = (long long)bp.a.in_w * bp.a.in_h;
+        thr[p + 1] = thr[p] + (px + fg::kBlock - 1) / fg::kBlock;
+        bins[p + 1] = bins[p] + (long long)bp.geo.nbx * bp.geo.nby;
+        T[p] = b->T[p];
+        max_samples = std::max(max_samples, bp.a.n_samples);
+        const ListPlan &lp = b->list_plans[p];
+        long long lb = 0;
+        plan[p] = make_int4(0, 0, 0, 0);
+        fill[p] = nullptr;
+        if (b->list_mem[p]) {
+            if (lp.q < 1 || lp.q * lp.q > 65536 || lp.classes < 1) return set_error(b->ctx, FG_EINVAL, "batch list plan");
+            plan[p] = make_int4(lp.q, lp.classes, lp.ew, 0);
+            fill[p] = reinterpret_cast<uint32_t *>(b->list_mem[p]);
+            lb = (long long)lp.classes * ((lp.q * lp.q + 63) / 64);
+        }
+        lists[p + 1] = lists[p] + lb;
+    }
+    g.thr_blocks = thr[n];
+    g.total_bins = bins[n];
+    g.list_blocks = lists[n];
+    g.sort_blocks = (max_samples + fg::kBlock - 1) / fg::kBlock;
+    if (g.thr_blocks > 0x7fffffffLL || g.list_blocks > 0x7fffffffLL) return set_error(b->ctx, FG_EINVAL, "batch too large");
+    // planes by radius distribution, each with the first draw chunk of its
+    // bins in that sequence
+    std::vector<int> kp[2];
+    std::vector<long long> ko[2];
+    for (int k = 0; k < 2; ++k) ko[k].push_back(0);
+    for (int p = 0; p < n; ++p) {
+        const int k = b->kind[p] == 0 ? 0 : 1;
+        kp[k].push_back(p);
+        ko[k].push_back(ko[k].back() + (bins[p + 1] - bins[p] + fg::kDrawChunk - 1) / fg::kDrawChunk);
+    }
+    put(thr.data(), thr.size() * 8, reinterpret_cast<const void **>(&g.thr_off));
+    put(bins.data(), bins.size() * 8, reinterpret_cast<const void **>(&g.bin_off));
+    put(T.data(), T.size() * sizeof(double *), reinterpret_cast<const void **>(&g.T));
+    for (int k = 0; k < 2; ++k) {
+        g.n_kind[k] = (int)kp[k].size();
+        g.kind_chunks[k] = ko[k].back();
+        put(kp[k].data(), kp[k].size() * 4, reinterpret_cast<const void **>(&g.kind_plane[k]));
+        put(ko[k].data(), ko[k].size() * 8, reinterpret_cast<const void **>(&g.kind_off[k]));
+    }
+    put(plan.data(), plan.size() * sizeof(int4), reinterpret_cast<const void **>(&g.list_plan));
+    put(fill.data(), fill.size() * sizeof(uint32_t *), reinterpret_cast<const void **>(&g.list_fill));
+    put(lists.data(), lists.size() * 8, reinterpret_cast<const void **>(&g.list_off));
+    int rc = batch_alloc(b, (void **)&b->gen_mem, host.size(), what);
+    if (rc != FG_OK) return rc;
+    for (const auto &f : fixups) *f.second = b->gen_mem + f.first;
+    g.planes = b->d_planes;
+    g.cnt = b->cnt;
+    g.mask = b->mask;
+    hipError_t e = hipMemcpyAsync(b->gen_mem, host.data(), host.size(), hipMemcpyHostToDevice, s);
+    // the count pass reads the plane records (their grain fields are set
+    // after it and uploaded again)
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_planes, b->planes.data(), sizeof(fg::BatchPlane) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? FG_OK : hip_error(b->ctx, e, what);
 }
 
 int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *const *lams,
@@ -1584,6 +1661,7 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     if ((rc = batch_alloc(b, (void **)&b->total64, 16, what)) != FG_OK) return rc;
     if ((rc = batch_alloc(b, (void **)&b->d_planes, sizeof(fg::BatchPlane) * n, what)) != FG_OK) return rc;
 
+    if ((rc = batch_gen_setup(b, what, s)) != FG_OK) return rc;
     // counting pass over the current inputs sizes the grain array
     e = batch_enqueue_counts(b, s);
     unsigned long long total = 0;
@@ -1605,6 +1683,7 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
         bp.bin_start = b->start + b->bin_off[p];
         bp.grains = b->grains;
     }
+    b->gen.grains = b->grains;
     e = hipMemcpyAsync(b->d_planes, b->planes.data(), sizeof(fg::BatchPlane) * n, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, what);

@@ -7,7 +7,11 @@
 
 namespace fg {
 
-constexpr int kBinSide = 16;  // a bin is 16 x 16 cells (one workgroup, one cell per thread)
+constexpr int kBinSide = 16;  // a bin is 16 x 16 cells (one wave; a lane holds four cells)
+// The draw pass (bin_grains) gives each wave kDrawChunk consecutive bins of
+// a plane: the wave's set-up (its arguments, and in a batch the search for
+// its plane) is spread over the chunk's bins.
+constexpr int kDrawChunk = 8;
 
 // The plane's extended cell grid: every cell some point's window can reach.
 struct PixGeom {
@@ -121,6 +125,45 @@ struct BatchPlane {
     const float4 *grains;       // the batch's grain array (bin_start holds batch offsets)
     long long item_base;        // first (plane, tile) item of this plane
 };
+
+// The generation of a batch's planes: the thresholds, the draws and the
+// sample tables each one launch over all planes, in which a workgroup (or
+// wave) finds its plane by a binary search over first-block offsets (one
+// entry more than the planes).  In the draw pass a wave takes kDrawChunk
+// consecutive bins of one plane (the offsets count chunks).  Every pointer
+// is device memory.
+struct BatchGen {
+    const BatchPlane *planes;     // n planes: their a, geo, st and bin_start
+    int n;
+    const long long *thr_off;     // first threshold workgroup of each plane (kBlock pixels each)
+    long long thr_blocks;
+    const long long *bin_off;     // first bin of each plane in cnt and mask
+    long long total_bins;
+    double *const *T;             // each plane's thresholds (in_w x in_h)
+    // the planes of each radius distribution (0 constant, 1 lognormal) and
+    // their first chunks in that sequence (n_kind[k] + 1 entries)
+    const int *kind_plane[2];
+    const long long *kind_off[2];
+    int n_kind[2];
+    long long kind_chunks[2];     // chunks of the planes of each distribution
+    uint32_t *cnt;
+    unsigned long long *mask;
+    float4 *grains;
+    // sample tables (each plane's st): the candidate lists' plan per plane,
+    // (q, classes, e, 0) with classes == 0 for a plane without lists, their
+    // counters, and the first splat-lists workgroup of each plane
+    const int4 *list_plan;
+    uint32_t *const *list_fill;
+    const long long *list_off;
+    long long list_blocks;
+    int sort_blocks;              // workgroups per plane of the sample sort: ceil(max n_samples / kBlock)
+};
+// Thresholds of every plane (sets kStatusLambda as launch_threshold does).
+hipError_t launch_batch_threshold(const BatchGen &bg, hipStream_t s);
+// bin_grains of every plane (bin_start of each plane is the batch scan's).
+hipError_t launch_batch_bin_grains(const BatchGen &bg, hipStream_t s);
+// Every plane's sample table and candidate lists.
+hipError_t launch_batch_sample_tables(const BatchGen &bg, hipStream_t s);
 
 // Adaptive sampling: tile_level[t] = first level (level_n ascending) covering
 // the largest budget in tile t (a.budget).

@@ -42,10 +42,9 @@ namespace fg {
 
 // Also flags lambda values outside the accepted range (kStatusLambda), once
 // per wave.
-__global__ __launch_bounds__(kBlock) void threshold_kernel(const float *lambda, long long n, float delta,
-                                                           double *T, int *status)
+__device__ __forceinline__ void threshold_body(const float *lambda, long long n, float delta, double *T,
+                                               int *status, long long p)
 {
-    const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
     bool bad = false;
     if (p < n) {
         const float lam = lambda[p];
@@ -53,6 +52,12 @@ __global__ __launch_bounds__(kBlock) void threshold_kernel(const float *lambda, 
         T[p] = cell_threshold(lam, delta);
     }
     if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(status, kStatusLambda);
+}
+
+__global__ __launch_bounds__(kBlock) void threshold_kernel(const float *lambda, long long n, float delta,
+                                                           double *T, int *status)
+{
+    threshold_body(lambda, n, delta, T, status, (long long)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // Cells of a bin, one wave per bin: lane l holds the four cells of column
@@ -472,22 +477,15 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
 // Pass 2: per bin (one wave), the nonempty cells from the mask, their counts
 // (seeded in full) and their grains at bin_start[bin].
 template <int KIND>
-__global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                               const uint32_t *bin_start,
-                                                               const unsigned long long *bin_mask, float4 *grains)
+__device__ __forceinline__ void bin_grains_body(const PixelArgs &a, const PixGeom &g, const double *T,
+                                                const uint32_t *bin_start, const unsigned long long *bin_mask,
+                                                float4 *grains, int bin, uint16_t *list, uint32_t *pre, BinRecs &rec)
 {
-    constexpr int NW = kBlock / 64;
-    __shared__ uint16_t s_list[NW][256];
-    __shared__ uint32_t s_pre[NW][256];
-    __shared__ BinRecs s_rec[NW];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * NW + wid;
-    if (bin >= g.nbx * g.nby) return;
-    if (a.grain_total && *a.grain_total > a.grain_cap) return;
+    const int lane = threadIdx.x & 63;
     unsigned long long m[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[q] = bin_mask[(size_t)bin * 4 + q];
-    const int n = compact_cells(m, s_list[wid]);
+    const int n = compact_cells(m, list);
     int ci0, cj0;
     bin_origin(g, bin, ci0, cj0);
     // counts from full seeds (the records of bin_counts' rewrite)
@@ -496,21 +494,102 @@ __global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixG
         uint32_t c = 0;
         const int idx = base + lane;
         if (idx < n) {
-            const int cell = s_list[wid][idx];
+            const int cell = list[idx];
             const int l = cell & 63, q = cell >> 6;
             const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
             float sx, sy;
             const double e = cell_threshold_at(a, T, i, j, sx, sy);
             Xoshiro rng{};
             c = cell_count_full(mix3(a.seed, kCellStream, i, j), e, rng);
-            put_u64x2(s_rec[wid][0][idx], rng.s0, rng.s1);
-            put_u64x2(s_rec[wid][1][idx], rng.s2, rng.s3);
+            put_u64x2(rec[0][idx], rng.s0, rng.s1);
+            put_u64x2(rec[1][idx], rng.s2, rng.s3);
         }
         const uint32_t incl = wave_inclusive_sum(c);
-        if (idx < n) s_pre[wid][idx] = carry + incl - c;
+        if (idx < n) pre[idx] = carry + incl - c;
         carry += __shfl(incl, 63, 64);
     }
-    bin_draws<KIND>(a, g, ci0, cj0, s_list[wid], n, s_rec[wid], s_pre[wid], carry, bin_start[bin], grains);
+    bin_draws<KIND>(a, g, ci0, cj0, list, n, rec, pre, carry, bin_start[bin], grains);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock, 4) void bin_grains_kernel(PixelArgs a, PixGeom g, const double *T,
+                                                               const uint32_t *bin_start,
+                                                               const unsigned long long *bin_mask, float4 *grains)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ uint32_t s_pre[NW][256];
+    __shared__ BinRecs s_rec[NW];
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (a.grain_total && *a.grain_total > a.grain_cap) return;
+    const int b0 = (blockIdx.x * NW + wid) * kDrawChunk, b1 = min(g.nbx * g.nby, b0 + kDrawChunk);
+#pragma unroll 1
+    for (int bin = b0; bin < b1; ++bin)
+        bin_grains_body<KIND>(a, g, T, bin_start, bin_mask, grains, bin, s_list[wid], s_pre[wid], s_rec[wid]);
+}
+
+// The batch's generation kernels run over all its planes in one launch each:
+// a workgroup or wave finds its plane (or group of planes) from its index in
+// the launch by a binary search over first-index offsets.  The offsets and
+// plane records are read through the constant address space: the index is
+// uniform, so these are scalar loads into SGPRs.
+template <class T>
+__device__ __forceinline__ T cload(const T *p)
+{
+    return *(const T *)((const __attribute__((address_space(4))) T *)p);
+}
+
+__device__ __forceinline__ const BatchPlane &cplane(const BatchPlane *planes, int p)
+{
+    return *(const BatchPlane *)((const __attribute__((address_space(4))) BatchPlane *)planes + p);
+}
+
+// The run i with off[i] <= x < off[i + 1], for 0 <= x < off[n] (empty runs
+// are passed over).
+__device__ __forceinline__ int find_run(const long long *off, int n, long long x)
+{
+    int lo = 0, hi = n;  // off[lo] <= x < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cload(off + mid) <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_threshold_kernel(BatchGen bg)
+{
+    const long long blk = blockIdx.x;
+    const int p = find_run(bg.thr_off, bg.n, blk);
+    const BatchPlane &bp = cplane(bg.planes, p);
+    const long long i = (blk - cload(bg.thr_off + p)) * kBlock + threadIdx.x;
+    threshold_body(bp.a.lambda, (long long)bp.a.in_w * bp.a.in_h, bp.a.delta, cload(bg.T + p), bp.a.status, i);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock, 4) void batch_bin_grains_kernel(BatchGen bg)
+{
+    constexpr int NW = kBlock / 64;
+    __shared__ uint16_t s_list[NW][256];
+    __shared__ uint32_t s_pre[NW][256];
+    __shared__ BinRecs s_rec[NW];
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const long long gc = (long long)blockIdx.x * NW + wid;
+    const long long *off = bg.kind_off[KIND];
+    const int nk = bg.n_kind[KIND];
+    if (gc >= cload(off + nk)) return;
+    const int i = find_run(off, nk, gc);
+    const int p = cload(bg.kind_plane[KIND] + i);
+    const BatchPlane &bp = cplane(bg.planes, p);
+    if (bp.a.grain_total && *bp.a.grain_total > bp.a.grain_cap) return;
+    const int b0 = (int)(gc - cload(off + i)) * kDrawChunk;
+    const int b1 = min(bp.geo.nbx * bp.geo.nby, b0 + kDrawChunk);
+    const double *T = cload(bg.T + p);
+    const unsigned long long *mask = bg.mask + 4 * cload(bg.bin_off + p);
+#pragma unroll 1
+    for (int bin = b0; bin < b1; ++bin)
+        bin_grains_body<KIND>(bp.a, bp.geo, T, bp.bin_start, mask, bg.grains, bin, s_list[wid], s_pre[wid],
+                              s_rec[wid]);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,11 +606,10 @@ __device__ __forceinline__ float sample_frac(float X)
 
 // Rank sort of the samples by (fraction, index): thread i writes sample i to
 // its rank.  O(N^2) compares, N <= 65535, run once per render.
-__global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs, int n, float zoom,
-                                                             SampleTable t)
+__device__ __forceinline__ void sample_sort_body(const float2 *offs, int n, float zoom, const SampleTable &t,
+                                                 int blk, float *s_key)
 {
-    __shared__ float s_key[kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blk * kBlock + threadIdx.x;
     float2 oi = make_float2(0.0f, 0.0f);
     float f = 0.0f;
     if (i < n) {
@@ -558,11 +636,17 @@ __global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs,
     }
 }
 
+__global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs, int n, float zoom,
+                                                             SampleTable t)
+{
+    __shared__ float s_key[kBlock];
+    sample_sort_body(offs, n, zoom, t, blockIdx.x, s_key);
+}
+
 // Bucket index (first sorted position with fraction >= q / Q, by binary
 // search) and the extents of the scaled offsets; one workgroup.
-__global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable t)
+__device__ __forceinline__ void sample_index_body(int n, const SampleTable &t, float4 *s_r)
 {
-    __shared__ float4 s_r[kBlock];
     const int Q = t.qbuckets;
     for (int q = threadIdx.x; q <= Q; q += kBlock) {
         const float v = (float)q / (float)Q;  // exact: Q is a power of two
@@ -599,6 +683,12 @@ __global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable
         __syncthreads();
     }
     if (threadIdx.x == 0) *const_cast<float4 *>(t.ext) = s_r[0];
+}
+
+__global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable t)
+{
+    __shared__ float4 s_r[kBlock];
+    sample_index_body(n, t, s_r);
 }
 
 // The table of the first n samples from the table of all of them: the full
@@ -712,18 +802,25 @@ __host__ __device__ inline size_t splat_list_base(int k, int q, int e, int n)
     return b;
 }
 
-__global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q, int e, uint32_t *fill, uint2 *cell,
-                                                          uint16_t *list, int *status)
+struct ListsShared {
+    float2 f[2048];
+    uint32_t scan[256];
+    uint32_t base;
+    bool fits;
+};
+
+__device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, int e, uint32_t *fill, uint2 *cell,
+                                                 uint16_t *list, int *status, int blk, ListsShared &sh)
 {
     // 64 cells per block, each split over 4 adjacent threads that take the
     // four quarters of the sample order: the scan over the block in thread
     // order keeps every list in sorted order
-    __shared__ float2 s_f[2048];
-    __shared__ uint32_t s_scan[256];
-    __shared__ uint32_t s_base;
-    __shared__ bool s_fits;
+    float2 *s_f = sh.f;
+    uint32_t *s_scan = sh.scan;
+    uint32_t &s_base = sh.base;
+    bool &s_fits = sh.fits;
     const int cells = q * q, parts = (cells + 63) / 64, N = st.n;
-    const int k = blockIdx.x / parts, c = (blockIdx.x - k * parts) * 64 + ((int)threadIdx.x >> 2);
+    const int k = blk / parts, c = (blk - k * parts) * 64 + ((int)threadIdx.x >> 2);
     const int quarter = threadIdx.x & 3, qn = (N + 3) / 4;
     const int p_lo = quarter * qn, p_hi = min(N, p_lo + qn);
     for (int p = threadIdx.x; p < N; p += 256) {
@@ -774,6 +871,46 @@ __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q,
         for (int p = p_lo; p < p_hi; ++p)
             if (member(p)) list[pos++] = (uint16_t)p;
     }
+}
+
+__global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q, int e, uint32_t *fill, uint2 *cell,
+                                                          uint16_t *list, int *status)
+{
+    __shared__ ListsShared sh;
+    splat_lists_body(st, q, e, fill, cell, list, status, blockIdx.x, sh);
+}
+
+// Batch sample tables: workgroup (x, p) sorts samples x * kBlock.. of plane p.
+__global__ __launch_bounds__(kBlock) void batch_sample_sort_kernel(BatchGen bg)
+{
+    __shared__ float s_key[kBlock];
+    const BatchPlane &bp = cplane(bg.planes, blockIdx.y);
+    if ((int)blockIdx.x * kBlock >= bp.a.n_samples) return;
+    sample_sort_body(bp.a.offs, bp.a.n_samples, bp.a.zoom, bp.st, blockIdx.x, s_key);
+}
+
+// One workgroup per plane: the table's index, and the candidate lists'
+// counters cleared for batch_splat_lists_kernel.
+__global__ __launch_bounds__(kBlock) void batch_sample_index_kernel(BatchGen bg)
+{
+    __shared__ float4 s_r[kBlock];
+    const int p = blockIdx.x;
+    const BatchPlane &bp = cplane(bg.planes, p);
+    sample_index_body(bp.a.n_samples, bp.st, s_r);
+    const int classes = cload(bg.list_plan + p).y;
+    uint32_t *fill = cload(bg.list_fill + p);
+    for (int k = threadIdx.x; k < classes; k += kBlock) fill[k] = 0u;
+}
+
+__global__ __launch_bounds__(256) void batch_splat_lists_kernel(BatchGen bg)
+{
+    __shared__ ListsShared sh;
+    const long long blk = blockIdx.x;
+    const int p = find_run(bg.list_off, bg.n, blk);
+    const BatchPlane &bp = cplane(bg.planes, p);
+    const int4 lp = cload(bg.list_plan + p);
+    splat_lists_body(bp.st, lp.x, lp.z, cload(bg.list_fill + p), const_cast<uint2 *>(bp.st.cl_cell),
+                     const_cast<uint16_t *>(bp.st.cl_list), bp.a.status, (int)(blk - cload(bg.list_off + p)), sh);
 }
 
 __global__ void store_table_kernel(SampleTable t, SampleTable *dst)
@@ -1415,16 +1552,17 @@ hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s)
     return hipGetLastError();
 }
 
-static dim3 bin_grid(const PixGeom &g)
+// chunk bins a wave
+static dim3 bin_grid(const PixGeom &g, int chunk)
 {
-    const long long nbins = (long long)g.nbx * g.nby;
-    return dim3((unsigned)((nbins + kBlock / 64 - 1) / (kBlock / 64)));
+    const long long waves = ((long long)g.nbx * g.nby + chunk - 1) / chunk;
+    return dim3((unsigned)((waves + kBlock / 64 - 1) / (kBlock / 64)));
 }
 
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
                             unsigned long long *bin_mask, hipStream_t s)
 {
-    hipLaunchKernelGGL(bin_count_kernel, bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_count, bin_mask);
+    hipLaunchKernelGGL(bin_count_kernel, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, T, bin_count, bin_mask);
     return hipGetLastError();
 }
 
@@ -1432,9 +1570,9 @@ hipError_t launch_bin_count_planes(const PixelArgs &a, const PixGeom &g, const F
                                    hipStream_t s)
 {
     switch (n_planes) {
-    case 1: hipLaunchKernelGGL(bin_count_planes_kernel<1>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
-    case 2: hipLaunchKernelGGL(bin_count_planes_kernel<2>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
-    case 3: hipLaunchKernelGGL(bin_count_planes_kernel<3>, bin_grid(g), dim3(kBlock), 0, s, a, g, fp); break;
+    case 1: hipLaunchKernelGGL(bin_count_planes_kernel<1>, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, fp); break;
+    case 2: hipLaunchKernelGGL(bin_count_planes_kernel<2>, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, fp); break;
+    case 3: hipLaunchKernelGGL(bin_count_planes_kernel<3>, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, fp); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1450,11 +1588,43 @@ hipError_t launch_bin_grains(const PixelArgs &a, const PixGeom &g, int dist_kind
                              hipStream_t s)
 {
     if (dist_kind == 0)
-        hipLaunchKernelGGL((bin_grains_kernel<0>), bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
+        hipLaunchKernelGGL((bin_grains_kernel<0>), bin_grid(g, kDrawChunk), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
                            grains);
     else
-        hipLaunchKernelGGL((bin_grains_kernel<1>), bin_grid(g), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
+        hipLaunchKernelGGL((bin_grains_kernel<1>), bin_grid(g, kDrawChunk), dim3(kBlock), 0, s, a, g, T, bin_start, bin_mask,
                            grains);
+    return hipGetLastError();
+}
+
+static dim3 wave_grid(long long waves) { return dim3((unsigned)((waves + kBlock / 64 - 1) / (kBlock / 64))); }
+
+hipError_t launch_batch_threshold(const BatchGen &bg, hipStream_t s)
+{
+    if (bg.thr_blocks <= 0 || bg.thr_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_threshold_kernel, dim3((unsigned)bg.thr_blocks), dim3(kBlock), 0, s, bg);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_bin_grains(const BatchGen &bg, hipStream_t s)
+{
+    if (bg.kind_chunks[0] > 0)
+        hipLaunchKernelGGL(batch_bin_grains_kernel<0>, wave_grid(bg.kind_chunks[0]), dim3(kBlock), 0, s, bg);
+    if (bg.kind_chunks[1] > 0)
+        hipLaunchKernelGGL(batch_bin_grains_kernel<1>, wave_grid(bg.kind_chunks[1]), dim3(kBlock), 0, s, bg);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sample_tables(const BatchGen &bg, hipStream_t s)
+{
+    if (bg.n <= 0 || bg.n > 65535 || bg.sort_blocks <= 0 || bg.list_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_sample_sort_kernel, dim3((unsigned)bg.sort_blocks, (unsigned)bg.n), dim3(kBlock), 0, s,
+                       bg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_sample_index_kernel, dim3((unsigned)bg.n), dim3(kBlock), 0, s, bg);
+    e = hipGetLastError();
+    if (e != hipSuccess || bg.list_blocks == 0) return e;
+    hipLaunchKernelGGL(batch_splat_lists_kernel, dim3((unsigned)bg.list_blocks), dim3(256), 0, s, bg);
     return hipGetLastError();
 }
 
