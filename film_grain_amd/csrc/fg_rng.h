@@ -125,10 +125,18 @@ struct Xoshiro {
         s2 = splitmix64(h + 2 * kGolden);
         s3 = splitmix64(h + 3 * kGolden);
         if ((s0 | s1 | s2 | s3) == 0) {
-            s0 = splitmix64(0);
-            s1 = splitmix64(kGolden);
-            s2 = splitmix64(2 * kGolden);
-            s3 = splitmix64(3 * kGolden);
+            uint64_t z = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+            // z stays opaque: folded to constants, the reseed's four words
+            // were hoisted out of this branch (taken with probability
+            // 2^-256) and spilled to scratch by every wave of the count
+            // kernels
+            asm volatile("" : "+v"(z));
+#endif
+            s0 = splitmix64(z);
+            s1 = splitmix64(z + kGolden);
+            s2 = splitmix64(z + 2 * kGolden);
+            s3 = splitmix64(z + 3 * kGolden);
         }
     }
 
