@@ -1448,8 +1448,7 @@ hipError_t batch_enqueue_counts(fg_batch *b, hipStream_t s)
             fp.mask[k] = b->mask + 4 * b->bin_off[p];
         }
         const fg::BatchPlane &bp = b->planes[p0];
-        e = np == 1 ? fg::launch_bin_count(bp.a, bp.geo, fp.T[0], fp.cnt[0], fp.mask[0], s)
-                    : fg::launch_bin_count_planes(bp.a, bp.geo, fp, np, s);
+        e = fg::launch_bin_count_planes(bp.a, bp.geo, fp, np, s);
     }
     if (e == hipSuccess) e = fg::launch_sum_u64(b->cnt, b->total_bins, b->total64, s);
     if (e == hipSuccess) e = fg::launch_scan_u32(b->cnt, b->start, b->total_bins, b->partials, s);

@@ -145,90 +145,6 @@ __device__ __forceinline__ void put_u64x2(uint4 &r, uint64_t x, uint64_t y)
 __device__ __forceinline__ uint64_t lo64(const uint4 &r) { return (uint64_t)r.x | (uint64_t)r.y << 32; }
 __device__ __forceinline__ uint64_t hi64(const uint4 &r) { return (uint64_t)r.z | (uint64_t)r.w << 32; }
 
-// The empty-cell test of the bin's 256 cells: nonempty masks m (bit l of word
-// q for cell (l, q)) and the records of the nonempty cells.  A cell is empty
-// when its Poisson count is 0 by the Knuth branch's first draw
-// (u1 <= exp(-lambda)), which needs only the two seed words that draw
-// depends on; the rejection branch (e < 0) and the all-zero seed case
-// (reseeded by seed_from_u64) are counted in full.
-__device__ __forceinline__ void bin_test(const PixelArgs &a, const PixGeom &g, const double *T, int ci0, int cj0,
-                                         BinRecs &rec, unsigned long long m[4])
-{
-    const int lane = threadIdx.x & 63;
-    const int ci = ci0 + (lane & 15), cjl = cj0 + 4 * (lane >> 4);
-    const uint64_t col = mix_col(a.seed, g.Ix0 + ci);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int cj = cjl + q;
-        bool ne = false;
-        if (ci < g.W && cj < g.H) {
-            float sx, sy;
-            const double e = cell_threshold_at(a, T, g.Ix0 + ci, g.Jy0 + cj, sx, sy);
-            const uint64_t h = mix_row(col, g.Jy0 + cj);
-            uint64_t s0 = 0, s3 = 0;
-            if (e <= 1.5) {
-                if (e < 0.0) {
-                    ne = true;
-                } else {
-                    s0 = splitmix64(h);
-                    s3 = splitmix64(h + 3 * kGolden);
-                    ne = (s0 | s3) == 0 || standard_f64(rotl64(s0 + s3, 23) + s0) > e;
-                }
-            }
-            if (ne) {
-                const int slot = q << 6 | lane;
-                put_u64x2(rec[0][slot], h, s0);
-                put_u64x2(rec[1][slot], s3, (uint64_t)__double_as_longlong(e));
-            }
-        }
-        m[q] = __ballot(ne);
-    }
-}
-
-// Counts of the n compacted nonempty cells from their test records (the
-// count only adds seed words s1 and s2), each cell's stream after the count
-// rewritten into its record at its compacted index and, with pre set, the
-// counts' exclusive prefix in pre[idx].  Returns the bin's total in 64 bits
-// (one cell's count alone can reach 2^32 - 1).
-__device__ __forceinline__ unsigned long long bin_counts(const uint16_t *list, int n, BinRecs &rec, uint32_t *pre)
-{
-    const int lane = threadIdx.x & 63;
-    unsigned long long total = 0;
-    for (int base = 0; base < n; base += 64) {
-        const int idx = base + lane;
-        uint32_t c = 0;
-        Xoshiro rng{};
-        if (idx < n) {
-            const int slot = list[idx];
-            const uint4 r0 = rec[0][slot], r1 = rec[1][slot];
-            const uint64_t h = lo64(r0);
-            const double e = __longlong_as_double((long long)hi64(r1));
-            if (e < 0.0) {
-                c = cell_count_full(h, e, rng);
-            } else {
-                rng.s0 = hi64(r0);
-                rng.s1 = splitmix64(h + kGolden);
-                rng.s2 = splitmix64(h + 2 * kGolden);
-                rng.s3 = lo64(r1);
-                if ((rng.s0 | rng.s1 | rng.s2 | rng.s3) == 0) rng.seed(h);  // seed_from_u64's reseed
-                double p = standard_f64(rng.next());
-                while (p > e) {
-                    p = p * standard_f64(rng.next());
-                    ++c;
-                }
-            }
-            put_u64x2(rec[0][idx], rng.s0, rng.s1);
-            put_u64x2(rec[1][idx], rng.s2, rng.s3);
-        }
-        const uint32_t incl = wave_inclusive_sum(c);
-        if (pre && idx < n) pre[idx] = (uint32_t)total + incl - c;
-        unsigned long long x = c;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        total += x;
-    }
-    return total;
-}
 
 // The grain draws of the bin's n compacted cells, whose streams are in rec
 // and whose counts' exclusive prefix is in pre (total: the bin's sum): cell
@@ -287,33 +203,6 @@ __device__ __forceinline__ void bin_draws(const PixelArgs &a, const PixGeom &g, 
     }
 }
 
-// Pass 1 over the extended grid: per bin (one wave), the empty-cell test of
-// its 256 cells, the nonempty mask, and the grain count.
-__global__ __launch_bounds__(kBlock, 4) void bin_count_kernel(PixelArgs a, PixGeom g, const double *T,
-                                                              uint32_t *bin_count, unsigned long long *bin_mask)
-{
-    constexpr int NW = kBlock / 64;
-    __shared__ uint16_t s_list[NW][256];
-    __shared__ BinRecs s_rec[NW];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int bin = blockIdx.x * NW + wid;
-    if (bin >= g.nbx * g.nby) return;
-    int ci0, cj0;
-    bin_origin(g, bin, ci0, cj0);
-    unsigned long long m[4];
-    bin_test(a, g, T, ci0, cj0, s_rec[wid], m);
-    const int n = compact_cells(m, s_list[wid]);
-    const unsigned long long total = bin_counts(s_list[wid], n, s_rec[wid], nullptr);
-    if (lane == 0) {
-        // a bin of 2^32 or more grains saturates (the plane's 64-bit sum of
-        // the bin counts, sum_bins_u64, then reaches 2^32 - 1 as well, and the
-        // host refuses the plane before any kernel uses the u32 scan)
-        bin_count[bin] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
-        if (total > 0xFFFFFFFFull) atomicOr(a.status, kStatusOverflow);
-    }
-    if (lane < 4) bin_mask[(size_t)bin * 4 + lane] = lane == 0 ? m[0] : (lane == 1 ? m[1] : (lane == 2 ? m[2] : m[3]));
-}
-
 // Count of a rejection-branch cell (lambda * delta^2 >= 12) from its seeded
 // stream, saturated as cell_count_full does.  Kept out of line: few cells take
 // it, and inlined its f64 transcendentals would hold registers across the
@@ -324,14 +213,23 @@ __device__ __noinline__ uint32_t rejection_count(Xoshiro rng, double lambda)
     return !(r > 0.0) ? 0u : (r >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)r);
 }
 
-// Pass 1 for the planes of one frame (NP <= kFramePlanes).  The reference
+// Pass 1 over the extended cell grid, per bin (one wave): each of its 256
+// cells' empty-cell test, the nonempty masks and the bin's grain count, for
+// the NP <= kFramePlanes planes of one frame (NP = 1 for a single plane).
+// A cell is empty when its Poisson count is 0 by the Knuth branch's first
+// draw (u1 <= exp(-lambda)), which needs only the two seed words that draw
+// depends on, s0 and s3; the rejection branch (e < 0) and the all-zero seed
+// case (reseeded by seed_from_u64) are counted in full.  The reference
 // renders every plane with the same seed, and a cell's stream depends only
 // on the seed and the cell (src/rng.rs:26-29, src/lib.rs:146-153), so the
 // planes share each cell's hash, seed words and Knuth uniforms; only their
-// thresholds differ.  The hashes and the empty-cell test's two seed words
-// are computed once per cell, and each plane's count of a cell runs on the
-// one uniform sequence against its own threshold.  The counts and masks
-// equal NP runs of bin_count_kernel.
+// thresholds differ.  The hashes and the test's two seed words are computed
+// once per cell and kept in an LDS record for the nonempty ones, and each
+// plane's count of a cell runs on the one uniform sequence against its own
+// threshold.  A bin of 2^32 or more grains saturates its count and sets
+// kStatusOverflow (the plane's 64-bit sum of the bin counts, sum_bins_u64,
+// then reaches 2^32 - 1 as well, and the host refuses the plane before any
+// kernel uses the u32 scan).
 template <int NP>
 __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a, PixGeom g, FramePlanes fp)
 {
@@ -372,8 +270,8 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
             bool any = false;
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                // as bin_test: the rejection branch and the all-zero seed
-                // case are counted in full
+                // the rejection branch and the all-zero seed case are
+                // counted in full
                 ne[p] = e[p] <= 1.5 && (e[p] < 0.0 || (s0 | s3) == 0 || u1 > e[p]);
                 any |= ne[p];
             }
@@ -488,7 +386,7 @@ __device__ __forceinline__ void bin_grains_body(const PixelArgs &a, const PixGeo
     const int n = compact_cells(m, list);
     int ci0, cj0;
     bin_origin(g, bin, ci0, cj0);
-    // counts from full seeds (the records of bin_counts' rewrite)
+    // counts from full seeds
     uint32_t carry = 0;
     for (int base = 0; base < n; base += 64) {
         uint32_t c = 0;
@@ -1562,7 +1460,11 @@ static dim3 bin_grid(const PixGeom &g, int chunk)
 hipError_t launch_bin_count(const PixelArgs &a, const PixGeom &g, const double *T, uint32_t *bin_count,
                             unsigned long long *bin_mask, hipStream_t s)
 {
-    hipLaunchKernelGGL(bin_count_kernel, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, T, bin_count, bin_mask);
+    FramePlanes fp{};
+    fp.T[0] = T;
+    fp.cnt[0] = bin_count;
+    fp.mask[0] = bin_mask;
+    hipLaunchKernelGGL(bin_count_planes_kernel<1>, bin_grid(g, 1), dim3(kBlock), 0, s, a, g, fp);
     return hipGetLastError();
 }
 

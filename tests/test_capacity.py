@@ -1,6 +1,6 @@
 """The grain-capacity limit of the pixel-wise HBM path, on the CPU.
 
-The count pass sums each bin's grains in 64 bits (bin_count_kernel) and the
+The count pass sums each bin's grains in 64 bits (bin_count_planes_kernel) and the
 host refuses a plane whose total exceeds fg_max_grains() before bin_grains
 writes anything, because bin offsets, grain indices and the splat's chunk
 indices are u32.  These tests check that limit and show, with the oracle's own
