@@ -1483,6 +1483,8 @@ int batch_gen_setup(fg_batch *b, const char *what, hipStream_t s)
         if (bytes) std::memcpy(host.data() + off, src, bytes);
         fixups.push_back({off, field});
     };
+    // the sample tables' launch has a workgroup row per plane
+    if (n > 65535) return set_error(b->ctx, FG_EINVAL, "a batch holds at most 65535 planes");
     fg::BatchGen &g = b->gen;
     g = fg::BatchGen{};
     g.n = n;

@@ -248,14 +248,17 @@ int fg_get_timings(fg_ctx *ctx, float ms[3]);
 /* ---- Batched rendering -------------------------------------------------
  * No counterpart in the reference, which renders one plane per call
  * (src/color.rs:47-64 loops over planes; src/lib.rs:151-165 dispatches each).
- * A batch is a fixed set of pixel-wise planes of any sizes and parameters
- * (N <= 2048 each), with their device buffers.  fg_batch_create plans it:
- * per-plane geometry and launch shapes, buffers sized from one counting pass
- * over the current inputs (grain capacity = 1.25 x that count + 65536), and,
- * with capture_graph set, one HIP graph of the whole pipeline: per-plane grain
- * generation, one scan over all planes' bins, per-plane sample tables, a
- * heaviest-first order of all (plane, tile) items, and one persistent splat
- * launch whose workgroups take items from an atomic counter.  fg_batch_run
+ * A batch is a fixed set of 1 to 65535 pixel-wise planes of any sizes and
+ * parameters (N <= 2048 each), with their device buffers.  fg_batch_create
+ * plans it: per-plane geometry and launch shapes, buffers sized from one
+ * counting pass over the current inputs (grain capacity = 1.25 x that count +
+ * 65536), and, with capture_graph set, one HIP graph of the whole pipeline:
+ * the grain generation (thresholds, draws and sample tables each one launch
+ * over all planes; counts one launch per image, the consecutive planes with
+ * one description and offset range counted together), one scan over all
+ * planes' bins, a heaviest-first order of all (plane, tile) items, and one
+ * persistent splat launch whose workgroups take items from an atomic
+ * counter.  fg_batch_run
  * enqueues the pipeline (or replays the graph) on `hip_stream`; the inputs may
  * change between runs, and a run whose grains exceed the capacity renders
  * nothing and makes fg_batch_status return FG_ECAPACITY (create the batch
