@@ -978,35 +978,112 @@ int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float 
         const int rc = pixel_args(ctx, d, lams[i], d_offs, outs[i], tw, th, lds, ctx->d_range, ctx->d_status, &a[i]);
         if (rc != FG_OK) return rc;
     }
-    // the previous call's kernels may still read the counts
+    // The frame's pipeline: thresholds and the joint count, every plane's
+    // scan, one host read of all the grain totals (their sizes), then each
+    // plane's draws and tile order, one sample table and its candidate lists
+    // for all the planes (they share the offsets), and the splats.  Every
+    // plane has its own bin starts, grains and tile order, so nothing waits
+    // for a plane's splat; the previous call's kernels may still read the
+    // context's buffers, hence the one acquire.
     int rc = scratch_acquire(ctx, what);
     if (rc != FG_OK) return rc;
+    const int kind = (int)d->dist_kind, block = splat_hbm_block(ctx, d);
     const long long nbins = (long long)geom.nbx * geom.nby;
     const size_t t_b = align256((size_t)d->in_w * d->in_h * sizeof(double));
     const size_t c_b = align256((size_t)(nbins + 1) * 4), m_b = align256((size_t)nbins * 32);
-    if ((rc = ensure_bytes(ctx, ctx->plane_counts, (size_t)n * (t_b + c_b + m_b), what)) != FG_OK) return rc;
+    const size_t per_plane = t_b + 2 * c_b + m_b;  // thresholds | counts | starts | masks
+    if ((rc = ensure_bytes(ctx, ctx->plane_counts, 256 + (size_t)n * per_plane, what)) != FG_OK) return rc;
     unsigned char *base = static_cast<unsigned char *>(ctx->plane_counts.p);
+    unsigned long long *d_tot = reinterpret_cast<unsigned long long *>(base);  // the planes' grain totals
+    uint32_t *start[fg::kFramePlanes];
     PreCounts pre[fg::kFramePlanes];
     fg::FramePlanes fp{};
-    hipError_t e = hipSuccess;
     for (int i = 0; i < n; ++i) {
-        unsigned char *b = base + (size_t)i * (t_b + c_b + m_b);
+        unsigned char *b = base + 256 + (size_t)i * per_plane;
         pre[i].T = reinterpret_cast<double *>(b);
         pre[i].cnt = reinterpret_cast<uint32_t *>(b + t_b);
-        pre[i].mask = reinterpret_cast<unsigned long long *>(b + t_b + c_b);
+        start[i] = reinterpret_cast<uint32_t *>(b + t_b + c_b);
+        pre[i].mask = reinterpret_cast<unsigned long long *>(b + t_b + 2 * c_b);
         fp.T[i] = pre[i].T;
         fp.cnt[i] = pre[i].cnt;
         fp.mask[i] = pre[i].mask;
-        if (e == hipSuccess) e = fg::launch_threshold(a[i], pre[i].T, s);
     }
-    if (e == hipSuccess) e = fg::launch_bin_count_planes(a[0], geom, fp, n, s);
+    // the rest of the scratch, sized before any kernel of this call runs
+    // (growing a buffer frees the old one)
+    size_t scan_bytes = 0, sort_bytes = 0;
+    hipError_t e = fg::scan_bins(nullptr, &scan_bytes, nullptr, nullptr, (int)(nbins + 1), s);
+    if (e == hipSuccess) e = fg::sort_tiles(nullptr, &sort_bytes, nullptr, nullptr, nullptr, nullptr, a[0].n_tiles, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    for (int i = 0; i < n; ++i) {
-        rc = run_pixel_hbm(ctx, a[i], geom, range, (int)d->dist_kind, splat_hbm_block(ctx, d), s, ScratchSource(),
-                           &pre[i]);
-        if (rc != FG_OK) return rc;
+    const int q = sample_buckets((uint32_t)d->n_samples);
+    fg::SampleTable st{};
+    splat_bounds(a[0], geom, range, &st.eps, &st.safe_r);
+    const ListPlan lp = list_plan(a[0], st.eps);
+    const size_t lists_total = lp.bytes((int)d->n_samples);
+    const bool lists = lists_total > 0 && lists_total < ((size_t)1 << 32);
+    const size_t tiles_b = align256((size_t)a[0].n_tiles * 16);
+    if ((rc = ensure_bytes(ctx, ctx->scan_temp, scan_bytes, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->tile_temp, sort_bytes, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->tiles, (size_t)n * tiles_b, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes((int)d->n_samples, q), what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->levels, 256, what)) != FG_OK) return rc;
+    if (lists && (rc = ensure_bytes(ctx, ctx->cells, lists_total, what)) != FG_OK) return rc;
+    mark(ctx, 0, s);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = fg::launch_threshold(a[i], pre[i].T, s);
+    if (e == hipSuccess) e = fg::launch_bin_count_planes(a[0], geom, fp, n, s);
+    // totals (cleared by a kernel, as the single-plane path does) and scans
+    if (e == hipSuccess) e = fg::launch_fill_u32(reinterpret_cast<uint32_t *>(d_tot), 2 * n, 0u, s);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        e = fg::launch_fill_u32(pre[i].cnt + nbins, 1, 0u, s);
+        if (e == hipSuccess) e = fg::launch_sum_u64(pre[i].cnt, nbins, d_tot + i, s);
+        if (e == hipSuccess)
+            e = fg::scan_bins(ctx->scan_temp.p, &scan_bytes, pre[i].cnt, start[i], (int)(nbins + 1), s);
     }
-    return FG_OK;
+    unsigned long long total[fg::kFramePlanes] = {};
+    if (e == hipSuccess) e = hipMemcpyAsync(total, d_tot, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    // the grain array of all the planes (nothing of this call runs now)
+    size_t goff[fg::kFramePlanes], gsum = 0;
+    for (int i = 0; i < n; ++i) {
+        if ((rc = check_total(ctx, total[i], what)) != FG_OK) return rc;
+        goff[i] = gsum;
+        gsum += (size_t)total[i];
+    }
+    if ((rc = ensure_bytes(ctx, ctx->grains, std::max<size_t>(gsum, 1) * sizeof(float4), what)) != FG_OK) return rc;
+    float4 *grains = static_cast<float4 *>(ctx->grains.p);
+    // one sample table and its candidate lists, which the splat reads from
+    // device memory
+    e = fg::launch_sample_table(a[0].offs, a[0].n_samples, a[0].zoom, q, ctx->samples.p, &st, s);
+    if (e == hipSuccess && lists) e = build_lists(lp, st, static_cast<unsigned char *>(ctx->cells.p), a[0].status, s);
+    fg::SampleTable *d_table = static_cast<fg::SampleTable *>(ctx->levels.p);
+    if (e == hipSuccess) e = fg::launch_store_table(st, d_table, s);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        e = fg::launch_bin_grains(a[i], geom, kind, pre[i].T, start[i], pre[i].mask, grains + goff[i], s);
+        unsigned char *tp = static_cast<unsigned char *>(ctx->tiles.p) + (size_t)i * tiles_b;
+        TileOrder order;
+        order.w = reinterpret_cast<uint32_t *>(tp);
+        order.w_sorted = reinterpret_cast<uint32_t *>(tp + (size_t)a[i].n_tiles * 4);
+        order.ids = reinterpret_cast<int *>(tp + (size_t)a[i].n_tiles * 8);
+        order.order = reinterpret_cast<int *>(tp + (size_t)a[i].n_tiles * 12);
+        if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a[i], geom, start[i], order.w, order.ids, s);
+        if (e == hipSuccess)
+            e = fg::sort_tiles(ctx->tile_temp.p, &sort_bytes, order.w, order.w_sorted, order.ids, order.order,
+                               a[i].n_tiles, s);
+        a[i].tile_list = order.order;
+        a[i].level_tables = d_table;
+        if (lists) {
+            a[i].cl_q = lp.q;
+            a[i].cl_classes = lp.classes;
+            a[i].cl_inv_dz = (float)(2 * lp.q);
+        }
+    }
+    mark(ctx, 1, s);
+    mark(ctx, 2, s);
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+        e = fg::launch_splat_hbm(a[i], geom, st, start[i], grains + goff[i], block, s);
+    mark(ctx, 3, s);
+    if (e != hipSuccess) return hip_error(ctx, e, what);
+    return scratch_release(ctx, s, what);
 }
 
 // Grain-wise formulation: 1 the reference's loop order over all (grain,
