@@ -1010,9 +1010,8 @@ int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float 
     }
     // the rest of the scratch, sized before any kernel of this call runs
     // (growing a buffer frees the old one)
-    size_t scan_bytes = 0, sort_bytes = 0;
+    size_t scan_bytes = 0;
     hipError_t e = fg::scan_bins(nullptr, &scan_bytes, nullptr, nullptr, (int)(nbins + 1), s);
-    if (e == hipSuccess) e = fg::sort_tiles(nullptr, &sort_bytes, nullptr, nullptr, nullptr, nullptr, a[0].n_tiles, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     const int q = sample_buckets((uint32_t)d->n_samples);
     fg::SampleTable st{};
@@ -1020,10 +1019,13 @@ int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float 
     const ListPlan lp = list_plan(a[0], st.eps);
     const size_t lists_total = lp.bytes((int)d->n_samples);
     const bool lists = lists_total > 0 && lists_total < ((size_t)1 << 32);
-    const size_t tiles_b = align256((size_t)a[0].n_tiles * 16);
+    // the persistent splat's plane records, (plane, tile) items, their
+    // weights and the item counter
+    const int n_items = n * a[0].n_tiles;
+    const size_t rec_b = align256(sizeof(fg::BatchPlane) * fg::kFramePlanes);
+    const size_t items_b = align256((size_t)n_items * sizeof(int2)), w_b = align256((size_t)n_items * 4);
     if ((rc = ensure_bytes(ctx, ctx->scan_temp, scan_bytes, what)) != FG_OK) return rc;
-    if ((rc = ensure_bytes(ctx, ctx->tile_temp, sort_bytes, what)) != FG_OK) return rc;
-    if ((rc = ensure_bytes(ctx, ctx->tiles, (size_t)n * tiles_b, what)) != FG_OK) return rc;
+    if ((rc = ensure_bytes(ctx, ctx->tiles, rec_b + items_b + w_b + 256, what)) != FG_OK) return rc;
     if ((rc = ensure_bytes(ctx, ctx->samples, fg::sample_table_bytes((int)d->n_samples, q), what)) != FG_OK) return rc;
     if ((rc = ensure_bytes(ctx, ctx->levels, 256, what)) != FG_OK) return rc;
     if (lists && (rc = ensure_bytes(ctx, ctx->cells, lists_total, what)) != FG_OK) return rc;
@@ -1057,30 +1059,37 @@ int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float 
     if (e == hipSuccess && lists) e = build_lists(lp, st, static_cast<unsigned char *>(ctx->cells.p), a[0].status, s);
     fg::SampleTable *d_table = static_cast<fg::SampleTable *>(ctx->levels.p);
     if (e == hipSuccess) e = fg::launch_store_table(st, d_table, s);
+    fg::FramePlaneRecords rec{};
     for (int i = 0; i < n && e == hipSuccess; ++i) {
         e = fg::launch_bin_grains(a[i], geom, kind, pre[i].T, start[i], pre[i].mask, grains + goff[i], s);
-        unsigned char *tp = static_cast<unsigned char *>(ctx->tiles.p) + (size_t)i * tiles_b;
-        TileOrder order;
-        order.w = reinterpret_cast<uint32_t *>(tp);
-        order.w_sorted = reinterpret_cast<uint32_t *>(tp + (size_t)a[i].n_tiles * 4);
-        order.ids = reinterpret_cast<int *>(tp + (size_t)a[i].n_tiles * 8);
-        order.order = reinterpret_cast<int *>(tp + (size_t)a[i].n_tiles * 12);
-        if (e == hipSuccess) e = fg::launch_pixel_tile_weights(a[i], geom, start[i], order.w, order.ids, s);
-        if (e == hipSuccess)
-            e = fg::sort_tiles(ctx->tile_temp.p, &sort_bytes, order.w, order.w_sorted, order.ids, order.order,
-                               a[i].n_tiles, s);
-        a[i].tile_list = order.order;
-        a[i].level_tables = d_table;
         if (lists) {
             a[i].cl_q = lp.q;
             a[i].cl_classes = lp.classes;
             a[i].cl_inv_dz = (float)(2 * lp.q);
         }
+        fg::BatchPlane &bp = rec.p[i];
+        bp.a = a[i];
+        bp.geo = geom;
+        bp.st = st;
+        bp.bin_start = start[i];
+        bp.grains = grains + goff[i];
+        bp.item_base = (long long)i * a[i].n_tiles;
     }
+    // the three splats as one persistent launch over the frame's (plane,
+    // tile) items, heaviest first
+    unsigned char *tb = static_cast<unsigned char *>(ctx->tiles.p);
+    fg::BatchPlane *d_planes = reinterpret_cast<fg::BatchPlane *>(tb);
+    int2 *items = reinterpret_cast<int2 *>(tb + rec_b);
+    uint32_t *w = reinterpret_cast<uint32_t *>(tb + rec_b + items_b);
+    uint32_t *counter = reinterpret_cast<uint32_t *>(tb + rec_b + items_b + w_b);
+    if (e == hipSuccess) e = fg::launch_store_frame_planes(rec, n, d_planes, s);
+    if (e == hipSuccess) e = fg::launch_batch_weights(d_planes, n, w, n_items, s);
+    if (e == hipSuccess) e = fg::launch_batch_order(d_planes, n, w, n_items, items, nullptr, s);
+    if (e == hipSuccess) e = fg::launch_fill_u32(counter, 1, 0u, s);
+    const int grid = std::min(n_items, ctx->num_cus * fg::splat_batch_blocks_per_cu(block, a[0].lds_bytes));
     mark(ctx, 1, s);
     mark(ctx, 2, s);
-    for (int i = 0; i < n && e == hipSuccess; ++i)
-        e = fg::launch_splat_hbm(a[i], geom, st, start[i], grains + goff[i], block, s);
+    if (e == hipSuccess) e = fg::launch_splat_batch(d_planes, items, n_items, counter, grid, block, a[0].lds_bytes, s);
     mark(ctx, 3, s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
     return scratch_release(ctx, s, what);

@@ -171,6 +171,12 @@ hipError_t launch_tile_levels(const PixelArgs &a, const int *level_n, int n_leve
                               hipStream_t s);
 // Pointers of a sample table laid out in buf (as launch_sample_table does).
 void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t);
+// The plane records of one frame, written to device memory by a kernel
+// (they travel as its argument: no host buffer has to outlive a copy).
+struct FramePlaneRecords {
+    BatchPlane p[kFramePlanes];
+};
+hipError_t launch_store_frame_planes(const FramePlaneRecords &r, int n, BatchPlane *dst, hipStream_t s);
 // Resident workgroups of the persistent splat kernel per CU.
 int splat_batch_blocks_per_cu(int block, int lds_bytes);
 // Persistent splat over (plane, tile) items in launch order (items[i] =

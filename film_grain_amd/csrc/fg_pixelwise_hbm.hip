@@ -1402,6 +1402,18 @@ __global__ __launch_bounds__(BLOCK, 6) void splat_batch_kernel(const BatchPlane 
     }
 }
 
+__global__ void store_frame_planes_kernel(FramePlaneRecords r, int n, BatchPlane *dst)
+{
+    if (blockIdx.x == 0 && (int)threadIdx.x < n) dst[threadIdx.x] = r.p[threadIdx.x];
+}
+
+hipError_t launch_store_frame_planes(const FramePlaneRecords &r, int n, BatchPlane *dst, hipStream_t s)
+{
+    if (n < 1 || n > kFramePlanes) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(store_frame_planes_kernel, dim3(1), dim3(64), 0, s, r, n, dst);
+    return hipGetLastError();
+}
+
 int splat_batch_blocks_per_cu(int block, int lds_bytes)
 {
     int nb = 0;
