@@ -100,7 +100,8 @@ def summarize(trace: str, fetch: str | None, write: str | None, copy_mib: int, k
 
 
 def markdown(s: dict, title: str, commands: list[str], notes: list[str]) -> str:
-    lines = [f"# {title}", "", "Commands (run from the repository root on one MI355X):", "", "```"]
+    lines = [f"# {title}", "", "Commands (one MI355X; `R` is the repository root, `S` any scratch directory):", "",
+             "```"]
     lines += commands
     lines += ["```", ""]
     if s["fetch_factor"] is not None and s["write_factor"] is not None:
