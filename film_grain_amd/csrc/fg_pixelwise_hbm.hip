@@ -1199,11 +1199,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             const int excl = (int)(incl - fl);
             const int dl = p0 - excl;
             const float rs2 = rad * rad;
-            // the chunk's grains (centre, r^2) in this wave's LDS records: a
-            // step reads its owner's with one load instead of broadcasting
-            // three values (the wave's LDS accesses run in order, so no
-            // barrier is needed)
-            s_grec[wid][lane] = make_float4(G.x, G.y, rs2, 0.0f);
+            // the chunk's grains (target origin, centre) in this wave's LDS
+            // records: a step reads its owner's with one load instead of
+            // broadcasting four values, and its r^2 with one lane read (the
+            // wave's LDS accesses run in order, so no barrier is needed)
+            s_grec[wid][lane] = make_float4(bx, by, G.x, G.y);
             int carry = 0;  // the last grain that starts before the step
             // the owner of each lane's entry of the step from s0, and the
             // entry's sorted position (lanes past the stream read its last
@@ -1242,11 +1242,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     const int s0 = sb + 64 * i;
                     if (s0 >= total) break;
                     const int p = pn[i];
-                    // the owner grain's centre and r^2 from its record; its
-                    // target origin is the same f32 expression as bx, by above
+                    // the owner grain's target origin and centre from its
+                    // record, its r^2 from its lane
                     const float4 gr = s_grec[wid][own[i]];
-                    const float ocx = gr.x, ocy = gr.y, ors2 = gr.z;
-                    const float obx = ocx * z - 0.5f, oby = ocy * z - 0.5f;
+                    const float obx = gr.x, oby = gr.y, ocx = gr.z, ocy = gr.w;
+                    const float ors2 = from_lane_f(rs2, own[i] << 2);
                     const float4 XO = XO_at(p);
                     const uint2 wm = word_bit(p);
                     // both reads issue before the test's branches (as in the
@@ -1362,7 +1362,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void splat_hbm_kernel(PixelArgs a, PixGeom geo, SampleTable st,
+__global__ __launch_bounds__(BLOCK, 6) void splat_hbm_kernel(PixelArgs a, PixGeom geo, SampleTable st,
                                                           const uint32_t *bin_start, const float4 *grains)
 {
     extern __shared__ __align__(16) unsigned char lds[];
