@@ -983,6 +983,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     // read with two loads, made the splat 9% slower.)
     const int tile_words = a.tile_w * a.tile_h * words;
     const uint32_t w4 = (uint32_t)words * 4u;
+    const uint32_t row_b = (uint32_t)t.TW * w4;  // bitset bytes of a tile row
     size_t off = 0;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
     off += align16((size_t)tile_words * 4);
@@ -1059,7 +1060,9 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             if (rs > 0.0f ||
                 (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
                  floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)))
-                atomicOr(reinterpret_cast<uint32_t *>(lds + ((uint32_t)(ty * t.TW + tx) * w4 + wm.x)), wm.y);
+                atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24((uint32_t)ty, row_b) +
+                                                             __umul24((uint32_t)tx, w4) + wm.x)),
+                         wm.y);
         }
     };
 
@@ -1093,9 +1096,11 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             const float yg = (py + 0.5f) * iz - XO.w;
             const float dx = xg - gcx_, dy = yg - gcy_;
             if (dx * dx + dy * dy <= rs2) {
-                // tile coordinates and the bitset's bytes are far below 2^24
-                const uint32_t px_i = __umul24((uint32_t)ty, (uint32_t)t.TW) + (uint32_t)tx;
-                atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24(px_i, w4) + wm.x)), wm.y);
+                // tile coordinates and the bitset's bytes are far below 2^24:
+                // two 24-bit products (full rate; the compiler otherwise
+                // forms the pixel index with a quarter-rate 64-bit mad)
+                const uint32_t off = __umul24((uint32_t)ty, row_b) + __umul24((uint32_t)tx, w4) + wm.x;
+                atomicOr(reinterpret_cast<uint32_t *>(lds + off), wm.y);
             }
         }
     };
