@@ -1226,11 +1226,13 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 }
                 return owner;
             };
-            // (unsigned: list positions are non-negative, so the load takes
-            // a 32-bit offset from the list's base)
+            // (a 32-bit byte offset from the list's base: the load takes the
+            // base in SGPRs and the offset in one VGPR, with no 64-bit
+            // address arithmetic)
             auto entry = [&](int s0, int owner) {
                 const uint32_t pos = (uint32_t)from_lane_i(dl, owner << 2) + (uint32_t)min(s0 + lane, total - 1);
-                return (int)st.cl_list[pos];
+                const unsigned char *lb = reinterpret_cast<const unsigned char *>(st.cl_list);
+                return (int)*reinterpret_cast<const uint16_t *>(lb + (pos << 1));
             };
             // kWalkStage steps at a time: their owners first, then all their
             // list loads in flight together (steps past the stream read its
