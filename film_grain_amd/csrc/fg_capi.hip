@@ -1086,7 +1086,8 @@ int do_pixelwise_planes(fg_ctx *ctx, const fg_plane_desc *d, int n, const float 
     if (e == hipSuccess) e = fg::launch_batch_weights(d_planes, n, w, n_items, s);
     if (e == hipSuccess) e = fg::launch_batch_order(d_planes, n, w, n_items, items, nullptr, s);
     if (e == hipSuccess) e = fg::launch_fill_u32(counter, 1, 0u, s);
-    const int grid = std::min(n_items, ctx->num_cus * fg::splat_batch_blocks_per_cu(block, a[0].lds_bytes));
+    const int grid =
+        std::max(1, std::min(n_items, ctx->num_cus * fg::splat_batch_blocks_per_cu(block, a[0].lds_bytes)));
     mark(ctx, 1, s);
     mark(ctx, 2, s);
     if (e == hipSuccess) e = fg::launch_splat_batch(d_planes, items, n_items, counter, grid, block, a[0].lds_bytes, s);
