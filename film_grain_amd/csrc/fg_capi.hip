@@ -1775,8 +1775,9 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
     e = hipMemcpyAsync(b->d_planes, b->planes.data(), sizeof(fg::BatchPlane) * n, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, what);
-    b->grid = (int)std::min<long long>(std::max(b->n_items, 1),
-                                       (long long)ctx->num_cus * fg::splat_batch_blocks_per_cu(b->block, b->lds));
+    b->grid = (int)std::max<long long>(
+        1, std::min<long long>(std::max(b->n_items, 1),
+                               (long long)ctx->num_cus * fg::splat_batch_blocks_per_cu(b->block, b->lds)));
     if (capture) {
         e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
