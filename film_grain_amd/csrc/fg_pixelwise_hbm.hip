@@ -230,12 +230,22 @@ __device__ __noinline__ uint32_t rejection_count(Xoshiro rng, double lambda)
 // kStatusOverflow (the plane's 64-bit sum of the bin counts, sum_bins_u64,
 // then reaches 2^32 - 1 as well, and the host refuses the plane before any
 // kernel uses the u32 scan).
+// A nonempty cell's record in the count pass: its hash and seed words s0
+// (16 B) and s3 (8 B); 24 B a cell keep a workgroup's LDS under 32 KB.
+struct CountRecs {
+    uint4 hs0[256];
+    uint2 s3[256];
+};
+
+// One plane runs five waves per SIMD (100 VGPRs, 26 KB of LDS a
+// workgroup); more planes need the registers of four.
 template <int NP>
-__global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a, PixGeom g, FramePlanes fp)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NP == 1 ? 5 : 4, 8))) void
+bin_count_planes_kernel(PixelArgs a, PixGeom g, FramePlanes fp)
 {
     constexpr int NW = kBlock / 64;
     __shared__ uint16_t s_list[NW][256];
-    __shared__ BinRecs s_rec[NW];
+    __shared__ CountRecs s_rec[NW];
     __shared__ unsigned long long s_mask[NW][NP + 1][4];  // the planes' nonempty masks, then their union
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int bin = blockIdx.x * NW + wid;
@@ -277,8 +287,8 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
             }
             if (any) {
                 const int slot = q << 6 | lane;
-                put_u64x2(s_rec[wid][0][slot], h, s0);
-                put_u64x2(s_rec[wid][1][slot], s3, 0ull);
+                put_u64x2(s_rec[wid].hs0[slot], h, s0);
+                s_rec[wid].s3[slot] = make_uint2((uint32_t)s3, (uint32_t)(s3 >> 32));
             }
         }
         unsigned long long any = 0ull;
@@ -304,7 +314,8 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
         if (base + lane < n) {
             const int slot = s_list[wid][base + lane];
             const int l = slot & 63, q = slot >> 6;
-            const uint4 r0 = s_rec[wid][0][slot], r1 = s_rec[wid][1][slot];
+            const uint4 r0 = s_rec[wid].hs0[slot];
+            const uint2 r1 = s_rec[wid].s3[slot];
             const uint64_t h = lo64(r0);
             const int i = g.Ix0 + ci0 + (l & 15), j = g.Jy0 + cj0 + 4 * (l >> 4) + q;
             double e[NP];
@@ -320,7 +331,7 @@ __global__ __launch_bounds__(kBlock, 4) void bin_count_planes_kernel(PixelArgs a
             seeded.s0 = hi64(r0);
             seeded.s1 = splitmix64(h + kGolden);
             seeded.s2 = splitmix64(h + 2 * kGolden);
-            seeded.s3 = lo64(r1);
+            seeded.s3 = (uint64_t)r1.x | (uint64_t)r1.y << 32;
             // a cell counted in full for the rejection branch alone has no
             // recorded seed words
             if (!knuth || (seeded.s0 | seeded.s1 | seeded.s2 | seeded.s3) == 0) seeded.seed(h);
