@@ -38,6 +38,7 @@ struct SampleTable {
                            // lies in that point's cell window
     // the splat's candidate lists over this order (launch_splat_lists), or
     // nullptr: (start, length) per (radius class, cell), and the entries
+    // (sorted position * 16)
     const uint2 *cl_cell;
     const uint16_t *cl_list;
 };

@@ -696,6 +696,8 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
 // 1 / (2 q) a window is (k + 2 + (2 eps + 2^-11) q) / q wide, so a sample
 // lies in at most splat_list_span(k, q, e) cells per axis when e >=
 // (2 eps + 2^-11) q, and class k needs at most n * span^2 list entries.
+// An entry is its sample's sorted position p times 16, the byte offset of
+// the sample's record in the splat's LDS (n <= 2048 keeps it in 16 bits).
 // Blocks of 64 cells of one class, four threads per cell; both passes test
 // every sample, and each block reserves its cells' room in the class's
 // region with one atomic.
@@ -778,7 +780,7 @@ __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, i
             cell[(size_t)k * cells + c] = make_uint2(pos, len);
         }
         for (int p = p_lo; p < p_hi; ++p)
-            if (member(p)) list[pos++] = (uint16_t)p;
+            if (member(p)) list[pos++] = (uint16_t)(p << 4);
     }
 }
 
@@ -985,26 +987,29 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const int bx0 = reg.bx0, bx1 = reg.bx1, by0 = reg.by0, by1 = reg.by1;
     const int rci0 = reg.ci0, rci1 = reg.ci1, rcj0 = reg.cj0, rcj1 = reg.cj1;
 
-    // ---- LDS: bitset | (o, word, bit)[N] | qidx[Q + 1] ----
+    // ---- LDS: (o, word, bit)[N] | qidx[Q + 1] | bitset ----
     // Sample k's bit of tile pixel i is bit k % 32 of bitset word i * words +
     // k / 32.  Per sorted position one 16-byte record holds the sample's
-    // offset in input pixels, that word's byte offset for pixel 0 and the
+    // offset in input pixels, that word's LDS byte offset for pixel 0 and the
     // bit's mask: a test reads it with one load, and a hit's address is one
     // multiply-add.  (Records of 24 bytes that also held the scaled offset,
     // read with two loads, made the splat 9% slower.)
     const int tile_words = a.tile_w * a.tile_h * words;
     const uint32_t w4 = (uint32_t)words * 4u;
     const uint32_t row_b = (uint32_t)t.TW * w4;  // bitset bytes of a tile row
+    // per sorted position one 16-byte record: the offset in input pixels,
+    // the byte offset in the workgroup's LDS of the bitset word of its sample
+    // and the bit's mask; the scaled offset X = O * zoom is recomputed (the
+    // table's own f32 product).  First in LDS: at a fixed address, a record
+    // is read at its byte offset alone (the candidate lists hold p * 16).
     size_t off = 0;
-    uint32_t *bits = reinterpret_cast<uint32_t *>(lds);
-    off += align16((size_t)tile_words * 4);
-    // per sorted position one 16-byte record: the offset in input pixels and
-    // the bitset word's byte offset and bit mask; the scaled offset X = O *
-    // zoom is recomputed (the table's own f32 product)
-    float4 *srec = reinterpret_cast<float4 *>(lds + off);
+    float4 *srec = reinterpret_cast<float4 *>(lds);
     off += align16((size_t)N * 16);
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)(Q + 1) * 2);
+    const uint32_t bits_off = (uint32_t)off;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lds + off);
+    off += align16((size_t)tile_words * 4);
     if (off > (size_t)a.lds_bytes) {
         if (tid == 0) atomicOr(a.status, kStatusRegion);
         return;
@@ -1013,7 +1018,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     for (int k = tid; k < N; k += BLOCK) {
         const float2 O = st.o[k];
         const uint32_t ki = st.k[k];
-        srec[k] = make_float4(O.x, O.y, __uint_as_float((ki >> 5) * 4u), __uint_as_float(1u << (ki & 31u)));
+        srec[k] = make_float4(O.x, O.y, __uint_as_float(bits_off + (ki >> 5) * 4u), __uint_as_float(1u << (ki & 31u)));
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     // (X, O) of sorted position p: the scaled offset and the offset in
@@ -1021,6 +1026,10 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     auto XO_at = [&](int p) {
         const float4 r = srec[p];
         return make_float4(r.x * z, r.y * z, r.x, r.y);
+    };
+    // the record at byte offset pb (= 16 p)
+    auto rec_at = [&](uint32_t pb) {
+        return *reinterpret_cast<const float4 *>(lds + pb);
     };
     // the (word byte offset, bit mask) pair of sorted position p
     auto word_bit = [&](int p) {
@@ -1243,14 +1252,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             auto entry = [&](int s0, int owner) {
                 const uint32_t pos = (uint32_t)from_lane_i(dl, owner << 2) + (uint32_t)min(s0 + lane, total - 1);
                 const unsigned char *lb = reinterpret_cast<const unsigned char *>(st.cl_list);
-                return (int)*reinterpret_cast<const uint16_t *>(lb + (pos << 1));
+                return (uint32_t)*reinterpret_cast<const uint16_t *>(lb + (pos << 1));
             };
             // kWalkStage steps at a time: their owners first, then all their
             // list loads in flight together (steps past the stream read its
             // last entry and test nothing), then their tests: one global
             // latency per batch of steps instead of one per step
             for (int sb = 0; sb < total; sb += 64 * kWalkStage) {
-                int own[kWalkStage], pn[kWalkStage];
+                int own[kWalkStage];
+                uint32_t pn[kWalkStage];
 #pragma unroll
                 for (int i = 0; i < kWalkStage; ++i) own[i] = owner_of(sb + 64 * i);
 #pragma unroll
@@ -1259,14 +1269,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 for (int i = 0; i < kWalkStage; ++i) {
                     const int s0 = sb + 64 * i;
                     if (s0 >= total) break;
-                    const int p = pn[i];
                     // the owner grain's target origin and centre from its
-                    // record, its r^2 from its lane
+                    // record, its r^2 from its lane; the entry is the
+                    // sample record's byte offset
                     const float4 gr = s_grec[wid][own[i]];
                     const float obx = gr.x, oby = gr.y, ocx = gr.z, ocy = gr.w;
                     const float ors2 = from_lane_f(rs2, own[i] << 2);
-                    const float4 XO = XO_at(p);
-                    const uint2 wm = word_bit(p);
+                    const float4 r = rec_at(pn[i]);
+                    const float4 XO = make_float4(r.x * z, r.y * z, r.x, r.y);
+                    const uint2 wm = make_uint2(__float_as_uint(r.z), __float_as_uint(r.w));
                     // both reads issue before the test's branches (as in the
                     // per-grain walk)
                     asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
