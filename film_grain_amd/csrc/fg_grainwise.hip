@@ -331,6 +331,12 @@ hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, 
 // four per CU).
 constexpr int kOwnerMap = 256;
 
+// A wave-uniform float moved to an SGPR.
+__device__ __forceinline__ float uniform_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Entry pos of a candidate list: a 32-bit byte offset from the list's base,
 // so that the load takes the base in SGPRs and the offset in one VGPR (no
 // 64-bit address arithmetic in the walk's steps: 19.55 -> 19.28 ms at C3).
@@ -358,7 +364,10 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
     const int tyi = tile / a.tiles_x, txi = tile - tyi * a.tiles_x;
     const int X0 = txi * a.tile_w, Y0 = tyi * a.tile_h;
     const int X1 = min(X0 + a.tile_w, a.out_w), Y1 = min(Y0 + a.tile_h, a.out_h);
-    const float fx0 = (float)X0, fx1 = (float)(X1 - 1), fy0 = (float)Y0, fy1 = (float)(Y1 - 1);
+    // per-tile floats held in SGPRs (this target has no scalar float ALU, so
+    // they are computed in VGPRs; kept there all kernel long, they spilled)
+    const float fx0 = uniform_f((float)X0), fx1 = uniform_f((float)(X1 - 1));
+    const float fy0 = uniform_f((float)Y0), fy1 = uniform_f((float)(Y1 - 1));
     const int words = a.words, N = a.n_samples, Q = st.qbuckets;
     const int tile_words = a.tile_w * a.tile_h * words;
 
@@ -383,7 +392,8 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     if (tid == 0) s_next = 0;
-    const float4 ext = *st.ext;
+    const float4 ext0 = *st.ext;
+    const float4 ext = make_float4(uniform_f(ext0.x), uniform_f(ext0.y), uniform_f(ext0.z), uniform_f(ext0.w));
 
     const float z = a.zoom, Rmax = a.r_out_max;
     const PixRegion reg = grain_tile_region(a, ext, X0, Y0, X1, Y1);
@@ -394,8 +404,8 @@ __global__ __launch_bounds__(BLOCK, 8) void grainwise_sorted(GrainArgs a, Sample
     // C in magnitude, so each of its f32 steps is off by at most C * 2^-24
     const float C = (float)max(a.out_w, a.out_h) + fmaxf(fmaxf(fabsf(ext.x), fabsf(ext.y)),
                                                          fmaxf(fabsf(ext.z), fabsf(ext.w))) + Rmax + 8.0f;
-    const float eps = C * 9.5367431640625e-07f + 1.52587890625e-05f;  // C * 2^-20 + 2^-16
-    const float Qf = (float)Q;
+    const float eps = uniform_f(C * 9.5367431640625e-07f + 1.52587890625e-05f);  // C * 2^-20 + 2^-16
+    const float Qf = uniform_f((float)Q);
     uint32_t *queue = s_queue[wid];
     int qcount = 0;
     // the lane's current grain: target origin and output radius
