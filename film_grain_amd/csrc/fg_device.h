@@ -203,6 +203,12 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x)
 // v of lane l (a per-lane index, 0 <= l < 64); ba = 4 l, the byte address
 // ds_bpermute takes
 __device__ __forceinline__ int from_lane_i(int v, int ba) { return __builtin_amdgcn_ds_bpermute(ba, v); }
+// A wave-uniform float moved to an SGPR (no scalar float ALU on this target:
+// a float computed from uniform values lands in a VGPR and holds it).
+__device__ __forceinline__ float uniform_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 __device__ __forceinline__ float from_lane_f(float v, int ba)
 {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(ba, __float_as_int(v)));

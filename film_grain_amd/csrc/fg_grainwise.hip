@@ -331,12 +331,6 @@ hipError_t launch_grain_tile_weights(const GrainArgs &a, const SampleTable &st, 
 // four per CU).
 constexpr int kOwnerMap = 256;
 
-// A wave-uniform float moved to an SGPR.
-__device__ __forceinline__ float uniform_f(float v)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
 // Entry pos of a candidate list: a 32-bit byte offset from the list's base,
 // so that the load takes the base in SGPRs and the offset in one VGPR (no
 // 64-bit address arithmetic in the walk's steps: 19.55 -> 19.28 ms at C3).
