@@ -934,6 +934,27 @@ __device__ __forceinline__ float lane_f(float v, int l)
 }
 __device__ __forceinline__ int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// LDS locations as 32-bit LDS addresses: a bitset word's address held in a
+// sample record is used as it is, with no workgroup base added per hit
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_addr(uint32_t *p)
+{
+    return (uint32_t)(uintptr_t)(lds_u32 *)p;
+}
+__device__ __forceinline__ void lds_or(uint32_t addr, uint32_t v)
+{
+    __hip_atomic_fetch_or((lds_u32 *)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// a * b + c for a, b < 2^24 with one full-rate v_mad_u32_u24 (b uniform).
+// Written out because the compiler turns two such terms into two products
+// and a three-way add.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+
 // One output tile of one plane: the body of the single-plane kernel and of the
 // persistent batch kernel (which therefore give identical planes).  lds is the
 // workgroup's dynamic LDS (a.lds_bytes), s_queue its per-wave queues.  Starts
@@ -998,16 +1019,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     const uint32_t w4 = (uint32_t)words * 4u;
     const uint32_t row_b = (uint32_t)t.TW * w4;  // bitset bytes of a tile row
     // per sorted position one 16-byte record: the offset in input pixels,
-    // the byte offset in the workgroup's LDS of the bitset word of its sample
-    // and the bit's mask; the scaled offset X = O * zoom is recomputed (the
-    // table's own f32 product).  First in LDS: at a fixed address, a record
-    // is read at its byte offset alone (the candidate lists hold p * 16).
+    // the LDS address of the bitset word of its sample for pixel 0 and the
+    // bit's mask; the scaled offset X = O * zoom is recomputed (the table's
+    // own f32 product).  First in LDS: at a fixed address, a record is read
+    // at its byte offset alone (the candidate lists hold p * 16).
     size_t off = 0;
     float4 *srec = reinterpret_cast<float4 *>(lds);
     off += align16((size_t)N * 16);
     uint16_t *sq = reinterpret_cast<uint16_t *>(lds + off);
     off += align16((size_t)(Q + 1) * 2);
-    const uint32_t bits_off = (uint32_t)off;
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds + off);
     off += align16((size_t)tile_words * 4);
     if (off > (size_t)a.lds_bytes) {
@@ -1018,7 +1038,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     for (int k = tid; k < N; k += BLOCK) {
         const float2 O = st.o[k];
         const uint32_t ki = st.k[k];
-        srec[k] = make_float4(O.x, O.y, __uint_as_float(bits_off + (ki >> 5) * 4u), __uint_as_float(1u << (ki & 31u)));
+        srec[k] = make_float4(O.x, O.y, __uint_as_float(lds_addr(bits + (ki >> 5))), __uint_as_float(1u << (ki & 31u)));
     }
     for (int q = tid; q <= Q; q += BLOCK) sq[q] = st.qidx[q];
     // (X, O) of sorted position p: the scaled offset and the offset in
@@ -1045,6 +1065,9 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
 
     const float slack = 4e-3f + 2e-6f * (float)max(a.out_w, a.out_h);
     const float eps = st.eps, safe_r = st.safe_r, Qf = (float)Q;
+    // A pixel's point is (x + 0.5) * iz - O.  For integral x, x + 0.5 is exact,
+    // so fma(x, iz, iz / 2) rounds the same exact product once: the same f32.
+    const float hiz = 0.5f * iz;
     uint32_t *queue = s_queue[wid];
     int qcount = 0;
 
@@ -1071,8 +1094,8 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         const float4 XO = XO_at(p);
         const float2 O = make_float2(XO.z, XO.w);
         const uint2 wm = word_bit(p);
-        const float xg = ((float)(t.X0 + tx) + 0.5f) * iz - O.x;
-        const float yg = ((float)(t.Y0 + ty) + 0.5f) * iz - O.y;
+        const float xg = __builtin_fmaf((float)(t.X0 + tx), iz, hiz) - O.x;
+        const float yg = __builtin_fmaf((float)(t.Y0 + ty), iz, hiz) - O.y;
         const float dx = xg - cx, dy = yg - cy;
         const float r = fabsf(rs);
         if (dx * dx + dy * dy <= r * r) {
@@ -1080,9 +1103,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             if (rs > 0.0f ||
                 (floor_div(xg - rm, delta, rd, fast) <= gi && gi <= floor_div(xg + rm, delta, rd, fast) &&
                  floor_div(yg - rm, delta, rd, fast) <= gj && gj <= floor_div(yg + rm, delta, rd, fast)))
-                atomicOr(reinterpret_cast<uint32_t *>(lds + (__umul24((uint32_t)ty, row_b) +
-                                                             __umul24((uint32_t)tx, w4) + wm.x)),
-                         wm.y);
+                lds_or(mad_u24((uint32_t)ty, row_b, mad_u24((uint32_t)tx, w4, wm.x)), wm.y);
         }
     };
 
@@ -1100,28 +1121,30 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
     };
 
     // The direct test of a narrow grain whose disk test implies its cell
-    // window (rs > 0) at the sample with offsets XO and bitset word/bit wm.
-    // A pixel whose point passes the disk test lies within rz of the
-    // sample's target (bx + X.x, by + X.y) in each axis (the candidate box
-    // is a superset); with rz < 0.5 that is only the pixel nearest the
-    // target.  So the drain's test (same f32 expression, rs2 = r * r) of
-    // that one pixel, when it lies in the tile, sets exactly the bits the
-    // queue would have set.  The grain's values may differ between lanes.
+    // window (rs > 0) at the sample with record r (offset O, bitset word
+    // address and bit).  A pixel whose point passes the disk test lies
+    // within rz of the sample's target (bx + X.x, by + X.y), X = O * zoom, in
+    // each axis (the candidate box is a superset); with rz < 0.5 that is
+    // only the pixel nearest the target.  So the drain's test (same f32
+    // expression, rs2 = r * r) of that one pixel, when it lies in the tile,
+    // sets exactly the bits the queue would have set.  The target is taken
+    // with one rounding, fma(O, zoom, bx): it moves by far less than the
+    // slack in rz, so the nearest pixel is the same whenever one can pass.
+    // The grain's values may differ between lanes.
     const int tw = t.X1 - t.X0, th = t.Y1 - t.Y0;
-    auto test = [&](bool ok, float4 XO, uint2 wm, float gbx, float gby, float gcx_, float gcy_, float rs2) {
-        const float px = rintf(gbx + XO.x), py = rintf(gby + XO.y);
+    auto test = [&](bool ok, float4 r, float gbx, float gby, float gcx_, float gcy_, float rs2) {
+        const float px = rintf(__builtin_fmaf(r.x, z, gbx)), py = rintf(__builtin_fmaf(r.y, z, gby));
         const int tx = (int)px - t.X0, ty = (int)py - t.Y0;
         if (ok & ((unsigned)tx < (unsigned)tw) & ((unsigned)ty < (unsigned)th)) {
-            const float xg = (px + 0.5f) * iz - XO.z;
-            const float yg = (py + 0.5f) * iz - XO.w;
+            const float xg = __builtin_fmaf(px, iz, hiz) - r.x;
+            const float yg = __builtin_fmaf(py, iz, hiz) - r.y;
             const float dx = xg - gcx_, dy = yg - gcy_;
-            if (dx * dx + dy * dy <= rs2) {
+            if (dx * dx + dy * dy <= rs2)
                 // tile coordinates and the bitset's bytes are far below 2^24:
-                // two 24-bit products (full rate; the compiler otherwise
+                // two 24-bit multiply-adds (full rate; the compiler otherwise
                 // forms the pixel index with a quarter-rate 64-bit mad)
-                const uint32_t off = __umul24((uint32_t)ty, row_b) + __umul24((uint32_t)tx, w4) + wm.x;
-                atomicOr(reinterpret_cast<uint32_t *>(lds + off), wm.y);
-            }
+                lds_or(mad_u24((uint32_t)ty, row_b, mad_u24((uint32_t)tx, w4, __float_as_uint(r.z))),
+                       __float_as_uint(r.w));
         }
     };
 
@@ -1222,17 +1245,25 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
             const uint32_t incl = wave_inclusive_sum(fl);
             const int total = lane_i((int)incl, 63);
             const int excl = (int)(incl - fl);
-            const int dl = p0 - excl;
+            // Entry e of the stream is list entry dl + e of its owner grain,
+            // dl = p0 - excl >= -excl > -total.  A step from s loads entry
+            // s + min(lane, total - 1 - s), and the steps of a batch that
+            // start past the stream have s < total + 64 (kWalkStage - 1), so
+            // biased by bias = total + 64 kWalkStage, the lane's part
+            // dlb + min(lane, total - 1 - s) is positive: a valid unsigned
+            // 32-bit offset from the batch's base.
+            const int bias = total + 64 * kWalkStage;
+            const int dlb = p0 - excl + bias;
             const float rs2 = rad * rad;
             // the chunk's grains (target origin, centre) in this wave's LDS
             // records: a step reads its owner's with one load instead of
             // broadcasting four values, and its r^2 with one lane read (the
             // wave's LDS accesses run in order, so no barrier is needed)
             s_grec[wid][lane] = make_float4(bx, by, G.x, G.y);
-            int carry = 0;  // the last grain that starts before the step
-            // the owner of each lane's entry of the step from s0, and the
-            // entry's sorted position (lanes past the stream read its last
-            // entry and test nothing)
+            const unsigned char *grec = reinterpret_cast<const unsigned char *>(&s_grec[wid][0]);
+            int carry = 0;  // 4 x the last grain that starts before the step
+            // 4 x the owner of each lane's entry of the step from s0 (a lane
+            // read's byte address)
             auto owner_of = [&](int s0) {
                 const int e = s0 + lane;
                 int owner = carry;
@@ -1240,31 +1271,34 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     const int g = __ffsll((long long)lrem) - 1;
                     const int ex = lane_i(excl, g);
                     if (ex >= s0 + 64) break;
-                    owner = e >= ex ? g : owner;
-                    carry = g;
+                    owner = e >= ex ? (g << 2) : owner;
+                    carry = g << 2;
                     lrem &= lrem - 1ull;
                 }
                 return owner;
             };
-            // (a 32-bit byte offset from the list's base: the load takes the
-            // base in SGPRs and the offset in one VGPR, with no 64-bit
-            // address arithmetic)
-            auto entry = [&](int s0, int owner) {
-                const uint32_t pos = (uint32_t)from_lane_i(dl, owner << 2) + (uint32_t)min(s0 + lane, total - 1);
-                const unsigned char *lb = reinterpret_cast<const unsigned char *>(st.cl_list);
-                return (uint32_t)*reinterpret_cast<const uint16_t *>(lb + (pos << 1));
+            // Lanes past the stream read its last entry (the stream's last
+            // pair again: its bit is ORed twice, which changes nothing).  The
+            // load takes a base in SGPRs (the batch's first entry, less the
+            // bias), a 32-bit byte offset in one VGPR and the step's 128
+            // bytes as its immediate offset: no 64-bit address arithmetic.
+            auto entry = [&](const unsigned char *lbb, int i, int s0, int own4) {
+                const uint32_t pos = (uint32_t)from_lane_i(dlb, own4) + (uint32_t)min(lane, total - 1 - s0);
+                return (uint32_t)*reinterpret_cast<const uint16_t *>(lbb + 128 * i + (pos << 1));
             };
             // kWalkStage steps at a time: their owners first, then all their
             // list loads in flight together (steps past the stream read its
             // last entry and test nothing), then their tests: one global
             // latency per batch of steps instead of one per step
             for (int sb = 0; sb < total; sb += 64 * kWalkStage) {
+                const unsigned char *lbb =
+                    reinterpret_cast<const unsigned char *>(st.cl_list) + 2 * ((long long)sb - bias);
                 int own[kWalkStage];
                 uint32_t pn[kWalkStage];
 #pragma unroll
                 for (int i = 0; i < kWalkStage; ++i) own[i] = owner_of(sb + 64 * i);
 #pragma unroll
-                for (int i = 0; i < kWalkStage; ++i) pn[i] = entry(sb + 64 * i, own[i]);
+                for (int i = 0; i < kWalkStage; ++i) pn[i] = entry(lbb, i, sb + 64 * i, own[i]);
 #pragma unroll
                 for (int i = 0; i < kWalkStage; ++i) {
                     const int s0 = sb + 64 * i;
@@ -1272,16 +1306,13 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                     // the owner grain's target origin and centre from its
                     // record, its r^2 from its lane; the entry is the
                     // sample record's byte offset
-                    const float4 gr = s_grec[wid][own[i]];
-                    const float obx = gr.x, oby = gr.y, ocx = gr.z, ocy = gr.w;
-                    const float ors2 = from_lane_f(rs2, own[i] << 2);
+                    const float4 gr = *reinterpret_cast<const float4 *>(grec + (own[i] << 2));
+                    const float ors2 = from_lane_f(rs2, own[i]);
                     const float4 r = rec_at(pn[i]);
-                    const float4 XO = make_float4(r.x * z, r.y * z, r.x, r.y);
-                    const uint2 wm = make_uint2(__float_as_uint(r.z), __float_as_uint(r.w));
                     // both reads issue before the test's branches (as in the
                     // per-grain walk)
-                    asm volatile("" ::"v"(XO.z), "v"(XO.w), "v"(wm.x), "v"(wm.y));
-                    test(lane < total - s0, XO, wm, obx, oby, ocx, ocy, ors2);
+                    asm volatile("" ::"v"(r.x), "v"(r.y), "v"(r.z), "v"(r.w));
+                    test(true, r, gr.x, gr.y, gr.z, gr.w, ors2);
                 }
             }
         }
@@ -1342,16 +1373,15 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 for (int i0 = 0; i0 < ulen; i0 += 128) {
                     const int ia = i0 + lane, ib = ia + 64;
                     const int pa = position(up0 + ia), pb = position(up0 + ib);
-                    const float4 XOa = XO_at(pa), XOb = XO_at(pb);
-                    const uint2 wa = word_bit(pa), wb = word_bit(pb);
+                    const float4 ra = srec[pa], rb = srec[pb];
                     // all the reads issue together, ahead of the tests (an
                     // empty asm that takes the values as operands keeps the
-                    // compiler from moving the offset and index reads into
-                    // the branches after the dependent ones)
-                    asm volatile("" ::"v"(XOa.z), "v"(XOa.w), "v"(wa.x), "v"(wa.y), "v"(XOb.z), "v"(XOb.w),
-                                 "v"(wb.x), "v"(wb.y));
-                    test(ia < ulen, XOa, wa, ubx, uby, ucx, ucy, urs2);
-                    if (i0 + 64 < ulen) test(ib < ulen, XOb, wb, ubx, uby, ucx, ucy, urs2);
+                    // compiler from moving the record reads into the
+                    // branches after the dependent ones)
+                    asm volatile("" ::"v"(ra.x), "v"(ra.y), "v"(ra.z), "v"(ra.w), "v"(rb.x), "v"(rb.y),
+                                 "v"(rb.z), "v"(rb.w));
+                    test(ia < ulen, ra, ubx, uby, ucx, ucy, urs2);
+                    if (i0 + 64 < ulen) test(ib < ulen, rb, ubx, uby, ucx, ucy, urs2);
                 }
                 continue;
             }
