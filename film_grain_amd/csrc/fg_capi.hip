@@ -565,7 +565,7 @@ hipError_t tile_order_sort(fg_ctx *ctx, int n, const TileOrder &t, hipStream_t s
 
 // Candidate lists of the splat's direct test for one sample table: classes
 // of 1/64 output pixel up to the narrow limit rz < 0.5 and the plane's
-// largest rz, 32 x 32 cells.
+// largest rz, 32 x 32 cells, each with its row-bucket counts.
 struct ListPlan {
     int q = 32, classes = 0, ew = 0;
     // Lists for every table of at most 2048 samples (beyond, the room of
@@ -573,10 +573,11 @@ struct ListPlan {
     // well: the splat packs the lists of consecutive grains into its 64-lane
     // steps (a 1080p RGB frame at N = 256, zoom 1, r = 0.1: 16.3 ms with
     // lists, 26.9 ms walking x-runs).
-    size_t bytes(int n) const  // fill counters | (start, length) cells | entries
+    size_t bytes(int n) const  // fill counters | (start, length) cells | row-bucket counts | entries
     {
         if (classes == 0 || n < 1 || n > 2048) return 0;
-        const size_t b = 256 + (size_t)classes * q * q * 8 + fg::splat_lists_entries(q, classes, ew, n) * 2;
+        const size_t b = 256 + (size_t)classes * q * q * (8 + 2 * fg::kRowBuckets) +
+                         fg::splat_lists_entries(q, classes, ew, n) * 2;
         return (b + 255) & ~(size_t)255;
     }
 };
@@ -595,10 +596,12 @@ hipError_t build_lists(const ListPlan &lp, fg::SampleTable &t, unsigned char *at
 {
     uint32_t *fill = reinterpret_cast<uint32_t *>(at);
     uint2 *cs = reinterpret_cast<uint2 *>(at + 256);
-    uint16_t *cl = reinterpret_cast<uint16_t *>(cs + (size_t)lp.classes * lp.q * lp.q);
+    uint16_t *yx = reinterpret_cast<uint16_t *>(cs + (size_t)lp.classes * lp.q * lp.q);
+    uint16_t *cl = yx + (size_t)lp.classes * lp.q * lp.q * fg::kRowBuckets;
     t.cl_cell = cs;
+    t.cl_yidx = yx;
     t.cl_list = cl;
-    return fg::launch_splat_lists(t, lp.q, lp.classes, lp.ew, fill, cs, cl, status, s);
+    return fg::launch_splat_lists(t, lp.q, lp.classes, lp.ew, fill, cs, yx, cl, status, s);
 }
 
 // Where a render's scratch comes from: the context's grow-only buffers
@@ -1711,7 +1714,8 @@ int batch_create(fg_ctx *ctx, const fg_plane_desc *descs, int n, const float *co
             if ((rc = batch_alloc(b, (void **)&b->list_mem[p], lb, what)) != FG_OK) return rc;
             const ListPlan &lp = b->list_plans[p];
             bp.st.cl_cell = reinterpret_cast<const uint2 *>(b->list_mem[p] + 256);
-            bp.st.cl_list = reinterpret_cast<const uint16_t *>(bp.st.cl_cell + (size_t)lp.classes * lp.q * lp.q);
+            bp.st.cl_yidx = reinterpret_cast<const uint16_t *>(bp.st.cl_cell + (size_t)lp.classes * lp.q * lp.q);
+            bp.st.cl_list = bp.st.cl_yidx + (size_t)lp.classes * lp.q * lp.q * fg::kRowBuckets;
             bp.a.cl_q = lp.q;
             bp.a.cl_classes = lp.classes;
             bp.a.cl_inv_dz = (float)(2 * lp.q);

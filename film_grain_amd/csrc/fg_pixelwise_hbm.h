@@ -37,11 +37,18 @@ struct SampleTable {
     float safe_r;          // a grain of radius <= safe_r that passes the disk test of a point
                            // lies in that point's cell window
     // the splat's candidate lists over this order (launch_splat_lists), or
-    // nullptr: (start, length) per (radius class, cell), and the entries
-    // (sorted position * 16)
+    // nullptr: (start, length) per (radius class, cell), the entries (sorted
+    // position * 16) of each list in order of the scaled y offset, and per
+    // cell kRowBuckets counts: entry b - 1 is the list's entries in row
+    // buckets below b (row_bucket; entry kRowBuckets - 1 is its length)
     const uint2 *cl_cell;
     const uint16_t *cl_list;
+    const uint16_t *cl_yidx;
 };
+
+// Row buckets of the candidate lists: 16 intervals of the scaled y offsets
+// X.y of a table, from its extent.
+constexpr int kRowBuckets = 16;
 
 // Bytes of the device buffer behind a SampleTable for n samples.
 size_t sample_table_bytes(int n, int qbuckets);
@@ -56,14 +63,15 @@ hipError_t launch_sample_prefix(const SampleTable &full, int n, int qbuckets, vo
 // Candidate lists of the HBM splat (PixelArgs::cl_*): q * q cells per class,
 // class k of radius (k + 1) / (2 q).  e is an integer >= (2 st.eps + 2^-11)
 // q; list needs splat_lists_entries(q, classes, e, st.n) entries, cell
-// classes * q * q (start, length) pairs, fill classes counters (cleared here).
+// classes * q * q (start, length) pairs, yidx classes * q * q * kRowBuckets
+// counts, fill classes counters (cleared here).
 // A class beyond its room sets kStatusCapacity in *status.
 size_t splat_lists_entries(int q, int classes, int e, int n);
 // Writes t to device memory at dst (the single-plane splat reads its table
 // from PixelArgs::level_tables).
 hipError_t launch_store_table(const SampleTable &t, SampleTable *dst, hipStream_t s);
 hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
-                              uint16_t *list, int *status, hipStream_t s);
+                              uint16_t *yidx, uint16_t *list, int *status, hipStream_t s);
 
 // Also sets kStatusLambda in *a.status for lambda values outside the accepted range.
 hipError_t launch_threshold(const PixelArgs &a, double *T, hipStream_t s);

@@ -698,6 +698,9 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
 // (2 eps + 2^-11) q, and class k needs at most n * span^2 list entries.
 // An entry is its sample's sorted position p times 16, the byte offset of
 // the sample's record in the splat's LDS (n <= 2048 keeps it in 16 bits).
+// A list holds its entries in order of the scaled y offset X.y (ties by p),
+// with per-cell counts by row bucket (row_bucket), so that the splat walks
+// only the entries whose target row can lie in its tile.
 // Blocks of 64 cells of one class, four threads per cell; both passes test
 // every sample, and each block reserves its cells' room in the class's
 // region with one atomic.
@@ -713,19 +716,43 @@ __host__ __device__ inline size_t splat_list_base(int k, int q, int e, int n)
     return b;
 }
 
+// kRowBuckets intervals over a table's scaled y offsets [ext.z, ext.w]:
+// bucket(y) = floor((y - ext.z) * inv_w) clamped to [0, kRowBuckets - 1].
+// It is monotone in y, so the samples with y in [lo, hi] lie in buckets
+// bucket(lo)..bucket(hi).  The list builder and the splat compute it with
+// the same code from the same extent, so they agree on every bucket.
+struct RowBuckets {
+    float y0, inv_w;
+};
+__device__ __forceinline__ RowBuckets row_buckets(const float4 &ext)
+{
+    RowBuckets r;
+    r.y0 = ext.z;
+    r.inv_w = (float)kRowBuckets / ((ext.w - ext.z) + 0.015625f);
+    return r;
+}
+__device__ __forceinline__ int row_bucket(const RowBuckets &r, float y)
+{
+    return min(max((int)floorf((y - r.y0) * r.inv_w), 0), kRowBuckets - 1);
+}
+
 struct ListsShared {
-    float2 f[2048];
+    float2 f[2048];       // frac(X) of sorted position p
+    float y[2048];        // X.y of sorted position p
+    uint16_t ord[2048];   // the sorted positions in (X.y, p) order
     uint32_t scan[256];
+    int jb[kRowBuckets + 1];  // jb[b]: first index of that order in bucket >= b
     uint32_t base;
     bool fits;
 };
 
 __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, int e, uint32_t *fill, uint2 *cell,
-                                                 uint16_t *list, int *status, int blk, ListsShared &sh)
+                                                 uint16_t *yidx, uint16_t *list, int *status, int blk,
+                                                 ListsShared &sh)
 {
     // 64 cells per block, each split over 4 adjacent threads that take the
-    // four quarters of the sample order: the scan over the block in thread
-    // order keeps every list in sorted order
+    // four quarters of the (X.y, p) order: the scan over the block in thread
+    // order keeps every list in that order
     float2 *s_f = sh.f;
     uint32_t *s_scan = sh.scan;
     uint32_t &s_base = sh.base;
@@ -737,8 +764,29 @@ __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, i
     for (int p = threadIdx.x; p < N; p += 256) {
         const float2 X = st.xy[p];
         s_f[p] = make_float2(X.x - floorf(X.x), X.y - floorf(X.y));
+        sh.y[p] = X.y;
     }
     __syncthreads();
+    // the (X.y, p) order by rank
+    for (int p = threadIdx.x; p < N; p += 256) {
+        const float y = sh.y[p];
+        int r = 0;
+        for (int j = 0; j < N; ++j) {
+            const float yj = sh.y[j];
+            r += (yj < y || (yj == y && j < p)) ? 1 : 0;
+        }
+        sh.ord[r] = (uint16_t)p;
+    }
+    __syncthreads();
+    // bucket boundaries: the buckets of the order are non-decreasing, and
+    // index i starts buckets (bucket(i - 1), bucket(i)] (bucket(-1) = 0,
+    // bucket(N) = kRowBuckets)
+    const RowBuckets rb = row_buckets(*st.ext);
+    for (int i = threadIdx.x; i <= N; i += 256) {
+        const int prev = i == 0 ? 0 : row_bucket(rb, sh.y[sh.ord[i - 1]]);
+        const int cur = i == N ? kRowBuckets : row_bucket(rb, sh.y[sh.ord[i]]);
+        for (int b = prev + 1; b <= cur; ++b) sh.jb[b] = i;
+    }
     const float m = 2.44140625e-4f;  // 2^-12
     const float R = (float)(k + 1) / (float)(2 * q);
     const float w = ((R + R) + (st.eps + st.eps)) + (1.0f / (float)q + (m + m));
@@ -753,7 +801,7 @@ __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, i
     };
     uint32_t n = 0;
     if (c < cells)
-        for (int p = p_lo; p < p_hi; ++p) n += member(p) ? 1u : 0u;
+        for (int i = p_lo; i < p_hi; ++i) n += member(sh.ord[i]) ? 1u : 0u;
     // the block's cells take one contiguous range of the class's room
     s_scan[threadIdx.x] = n;
     __syncthreads();
@@ -775,20 +823,40 @@ __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, i
     __syncthreads();
     if (c < cells && s_fits) {
         uint32_t pos = s_base + s_scan[threadIdx.x] - n;
+        // the cell's entries before this quarter
+        const int t0 = (int)threadIdx.x - quarter;
+        uint32_t rel = s_scan[threadIdx.x] - n - (t0 > 0 ? s_scan[t0 - 1] : 0u);
+        uint16_t *yx = yidx + ((size_t)k * cells + c) * kRowBuckets;
         if (quarter == 0) {
             const uint32_t len = s_scan[threadIdx.x + 3] - (s_scan[threadIdx.x] - n);
             cell[(size_t)k * cells + c] = make_uint2(pos, len);
+            // buckets that start past the last sample hold no entry
+            for (int b = 1; b <= kRowBuckets; ++b)
+                if (sh.jb[b] == N) yx[b - 1] = (uint16_t)len;
         }
-        for (int p = p_lo; p < p_hi; ++p)
-            if (member(p)) list[pos++] = (uint16_t)(p << 4);
+        // each other bucket count is written by the quarter holding the
+        // bucket's first index: the cell's entries before that index
+        int b = 1;
+        while (b <= kRowBuckets && sh.jb[b] < p_lo) ++b;
+        for (int i = p_lo; i < p_hi; ++i) {
+            while (b <= kRowBuckets && sh.jb[b] <= i) {
+                yx[b - 1] = (uint16_t)rel;
+                ++b;
+            }
+            const int p = sh.ord[i];
+            if (member(p)) {
+                list[pos++] = (uint16_t)(p << 4);
+                ++rel;
+            }
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void splat_lists_kernel(SampleTable st, int q, int e, uint32_t *fill, uint2 *cell,
-                                                          uint16_t *list, int *status)
+                                                          uint16_t *yidx, uint16_t *list, int *status)
 {
     __shared__ ListsShared sh;
-    splat_lists_body(st, q, e, fill, cell, list, status, blockIdx.x, sh);
+    splat_lists_body(st, q, e, fill, cell, yidx, list, status, blockIdx.x, sh);
 }
 
 // Batch sample tables: workgroup (x, p) sorts samples x * kBlock.. of plane p.
@@ -821,7 +889,8 @@ __global__ __launch_bounds__(256) void batch_splat_lists_kernel(BatchGen bg)
     const BatchPlane &bp = cplane(bg.planes, p);
     const int4 lp = cload(bg.list_plan + p);
     splat_lists_body(bp.st, lp.x, lp.z, cload(bg.list_fill + p), const_cast<uint2 *>(bp.st.cl_cell),
-                     const_cast<uint16_t *>(bp.st.cl_list), bp.a.status, (int)(blk - cload(bg.list_off + p)), sh);
+                     const_cast<uint16_t *>(bp.st.cl_yidx), const_cast<uint16_t *>(bp.st.cl_list), bp.a.status,
+                     (int)(blk - cload(bg.list_off + p)), sh);
 }
 
 __global__ void store_table_kernel(SampleTable t, SampleTable *dst)
@@ -841,15 +910,15 @@ size_t splat_lists_entries(int q, int classes, int e, int n)
 }
 
 hipError_t launch_splat_lists(const SampleTable &st, int q, int classes, int e, uint32_t *fill, uint2 *cell,
-                              uint16_t *list, int *status, hipStream_t s)
+                              uint16_t *yidx, uint16_t *list, int *status, hipStream_t s)
 {
     if (q < 1 || q * q > 65536 || classes < 1 || st.n > 2048) return hipErrorInvalidValue;
     // a kernel, not hipMemsetAsync: batches capture this into a graph
     hipError_t err = launch_fill_u32(fill, classes, 0u, s);
     if (err != hipSuccess) return err;
     const int parts = (q * q + 63) / 64;
-    hipLaunchKernelGGL(splat_lists_kernel, dim3(classes * parts), dim3(256), 0, s, st, q, e, fill, cell, list,
-                       status);
+    hipLaunchKernelGGL(splat_lists_kernel, dim3(classes * parts), dim3(256), 0, s, st, q, e, fill, cell, yidx,
+                       list, status);
     return hipGetLastError();
 }
 
@@ -1057,6 +1126,7 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
         return make_uint2(__float_as_uint(r.z), __float_as_uint(r.w));
     };
     const float4 ext = *st.ext;
+    const RowBuckets rbk = row_buckets(ext);
     // the tile's 64-grain chunks go to its waves on demand (the first NW
     // statically), so that a wave that drew sparse chunks takes more
     __shared__ int s_next_chunk;
@@ -1222,9 +1292,19 @@ __device__ __forceinline__ void splat_tile(const PixelArgs &a, const PixGeom &ge
                 const float fx = ncx - floorf(ncx), fy = ncy - floorf(ncy);
                 const float qf = (float)a.cl_q;
                 const int qa = min((int)(fx * qf), a.cl_q - 1), qb = min((int)(fy * qf), a.cl_q - 1);
-                const uint2 se = st.cl_cell[(size_t)k * (a.cl_q * a.cl_q) + qb * a.cl_q + qa];
-                p0 = (int)se.x;
-                len = (int)se.y;
+                const size_t cix = (size_t)k * (a.cl_q * a.cl_q) + qb * a.cl_q + qa;
+                const uint2 se = st.cl_cell[cix];
+                // Only the list's entries whose target row can be a tile
+                // row: rint(by + X.y) in [Y0, Y1 - 1] needs X.y in
+                // [Y0 - 0.5 - by, Y1 - 0.5 - by].  Widened by 2^-4 for the
+                // rounding of the target and of the bounds, the row buckets
+                // of that interval are one run of the list's entries.
+                const int b0 = row_bucket(rbk, (t.fy0 - 0.5f - by) - 0.0625f);
+                const int b1 = row_bucket(rbk, (t.fy1 + 0.5f - by) + 0.0625f);
+                const uint16_t *yx = st.cl_yidx + cix * kRowBuckets;
+                const int e0 = b0 > 0 ? (int)yx[b0 - 1] : 0;
+                p0 = (int)se.x + e0;
+                len = (int)yx[b1] - e0;
                 listed = true;
             }
         }

@@ -241,3 +241,55 @@ def test_splat_list_contains_every_candidate(N, z, sd, out_side):
     for k in range(32):
         total = sum(int(splat_list_members(fx, fy, k, c % q, c // q, q, eps).sum()) for c in range(q * q))
         assert total <= N * splat_list_span(k, q, e) ** 2, (k, total)
+
+
+def row_bucket(X_y, ext_z, ext_w, y):
+    """Row bucket of y over the scaled y offsets (row_bucket in
+    fg_pixelwise_hbm.hip), in f32."""
+    inv_w = f32(f32(16) / f32(f32(ext_w - ext_z) + f32(0.015625)))
+    b = np.floor((np.asarray(y, dtype=f32) - ext_z).astype(f32) * inv_w)
+    return np.clip(b, 0, 15).astype(np.int64)
+
+
+@pytest.mark.parametrize("N,z,sd,out_side,tile", [(256, 2.0, 0.8, 3840, 32), (100, 1.0, 0.4, 1920, 16),
+                                                   (513, 0.7, 1.5, 1024, 64), (2048, 3.0, 0.8, 4096, 32)])
+def test_splat_row_trim_keeps_every_tile_row(N, z, sd, out_side, tile):
+    """The splat walks, of a grain's candidate list, only the row buckets of
+    [Y0 - 0.5 - by, Y1 - 0.5 - by] widened by 2^-4, the list being ordered by
+    X.y with per-bucket counts.  Every sample whose target row rint(fma(oy,
+    z, by)) lies in the tile (the test's tile check, evaluated with the
+    fused product) must have its bucket in that range; and the range drops
+    the entries of rows far from the tile."""
+    rng = np.random.default_rng(7 * N + tile)
+    zf = f32(z)
+    o = rng.normal(0, sd, size=(N, 2)).astype(f32)
+    Xy = (o[:, 1] * zf).astype(f32)
+    ext_z, ext_w = Xy.min(), Xy.max()
+    bucket = row_bucket(Xy, ext_z, ext_w, Xy)
+    order = np.lexsort((np.arange(N), Xy))
+    # bucket counts of the (X.y, p) order are non-decreasing: the entries
+    # of buckets b0..b1 are one run of the order
+    assert np.all(np.diff(bucket[order]) >= 0)
+    kept = total = 0
+    tiles = out_side // tile
+    for i in range(400):
+        Y0 = int(rng.integers(0, tiles)) * tile
+        Y1 = min(Y0 + tile, out_side)
+        # grain centres over the rows whose samples can reach the tile, and
+        # targets of sample 0 near the tile's edge rows
+        cy_in = rng.uniform((Y0 - 12) / z, (Y1 + 12) / z)
+        if i % 2:
+            t = f32(Y0 - 0.5 + rng.choice([0.0, Y1 - Y0]) + rng.uniform(-1e-4, 1e-4))
+            cy_in = float((t - Xy[0] + f32(0.5)) / zf)
+        by = f32(f32(f32(cy_in) * zf) - f32(0.5))
+        target = (o[:, 1].astype(np.float64) * float(zf) + float(by)).astype(f32)
+        row = np.rint(target)
+        in_tile = (row >= Y0) & (row <= Y1 - 1)
+        fy0, fy1 = f32(Y0), f32(Y1 - 1)
+        b0 = row_bucket(Xy, ext_z, ext_w, f32(f32(f32(fy0 - f32(0.5)) - by) - f32(0.0625)))
+        b1 = row_bucket(Xy, ext_z, ext_w, f32(f32(f32(fy1 + f32(0.5)) - by) + f32(0.0625)))
+        walked = (bucket >= b0) & (bucket <= b1)
+        assert not np.any(in_tile & ~walked), (N, z, i, Y0, float(by))
+        kept += int(walked.sum())
+        total += N
+    assert kept < total
