@@ -31,6 +31,8 @@ struct SampleTable {
     const uint16_t *qidx;  // qbuckets + 1 entries: first position whose fraction >= q / qbuckets
     const float4 *ext;     // (min X, max X, min Y, max Y) of the scaled offsets
     const float *key;      // the fractions, same order
+    const uint16_t *yord;  // the sorted positions in (X.y, position) order
+    const uint16_t *ybk;   // kRowBuckets + 1 entries: first index of yord whose row bucket is >= b
     int n;                 // samples in the table
     int qbuckets;          // power of two
     float eps;             // widening of the fraction interval (covers f32 rounding of the box)

@@ -552,10 +552,38 @@ __global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs,
     sample_sort_body(offs, n, zoom, t, blockIdx.x, s_key);
 }
 
-// Bucket index (first sorted position with fraction >= q / Q, by binary
-// search) and the extents of the scaled offsets; one workgroup.
-__device__ __forceinline__ void sample_index_body(int n, const SampleTable &t, float4 *s_r)
+// kRowBuckets intervals over a table's scaled y offsets [ext.z, ext.w]:
+// bucket(y) = floor((y - ext.z) * inv_w) clamped to [0, kRowBuckets - 1].
+// It is monotone in y, so the samples with y in [lo, hi] lie in buckets
+// bucket(lo)..bucket(hi).  The list builder and the splat compute it with
+// the same code from the same extent, so they agree on every bucket.
+struct RowBuckets {
+    float y0, inv_w;
+};
+__device__ __forceinline__ RowBuckets row_buckets(const float4 &ext)
 {
+    RowBuckets r;
+    r.y0 = ext.z;
+    r.inv_w = (float)kRowBuckets / ((ext.w - ext.z) + 0.015625f);
+    return r;
+}
+__device__ __forceinline__ int row_bucket(const RowBuckets &r, float y)
+{
+    return min(max((int)floorf((y - r.y0) * r.inv_w), 0), kRowBuckets - 1);
+}
+
+// Bucket index (first sorted position with fraction >= q / Q, by binary
+// search), the extents of the scaled offsets, and the (X.y, position) order
+// with its row-bucket starts; one workgroup.
+struct IndexShared {
+    float4 r[kBlock];
+    float y[2048];
+    uint16_t ord[2048];
+};
+
+__device__ __forceinline__ void sample_index_body(int n, const SampleTable &t, IndexShared &sh)
+{
+    float4 *s_r = sh.r;
     const int Q = t.qbuckets;
     for (int q = threadIdx.x; q <= Q; q += kBlock) {
         const float v = (float)q / (float)Q;  // exact: Q is a power of two
@@ -592,12 +620,39 @@ __device__ __forceinline__ void sample_index_body(int n, const SampleTable &t, f
         __syncthreads();
     }
     if (threadIdx.x == 0) *const_cast<float4 *>(t.ext) = s_r[0];
+    // the (X.y, position) order by rank (n <= 2048 for these; larger tables
+    // have no row order and leave it unused)
+    if (n > 2048) return;
+    const RowBuckets rb = row_buckets(s_r[0]);
+    for (int k = threadIdx.x; k < n; k += kBlock) sh.y[k] = t.xy[k].y;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        const float y = sh.y[k];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const float yj = sh.y[j];
+            r += (yj < y || (yj == y && j < k)) ? 1 : 0;
+        }
+        sh.ord[r] = (uint16_t)k;
+    }
+    __syncthreads();
+    uint16_t *yord = const_cast<uint16_t *>(t.yord), *ybk = const_cast<uint16_t *>(t.ybk);
+    for (int i = threadIdx.x; i < n; i += kBlock) yord[i] = sh.ord[i];
+    // the row buckets of the order are non-decreasing; index i starts
+    // buckets (bucket(i - 1), bucket(i)], with bucket(-1) = 0 and
+    // bucket(n) = kRowBuckets
+    if (threadIdx.x == 0) ybk[0] = 0;
+    for (int i = threadIdx.x; i <= n; i += kBlock) {
+        const int prev = i == 0 ? 0 : row_bucket(rb, sh.y[sh.ord[i - 1]]);
+        const int cur = i == n ? kRowBuckets : row_bucket(rb, sh.y[sh.ord[i]]);
+        for (int b = prev + 1; b <= cur; ++b) ybk[b] = (uint16_t)i;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void sample_index_kernel(int n, SampleTable t)
 {
-    __shared__ float4 s_r[kBlock];
-    sample_index_body(n, t, s_r);
+    __shared__ IndexShared sh;
+    sample_index_body(n, t, sh);
 }
 
 // The table of the first n samples from the table of all of them: the full
@@ -635,8 +690,8 @@ __host__ __device__ static inline size_t align16(size_t b) { return (b + 15) & ~
 
 size_t sample_table_bytes(int n, int qbuckets)
 {
-    return 16 + align16((size_t)n * 8) * 2 + align16((size_t)n * 4) + align16((size_t)n * 2) +
-           align16((size_t)(qbuckets + 1) * 2);
+    return 16 + align16((size_t)n * 8) * 2 + align16((size_t)n * 4) + align16((size_t)n * 2) * 2 +
+           align16((size_t)(qbuckets + 1) * 2) + align16((size_t)(kRowBuckets + 1) * 2);
 }
 
 hipError_t launch_sample_table(const float2 *offs, int n, float zoom, int qbuckets, void *buf, SampleTable *t,
@@ -677,6 +732,10 @@ void sample_table_layout(void *buf, int n, int qbuckets, SampleTable *t)
     t->k = reinterpret_cast<uint16_t *>(p);
     p += align16((size_t)n * 2);
     t->qidx = reinterpret_cast<uint16_t *>(p);
+    p += align16((size_t)(qbuckets + 1) * 2);
+    t->yord = reinterpret_cast<uint16_t *>(p);
+    p += align16((size_t)n * 2);
+    t->ybk = reinterpret_cast<uint16_t *>(p);
     t->n = n;
     t->qbuckets = qbuckets;
 }
@@ -716,32 +775,11 @@ __host__ __device__ inline size_t splat_list_base(int k, int q, int e, int n)
     return b;
 }
 
-// kRowBuckets intervals over a table's scaled y offsets [ext.z, ext.w]:
-// bucket(y) = floor((y - ext.z) * inv_w) clamped to [0, kRowBuckets - 1].
-// It is monotone in y, so the samples with y in [lo, hi] lie in buckets
-// bucket(lo)..bucket(hi).  The list builder and the splat compute it with
-// the same code from the same extent, so they agree on every bucket.
-struct RowBuckets {
-    float y0, inv_w;
-};
-__device__ __forceinline__ RowBuckets row_buckets(const float4 &ext)
-{
-    RowBuckets r;
-    r.y0 = ext.z;
-    r.inv_w = (float)kRowBuckets / ((ext.w - ext.z) + 0.015625f);
-    return r;
-}
-__device__ __forceinline__ int row_bucket(const RowBuckets &r, float y)
-{
-    return min(max((int)floorf((y - r.y0) * r.inv_w), 0), kRowBuckets - 1);
-}
-
 struct ListsShared {
     float2 f[2048];       // frac(X) of sorted position p
-    float y[2048];        // X.y of sorted position p
-    uint16_t ord[2048];   // the sorted positions in (X.y, p) order
+    uint16_t ord[2048];   // the table's (X.y, position) order
+    int jb[kRowBuckets + 1];  // jb[b]: first index of that order in row bucket >= b
     uint32_t scan[256];
-    int jb[kRowBuckets + 1];  // jb[b]: first index of that order in bucket >= b
     uint32_t base;
     bool fits;
 };
@@ -764,29 +802,10 @@ __device__ __forceinline__ void splat_lists_body(const SampleTable &st, int q, i
     for (int p = threadIdx.x; p < N; p += 256) {
         const float2 X = st.xy[p];
         s_f[p] = make_float2(X.x - floorf(X.x), X.y - floorf(X.y));
-        sh.y[p] = X.y;
+        sh.ord[p] = st.yord[p];
     }
+    if (threadIdx.x <= kRowBuckets) sh.jb[threadIdx.x] = st.ybk[threadIdx.x];
     __syncthreads();
-    // the (X.y, p) order by rank
-    for (int p = threadIdx.x; p < N; p += 256) {
-        const float y = sh.y[p];
-        int r = 0;
-        for (int j = 0; j < N; ++j) {
-            const float yj = sh.y[j];
-            r += (yj < y || (yj == y && j < p)) ? 1 : 0;
-        }
-        sh.ord[r] = (uint16_t)p;
-    }
-    __syncthreads();
-    // bucket boundaries: the buckets of the order are non-decreasing, and
-    // index i starts buckets (bucket(i - 1), bucket(i)] (bucket(-1) = 0,
-    // bucket(N) = kRowBuckets)
-    const RowBuckets rb = row_buckets(*st.ext);
-    for (int i = threadIdx.x; i <= N; i += 256) {
-        const int prev = i == 0 ? 0 : row_bucket(rb, sh.y[sh.ord[i - 1]]);
-        const int cur = i == N ? kRowBuckets : row_bucket(rb, sh.y[sh.ord[i]]);
-        for (int b = prev + 1; b <= cur; ++b) sh.jb[b] = i;
-    }
     const float m = 2.44140625e-4f;  // 2^-12
     const float R = (float)(k + 1) / (float)(2 * q);
     const float w = ((R + R) + (st.eps + st.eps)) + (1.0f / (float)q + (m + m));
@@ -872,10 +891,10 @@ __global__ __launch_bounds__(kBlock) void batch_sample_sort_kernel(BatchGen bg)
 // counters cleared for batch_splat_lists_kernel.
 __global__ __launch_bounds__(kBlock) void batch_sample_index_kernel(BatchGen bg)
 {
-    __shared__ float4 s_r[kBlock];
+    __shared__ IndexShared sh;
     const int p = blockIdx.x;
     const BatchPlane &bp = cplane(bg.planes, p);
-    sample_index_body(bp.a.n_samples, bp.st, s_r);
+    sample_index_body(bp.a.n_samples, bp.st, sh);
     const int classes = cload(bg.list_plan + p).y;
     uint32_t *fill = cload(bg.list_fill + p);
     for (int k = threadIdx.x; k < classes; k += kBlock) fill[k] = 0u;
