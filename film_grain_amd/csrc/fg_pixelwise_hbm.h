@@ -32,7 +32,7 @@ struct SampleTable {
     const uint16_t *qidx;  // qbuckets + 1 entries: first position whose fraction >= q / qbuckets
     const float4 *ext;     // (min X, max X, min Y, max Y) of the scaled offsets
     const float *key;      // the fractions, same order
-    const uint16_t *yord;  // the sorted positions in (X.y, position) order
+    const uint16_t *yord;  // the sorted positions in (X.y, position) order (filled for n <= 2048)
     const uint16_t *ybk;   // kRowBuckets + 1 entries: first index of yord whose row bucket is >= b
     int n;                 // samples in the table
     int qbuckets;          // power of two
@@ -49,8 +49,8 @@ struct SampleTable {
     const uint16_t *cl_yidx;
 };
 
-// Row buckets of the candidate lists: 16 intervals of the scaled y offsets
-// X.y of a table, from its extent.
+// Row buckets of a table's (X.y, position) order and of its candidate
+// lists: 16 intervals of the scaled y offsets X.y, from their extent.
 constexpr int kRowBuckets = 16;
 
 // Bytes of the device buffer behind a SampleTable for n samples.
