@@ -555,8 +555,9 @@ __global__ __launch_bounds__(kBlock) void sample_sort_kernel(const float2 *offs,
 // kRowBuckets intervals over a table's scaled y offsets [ext.z, ext.w]:
 // bucket(y) = floor((y - ext.z) * inv_w) clamped to [0, kRowBuckets - 1].
 // It is monotone in y, so the samples with y in [lo, hi] lie in buckets
-// bucket(lo)..bucket(hi).  The list builder and the splat compute it with
-// the same code from the same extent, so they agree on every bucket.
+// bucket(lo)..bucket(hi).  sample_index (which orders the table) and the
+// splat compute it with the same code from the same extent; the splat's
+// bounds are widened by 2^-4 besides, far more than a last-bit difference.
 struct RowBuckets {
     float y0, inv_w;
 };
