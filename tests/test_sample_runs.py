@@ -293,3 +293,48 @@ def test_splat_row_trim_keeps_every_tile_row(N, z, sd, out_side, tile):
         kept += int(walked.sum())
         total += N
     assert kept < total
+
+
+def quarter_bucket_counts(member, jb, n):
+    """The row-bucket counts of one cell as splat_lists_kernel writes them
+    (fg_pixelwise_hbm.hip): four threads take the quarters of the (X.y,
+    position) order; a bucket count is written by the quarter holding the
+    bucket's first index, and buckets starting at n by quarter 0 with the
+    list's length.  member: the cell's membership over the order; jb[b]: the
+    first index of bucket >= b, b = 0..16."""
+    qn = (n + 3) // 4
+    counts = [member[q * qn:min(n, q * qn + qn)].sum() for q in range(4)]
+    yx = [None] * 16
+    length = int(sum(counts))
+    for b in range(1, 17):
+        if jb[b] == n:
+            yx[b - 1] = length
+    for q in range(4):
+        p_lo, p_hi = q * qn, min(n, q * qn + qn)
+        rel = int(sum(counts[:q]))
+        b = 1
+        while b <= 16 and jb[b] < p_lo:
+            b += 1
+        for i in range(p_lo, p_hi):
+            while b <= 16 and jb[b] <= i:
+                assert yx[b - 1] is None
+                yx[b - 1] = rel
+                b += 1
+            rel += int(member[i])
+    return yx
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 64, 129, 256, 2048])
+def test_list_bucket_counts_equal_brute_force(n):
+    """Every one of a cell's 16 counts is written exactly once and equals the
+    number of its members in buckets below b, for bucket layouts with empty
+    leading, inner and trailing buckets and quarters that are empty."""
+    rng = np.random.default_rng(n)
+    for trial in range(40):
+        # non-decreasing buckets over the order, some buckets skipped
+        bucket = np.sort(rng.integers(0, 16, size=n)) if trial % 4 else np.full(n, trial % 16)
+        member = rng.random(n) < rng.choice([0.0, 0.3, 1.0])
+        jb = [0] + [int(np.searchsorted(bucket, b, side="left")) for b in range(1, 16)] + [n]
+        yx = quarter_bucket_counts(member, jb, n)
+        want = [int(member[bucket < b].sum()) for b in range(1, 17)]
+        assert yx == want, (n, trial)
